@@ -1,0 +1,197 @@
+"""CPU oracle (test infrastructure only): ctypes binding of liboracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.  See oracle.cpp for what each
+function restates and which reference file it follows.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_ANALYZER = HERE / "_ref" / "bvh_analyzer"
+
+HIT_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("inst_id", "<u4"), ("prim_id", "<u4"), ("t", "<f4"),
+                      ("pad", "<u4", 3)])
+STATS_DTYPE = np.dtype([("internal_nodes", "<u4"), ("leaves", "<u4"), ("box_tests", "<u4"), ("tri_tests", "<u4")])
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} not built (make -C oracle)")
+        lib = ctypes.CDLL(str(LIB))
+        P = c_void_p
+        sig = {
+            "orc_mesh_bounds": ([P, c_uint32, P, c_uint32, P, P], None),
+            "orc_morton_codes": ([P, c_uint32, P, c_uint32, P], None),
+            "orc_morton3": ([c_float, c_float, c_float], c_uint32),
+            "orc_sort_pairs": ([P, P, c_uint32], None),
+            "orc_emit_hierarchy": ([P, P, c_uint32, P], None),
+            "orc_refit": ([P, c_uint32, P, c_uint32, P], None),
+            "orc_restructure": ([c_uint32, P], None),
+            "orc_blas_bytes": ([c_uint32], c_size_t),
+            "orc_build_blas": ([P, c_uint32, P, c_uint32, c_int, P], None),
+            "orc_tlas_bytes": ([c_uint32], c_size_t),
+            "orc_build_tlas": ([P, P, P, c_uint32, P], None),
+            "orc_trace": ([P, P, P, c_uint32, c_int, P, P, c_int], None),
+            "orc_brute_force": ([P, P, P, c_uint32, P, c_int], None),
+            "orc_triangle_fp64": ([P, P, P, P, c_double, POINTER(c_double), POINTER(c_double), POINTER(c_double)], c_int),
+            "orc_is_valid": ([P], c_int),
+            "orc_check_consistency": ([P], c_int),
+            "orc_sah": ([P], c_float),
+            "orc_height": ([P], c_uint32),
+            "orc_write_analyzer_dump": ([P, P, c_uint32, c_uint32, c_char_p, c_char_p, c_char_p], c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p) if a is not None else None
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def morton3(x, y, z) -> int:
+    return int(load().orc_morton3(x, y, z))
+
+
+def mesh_bounds(v, i):
+    v, i = _c(v, np.float32), _c(i, np.uint32)
+    lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    load().orc_mesh_bounds(_p(v), v.shape[1], _p(i), i.shape[0], _p(lo), _p(hi))
+    return lo, hi
+
+
+def morton_codes(v, i):
+    v, i = _c(v, np.float32), _c(i, np.uint32)
+    out = np.zeros(i.shape[0], np.uint32)
+    load().orc_morton_codes(_p(v), v.shape[1], _p(i), i.shape[0], _p(out))
+    return out
+
+
+def sort_pairs(keys, vals):
+    k, v = _c(keys, np.uint32).copy(), _c(vals, np.uint32).copy()
+    load().orc_sort_pairs(_p(k), _p(v), k.size)
+    return k, v
+
+
+def emit_hierarchy(sorted_codes, sorted_prims):
+    n = len(sorted_codes)
+    nodes = np.zeros((max(2 * n - 1, 0), 16), np.uint32)
+    load().orc_emit_hierarchy(_p(_c(sorted_codes, np.uint32)), _p(_c(sorted_prims, np.uint32)), n, _p(nodes))
+    return nodes
+
+
+def build_blas(v, i, restructure=False) -> np.ndarray:
+    v, i = _c(v, np.float32), _c(i, np.uint32)
+    n = i.shape[0]
+    buf = np.zeros(load().orc_blas_bytes(n), np.uint8)
+    load().orc_build_blas(_p(v), v.shape[1], _p(i), n, int(restructure), _p(buf))
+    return buf
+
+
+def restructure(buf: np.ndarray, tris: int) -> np.ndarray:
+    out = buf.copy()
+    load().orc_restructure(tris, _p(out[128:]))
+    return out
+
+
+def refit(buf: np.ndarray, v, i) -> np.ndarray:
+    v, i = _c(v, np.float32), _c(i, np.uint32)
+    out = buf.copy()
+    load().orc_refit(_p(v), v.shape[1], _p(i), i.shape[0], _p(out[128:]))
+    return out
+
+
+def build_tlas(xforms, blas_addrs, blas_bufs) -> np.ndarray:
+    n = len(blas_bufs)
+    x = _c(xforms, np.float32).reshape(n, 12) if n else np.zeros((1, 12), np.float32)
+    a = np.asarray(blas_addrs, dtype=np.uint64) if n else np.zeros(1, np.uint64)
+    ptrs = (c_void_p * max(n, 1))(*[b.ctypes.data for b in blas_bufs])
+    buf = np.zeros(load().orc_tlas_bytes(n), np.uint8)
+    load().orc_build_tlas(_p(x), _p(a), ptrs, n, _p(buf))
+    return buf
+
+
+def _blas_ptrs(accel: np.ndarray, blas_map):
+    """Array of host pointers, one per TLAS instance, from {device address: host buffer}."""
+    hdr = accel[:128].view(np.uint32)
+    if hdr[1] != 2:
+        return None, ()
+    n = int(hdr[2])
+    off = int(accel[40:48].view(np.uint64)[0])
+    addrs = accel[off:off + 8 * n].view(np.uint64)
+    bufs = [blas_map[int(a)] for a in addrs]
+    return (c_void_p * max(n, 1))(*[b.ctypes.data for b in bufs]), bufs
+
+
+def trace(accel: np.ndarray, rays, any_hit=False, blas_map=None, stats=False, threads=0):
+    rays = _c(rays, np.float32)
+    n = rays.shape[0]
+    hits = np.zeros(n, HIT_DTYPE)
+    st = np.zeros(n, STATS_DTYPE) if stats else None
+    ptrs, keep = _blas_ptrs(accel, blas_map or {})
+    load().orc_trace(_p(accel), ptrs, _p(rays), n, int(any_hit), _p(hits), _p(st), threads)
+    return (hits, st) if stats else hits
+
+
+def brute_force(accel: np.ndarray, rays, blas_map=None, threads=0):
+    rays = _c(rays, np.float32)
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    ptrs, keep = _blas_ptrs(accel, blas_map or {})
+    load().orc_brute_force(_p(accel), ptrs, _p(rays), rays.shape[0], _p(hits), threads)
+    return hits
+
+
+def triangle_fp64(ray, v0, v1, v2, eps=1e-9):
+    t, u, v = c_double(), c_double(), c_double()
+    ok = load().orc_triangle_fp64(_p(_c(ray, np.float32)), _p(_c(v0, np.float32)), _p(_c(v1, np.float32)),
+                                  _p(_c(v2, np.float32)), eps, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return bool(ok), t.value, u.value, v.value
+
+
+def is_valid(accel) -> bool:
+    return bool(load().orc_is_valid(_p(accel)))
+
+
+def check_consistency(accel) -> bool:
+    return bool(load().orc_check_consistency(_p(accel)))
+
+
+def sah(accel) -> float:
+    return float(load().orc_sah(_p(accel)))
+
+
+def height(accel) -> int:
+    return int(load().orc_height(_p(accel)))
+
+
+def write_analyzer_dump(accel, rays, width, height_, bvh_path, rays_path, cfg_path) -> None:
+    rc = load().orc_write_analyzer_dump(_p(accel), _p(_c(rays, np.float32)), width, height_, str(bvh_path).encode(),
+                                        str(rays_path).encode(), str(cfg_path).encode())
+    if rc:
+        raise RuntimeError("orc_write_analyzer_dump failed")
+
+
+def nodes_of(accel: np.ndarray) -> np.ndarray:
+    """Node array of a buffer as uint32 [count, 16] (float fields viewable via .view(np.float32))."""
+    hdr = accel[:128].view(np.uint32)
+    off = int(accel[16:24].view(np.uint64)[0])
+    count = int(hdr[3])
+    return accel[off:off + 64 * count].view(np.uint32).reshape(count, 16)

@@ -1,0 +1,223 @@
+"""Pythonic wrapper over the C ABI (see rr.py) for tests, benchmarks and the
+serving loop.  torch supplies device memory and the stream; every operation
+goes through the rr* entry points of libradeonrays.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from contextlib import contextmanager
+from ctypes import byref, c_size_t, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import rr
+
+_QUERY = {"closest": rr.RR_INTERSECT_QUERY_CLOSEST, "any": rr.RR_INTERSECT_QUERY_ANY}
+_OUTPUT = {"full": rr.RR_INTERSECT_QUERY_OUTPUT_FULL_HIT, "instance": rr.RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID,
+           "full_t": rr.RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T}
+_HIT_BYTES = {"full": 16, "instance": 4, "full_t": 32}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class Accel:
+    """A built BLAS or TLAS: its result buffer (uint8 tensor) and kind."""
+    buffer: "object"
+    kind: str              # "blas" or "tlas"
+    prim_count: int
+    children: tuple = ()   # BLASes a TLAS refers to (kept alive with it)
+
+    @property
+    def address(self) -> int:
+        return self.buffer.data_ptr()
+
+
+class Context:
+    """An RRContext on one HIP device whose submissions run on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        torch = _torch()
+        self.lib = rr.load()
+        self.device = device
+        self.torch_device = torch.device("cuda", device)
+        with torch.cuda.device(device):
+            self.stream = torch.cuda.current_stream(device)
+        ctx = c_void_p()
+        rr.check(self.lib.rrCreateContextHip(rr.RR_API_VERSION, device, c_void_p(self.stream.cuda_stream), byref(ctx)),
+                 "rrCreateContextHip")
+        self.ctx = ctx
+        cs = c_void_p()
+        rr.check(self.lib.rrGetCommandStreamFromHipStream(self.ctx, c_void_p(self.stream.cuda_stream), byref(cs)),
+                 "rrGetCommandStreamFromHipStream")
+        self.cs = cs  # immediate stream: rrCmd* enqueue on torch's stream
+
+    # -- handles ------------------------------------------------------------
+    @contextmanager
+    def ptrs(self, *addrs):
+        """RRDevicePtr handles for device addresses (None stays NULL), released on exit."""
+        handles = []
+        try:
+            for a in addrs:
+                if a is None:
+                    handles.append(None)
+                    continue
+                p = c_void_p()
+                rr.check(self.lib.rrGetDevicePtrFromHipPtr(self.ctx, c_void_p(int(a)), 0, byref(p)),
+                         "rrGetDevicePtrFromHipPtr")
+                handles.append(p)
+            yield handles
+        finally:
+            for p in handles:
+                if p is not None:
+                    self.lib.rrReleaseDevicePtr(self.ctx, p)
+
+    def empty(self, nbytes: int):
+        torch = _torch()
+        return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=self.torch_device)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.rrReleaseExternalCommandStream(self.ctx, self.cs)
+            self.lib.rrDestroyContext(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- builds ---------------------------------------------------------------
+    @staticmethod
+    def _mesh(vertices, indices, vptr, iptr):
+        v_stride = vertices.stride(0) * vertices.element_size()
+        prim = rr.RRTriangleMeshPrimitive(vptr, int(vertices.shape[0]), int(v_stride), iptr, int(indices.shape[0]),
+                                          rr.RR_INDEX_TYPE_UINT16 if indices.element_size() == 2 else rr.RR_INDEX_TYPE_UINT32)
+        inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, 1, ctypes.cast(ctypes.pointer(prim), c_void_p))
+        return prim, inp
+
+    @staticmethod
+    def _options(restructure: bool):
+        return rr.RRBuildOptions(0 if restructure else rr.RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD, None)
+
+    def geometry_requirements(self, tri_count: int, vertex_count: int = 3, restructure: bool = False):
+        prim = rr.RRTriangleMeshPrimitive(c_void_p(1), vertex_count, 12, c_void_p(1), tri_count, rr.RR_INDEX_TYPE_UINT32)
+        inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, 1, ctypes.cast(ctypes.pointer(prim), c_void_p))
+        req = rr.RRMemoryRequirements()
+        opts = self._options(restructure)
+        rr.check(self.lib.rrGetGeometryBuildMemoryRequirements(self.ctx, byref(inp), byref(opts), byref(req)),
+                 "rrGetGeometryBuildMemoryRequirements")
+        return req
+
+    def build_geometry(self, vertices, indices, restructure: bool = False, scratch=None, result=None) -> Accel:
+        """LBVH (+ treelet restructuring) of a mesh: vertices float32 [V, >=3], indices int32/uint16 [T, 3]."""
+        torch = _torch()
+        assert vertices.is_cuda and indices.is_cuda and vertices.dtype == torch.float32
+        indices = indices.contiguous()
+        tris = int(indices.shape[0])
+        req = self.geometry_requirements(tris, max(int(vertices.shape[0]), 1), restructure)
+        result = result if result is not None else self.empty(req.result_buffer_size)
+        scratch = scratch if scratch is not None else self.empty(req.temporary_build_buffer_size)
+        with self.ptrs(vertices.data_ptr(), indices.data_ptr(), scratch.data_ptr(), result.data_ptr()) as (v, i, s, r):
+            prim, inp = self._mesh(vertices, indices, v, i)
+            opts = self._options(restructure)
+            rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_BUILD, byref(inp), byref(opts), s, r,
+                                                 self.cs), "rrCmdBuildGeometry")
+        return Accel(result, "blas", tris)
+
+    def update_geometry(self, accel: Accel, vertices, indices) -> None:
+        """Refit `accel` to moved vertices (same triangles)."""
+        indices = indices.contiguous()
+        with self.ptrs(vertices.data_ptr(), indices.data_ptr(), accel.address) as (v, i, r):
+            prim, inp = self._mesh(vertices, indices, v, i)
+            rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_UPDATE, byref(inp), None, None, r,
+                                                 self.cs), "rrCmdBuildGeometry(update)")
+
+    def scene_requirements(self, count: int):
+        inp = rr.RRSceneBuildInput(None, count)
+        req = rr.RRMemoryRequirements()
+        rr.check(self.lib.rrGetSceneBuildMemoryRequirements(self.ctx, byref(inp), None, byref(req)),
+                 "rrGetSceneBuildMemoryRequirements")
+        return req
+
+    def build_scene(self, blases, transforms) -> Accel:
+        """TLAS over instances: blases[i] (Accel) placed by transforms[i] (3x4 row-major)."""
+        transforms = np.asarray(transforms, dtype=np.float32).reshape(len(blases), 3, 4)
+        n = len(blases)
+        req = self.scene_requirements(n)
+        result = self.empty(req.result_buffer_size)
+        scratch = self.empty(req.temporary_build_buffer_size)
+        with self.ptrs(*[b.address for b in blases], scratch.data_ptr(), result.data_ptr()) as hs:
+            geo, s, r = hs[:n], hs[n], hs[n + 1]
+            arr = (rr.RRInstance * max(n, 1))()
+            for k in range(n):
+                arr[k].geometry = geo[k]
+                for a in range(3):
+                    for b in range(4):
+                        arr[k].transform[a][b] = float(transforms[k, a, b])
+            inp = rr.RRSceneBuildInput(arr, n)
+            rr.check(self.lib.rrCmdBuildScene(self.ctx, byref(inp), None, s, r, self.cs), "rrCmdBuildScene")
+        return Accel(result, "tlas", n, tuple(blases))
+
+    # -- queries --------------------------------------------------------------
+    def trace_scratch(self, ray_count: int):
+        size = c_size_t()
+        rr.check(self.lib.rrGetTraceMemoryRequirements(self.ctx, max(ray_count, 1), byref(size)),
+                 "rrGetTraceMemoryRequirements")
+        return self.empty(size.value)
+
+    def intersect(self, accel: Accel, rays, query: str = "closest", output: str = "full", indirect=None, hits=None,
+                  scratch=None):
+        """rrCmdIntersect: rays float32 [R, 8] on the device; returns the hit buffer (uint8 tensor)."""
+        n = int(rays.shape[0])
+        hits = hits if hits is not None else self.empty(n * _HIT_BYTES[output])
+        scratch = scratch if scratch is not None else self.trace_scratch(n)
+        ind = indirect.data_ptr() if indirect is not None else None
+        with self.ptrs(accel.address, rays.data_ptr(), ind, hits.data_ptr(), scratch.data_ptr()) as (a, r, c, h, s):
+            rr.check(self.lib.rrCmdIntersect(self.ctx, a, _QUERY[query], r, n, c, _OUTPUT[output], h, s, self.cs),
+                     "rrCmdIntersect")
+        return hits
+
+    def intersect_with_stats(self, accel: Accel, rays, query: str = "closest"):
+        torch = _torch()
+        n = int(rays.shape[0])
+        hits = self.empty(n * 32)
+        stats = torch.empty((n, 4), dtype=torch.int32, device=self.torch_device)
+        scratch = self.trace_scratch(n)
+        with self.ptrs(accel.address, rays.data_ptr(), hits.data_ptr(), scratch.data_ptr(), stats.data_ptr()) as (
+                a, r, h, s, st):
+            rr.check(self.lib.rrxCmdIntersectWithStats(self.ctx, a, _QUERY[query], r, n, h, s, st, self.cs),
+                     "rrxCmdIntersectWithStats")
+        return hits, stats
+
+    def sort_pairs(self, keys, values, key_bits: int = 32, tmp=None, scratch=None):
+        """Stable radix sort of uint32 (stored as int32) key/value tensors in place."""
+        torch = _torch()
+        n = int(keys.numel())
+        size = c_size_t()
+        rr.check(self.lib.rrxGetSortMemoryRequirements(self.ctx, n, byref(size)), "rrxGetSortMemoryRequirements")
+        kt, vt = tmp if tmp is not None else (torch.empty_like(keys), torch.empty_like(values))
+        scratch = scratch if scratch is not None else self.empty(size.value)
+        with self.ptrs(keys.data_ptr(), values.data_ptr(), kt.data_ptr(), vt.data_ptr(), scratch.data_ptr()) as (
+                k, v, a, b, s):
+            rr.check(self.lib.rrxCmdSortPairs(self.ctx, k, v, a, b, n, key_bits, s, self.cs), "rrxCmdSortPairs")
+
+
+# -- decoding helpers ------------------------------------------------------------
+def decode_hits(buf, output: str = "full") -> dict:
+    """Host view of a hit buffer: dict of numpy arrays (u, v, inst_id, prim_id[, t])."""
+    a = buf.cpu().numpy() if hasattr(buf, "cpu") else np.asarray(buf)
+    if output == "instance":
+        return {"id": a.view(np.uint32)}
+    rec = _HIT_BYTES[output] // 4
+    w = a.view(np.uint32).reshape(-1, rec)
+    f = a.view(np.float32).reshape(-1, rec)
+    out = {"u": f[:, 0].copy(), "v": f[:, 1].copy(), "inst_id": w[:, 2].copy(), "prim_id": w[:, 3].copy()}
+    if output == "full_t":
+        out["t"] = f[:, 4].copy()
+    return out

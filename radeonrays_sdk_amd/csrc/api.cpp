@@ -1,0 +1,723 @@
+// api.cpp — the C ABI (radeonrays.h, radeonrays_hip.h, radeonrays_ext.h).
+//
+// Mirrors the reference shim src/core/src/radeonrays.cpp: every entry point
+// validates its pointers first (RR_ERROR_INVALID_PARAMETER), unsupported
+// primitive types give RR_ERROR_NOT_IMPLEMENTED, and no C++ exception
+// crosses the ABI (RR_ERROR_INTERNAL).  Behind it there is one backend: HIP
+// on the current device.
+//
+// Command streams: a stream from rrAllocateCommandStream records closures
+// that enqueue kernels; rrSumbitCommandStream runs them, in order, on the
+// context's HIP stream and returns an event (the reference records into a
+// Vulkan command buffer and submits it, vlk/device.cpp:221-240).  A stream
+// wrapped from a client hipStream_t enqueues immediately on that stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/radeonrays.h"
+#include "../../include/radeonrays_ext.h"
+#include "../../include/radeonrays_hip.h"
+#include "build.h"
+#include "rr_layout.h"
+#include "sort.h"
+#include "trace.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Logging (the reference's spdlog singleton, utils/logger.h:33-79, reduced to
+// a level and a sink).  Default level WARN; RR_LOG_LEVEL=1..5 overrides.
+struct Logger {
+    std::mutex mu;
+    int level = RR_LOG_LEVEL_WARN;
+    FILE* sink = stderr;
+    Logger() {
+        if (const char* e = std::getenv("RR_LOG_LEVEL")) {
+            const int v = std::atoi(e);
+            if (v >= RR_LOG_LEVEL_DEBUG && v <= RR_LOG_LEVEL_OFF) level = v;
+        }
+    }
+    void Log(int lvl, const char* fmt, ...) {
+        if (lvl < level || level == RR_LOG_LEVEL_OFF) return;
+        static const char* names[] = {"", "debug", "info", "warn", "error", "off"};
+        std::lock_guard<std::mutex> g(mu);
+        std::fprintf(sink, "[radeonrays] [%s] ", names[lvl]);
+        va_list ap;
+        va_start(ap, fmt);
+        std::vfprintf(sink, fmt, ap);
+        va_end(ap);
+        std::fputc('\n', sink);
+        std::fflush(sink);
+    }
+};
+Logger& Log() {
+    static Logger l;
+    return l;
+}
+#define RR_LOG(lvl, ...) Log().Log(lvl, __VA_ARGS__)
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const char* what) : std::runtime_error(what), code(c) {}
+};
+void Check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(e, (std::string(what) + ": " + hipGetErrorString(e)).c_str());
+}
+
+// ---------------------------------------------------------------------------
+// Handles.
+
+struct DevicePtr {
+    void*  base = nullptr;
+    size_t offset = 0;
+    bool   owned = false;   // from rrAllocateDeviceBuffer
+    size_t size = 0;
+    void*  mapped = nullptr;
+    void*  addr() const { return static_cast<uint8_t*>(base) + offset; }
+};
+
+using Command = std::function<void(hipStream_t)>;
+
+struct Event {
+    hipEvent_t ev = nullptr;
+};
+
+struct Context;
+
+struct CommandStream {
+    Context* ctx = nullptr;
+    bool external = false;
+    hipStream_t stream = nullptr;      // external streams only
+    std::vector<Command> commands;     // internal streams only
+    std::vector<void*> staging;        // pinned uploads owned by recorded commands
+};
+
+struct PendingFree {
+    hipEvent_t ev;
+    void* host;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    std::mutex mu;
+    std::unordered_map<uintptr_t, rr::AccelKind> kinds;  // result buffer -> BLAS/TLAS
+    std::vector<PendingFree> pending;                      // pinned staging to free once done
+    std::vector<Event*> event_pool;
+
+    void Reap(bool wait_all) {
+        std::vector<PendingFree> keep;
+        for (auto& p : pending) {
+            const hipError_t q = wait_all ? hipEventSynchronize(p.ev) : hipEventQuery(p.ev);
+            if (q == hipSuccess) {
+                (void)hipHostFree(p.host);
+                (void)hipEventDestroy(p.ev);
+            } else {
+                keep.push_back(p);
+            }
+        }
+        pending.swap(keep);
+    }
+    // Free `host` once everything enqueued on `s` so far has finished.
+    void FreeAfter(hipStream_t s, void* host) {
+        hipEvent_t ev;
+        Check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        Check(hipEventRecord(ev, s), "hipEventRecord");
+        pending.push_back({ev, host});
+    }
+};
+
+Context* AsCtx(RRContext c) { return reinterpret_cast<Context*>(c); }
+DevicePtr* AsPtr(RRDevicePtr p) { return reinterpret_cast<DevicePtr*>(p); }
+CommandStream* AsStream(RRCommandStream s) { return reinterpret_cast<CommandStream*>(s); }
+Event* AsEvent(RREvent e) { return reinterpret_cast<Event*>(e); }
+void* Addr(RRDevicePtr p) { return p ? AsPtr(p)->addr() : nullptr; }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Enqueue now (external stream) or record for submission (internal stream).
+void Record(CommandStream* cs, Command cmd) {
+    if (cs->external) {
+        cmd(cs->stream);
+        Check(hipGetLastError(), "kernel launch");
+    } else {
+        cs->commands.push_back(std::move(cmd));
+    }
+}
+
+template <typename F>
+RRError Guarded(const char* name, F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "%s: %s", name, e.what());
+        return (e.code == hipErrorOutOfMemory) ? RR_ERROR_OUT_OF_DEVICE_MEMORY : RR_ERROR_INTERNAL;
+    } catch (const std::bad_alloc&) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "%s: out of host memory", name);
+        return RR_ERROR_OUT_OF_HOST_MEMORY;
+    } catch (const std::exception& e) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "%s: %s", name, e.what());
+        return RR_ERROR_INTERNAL;
+    } catch (...) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "%s: unknown exception", name);
+        return RR_ERROR_INTERNAL;
+    }
+}
+
+#define RR_REQUIRE(cond, name)                                                 \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            RR_LOG(RR_LOG_LEVEL_ERROR, "%s: invalid parameter (%s)", name, #cond); \
+            return RR_ERROR_INVALID_PARAMETER;                                 \
+        }                                                                      \
+    } while (0)
+
+// Mesh description from a build input (GetTriangleMeshBuildInfo, radeonrays.cpp:59-74).
+RRError MeshFromInput(const RRGeometryBuildInput* in, rr::MeshDesc& m, const char* name) {
+    if (in->primitive_type != RR_PRIMITIVE_TYPE_TRIANGLE_MESH) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "%s: build input type not supported", name);
+        return RR_ERROR_NOT_IMPLEMENTED;
+    }
+    RR_REQUIRE(in->primitive_count == 1 && in->triangle_mesh_primitives, name);
+    const RRTriangleMeshPrimitive& p = in->triangle_mesh_primitives[0];
+    if (p.triangle_count > 0) {
+        RR_REQUIRE(p.vertices && p.triangle_indices && p.vertex_count > 0, name);
+        RR_REQUIRE(p.vertex_stride >= 12 && p.vertex_stride % 4 == 0, name);
+    }
+    RR_REQUIRE(p.index_type == RR_INDEX_TYPE_UINT32 || p.index_type == RR_INDEX_TYPE_UINT16, name);
+    RR_REQUIRE(p.triangle_count < (1u << 30), name);
+    m.vertices = Addr(p.vertices);
+    m.vertex_count = p.vertex_count;
+    m.vertex_stride = p.vertex_stride;
+    m.indices = Addr(p.triangle_indices);
+    m.index16 = p.index_type == RR_INDEX_TYPE_UINT16 ? 1u : 0u;
+    m.tri_count = p.triangle_count;
+    return RR_SUCCESS;
+}
+
+bool WantsRestructure(const RRBuildOptions* o) {
+    // intersector.cpp:116,170: restructure only with options that do not ask for a fast build.
+    return o && (o->build_flags & RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD) == 0;
+}
+
+RRError CreateContext(int device, hipStream_t stream, RRContext* out) {
+    return Guarded("rrCreateContext", [&]() -> RRError {
+        int count = 0;
+        Check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) return RR_ERROR_INVALID_PARAMETER;
+        auto* c = new Context;
+        c->device = device;
+        DeviceGuard g(device);
+        if (stream) {
+            c->stream = stream;
+            c->own_stream = false;
+        } else {
+            Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        }
+        *out = reinterpret_cast<RRContext>(c);
+        return RR_SUCCESS;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Context and logging
+
+RRError rrCreateContext(uint32_t api_version, RRApi api, RRContext* context) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrCreateContext(%u)", api_version);
+    RR_REQUIRE(context, "rrCreateContext");
+    if (api != RR_API_HIP) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "rrCreateContext: only RR_API_HIP is available");
+        return RR_ERROR_UNSUPPORTED_API;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    return CreateContext(dev, nullptr, context);
+}
+
+RRError rrCreateContextHip(uint32_t api_version, int device, void* hip_stream, RRContext* context) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrCreateContextHip(%u, %d)", api_version, device);
+    RR_REQUIRE(context, "rrCreateContextHip");
+    return CreateContext(device, static_cast<hipStream_t>(hip_stream), context);
+}
+
+RRError rrDestroyContext(RRContext context) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrDestroyContext");
+    RR_REQUIRE(context, "rrDestroyContext");
+    return Guarded("rrDestroyContext", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        DeviceGuard g(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        c->Reap(true);
+        for (Event* e : c->event_pool) {
+            (void)hipEventDestroy(e->ev);
+            delete e;
+        }
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrSetLogLevel(RRLogLevel log_level) {
+    if (log_level < RR_LOG_LEVEL_DEBUG || log_level > RR_LOG_LEVEL_OFF) return RR_ERROR_INVALID_PARAMETER;
+    Log().level = log_level;
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrSetLogLevel(%d)", (int)log_level);
+    return RR_SUCCESS;
+}
+
+RRError rrSetLogFile(char const* filename) {
+    RR_REQUIRE(filename, "rrSetLogFile");
+    FILE* f = std::fopen(filename, "a");
+    if (!f) return RR_ERROR_INTERNAL;
+    {
+        std::lock_guard<std::mutex> g(Log().mu);
+        if (Log().sink && Log().sink != stderr && Log().sink != stdout) std::fclose(Log().sink);
+        Log().sink = f;
+    }
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrSetLogFile(%s)", filename);
+    return RR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Memory requirements
+
+RRError rrGetGeometryBuildMemoryRequirements(RRContext context, const RRGeometryBuildInput* build_input,
+                                             const RRBuildOptions* build_options,
+                                             RRMemoryRequirements* memory_requirements) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrGetGeometryBuildMemoryRequirements");
+    RR_REQUIRE(context && build_input && memory_requirements, "rrGetGeometryBuildMemoryRequirements");
+    rr::MeshDesc m;
+    const RRError e = MeshFromInput(build_input, m, "rrGetGeometryBuildMemoryRequirements");
+    if (e != RR_SUCCESS) return e;
+    memory_requirements->result_buffer_size = rr::BlasResultSize(m.tri_count);
+    memory_requirements->temporary_build_buffer_size = rr::BlasBuildScratchBytes(m.tri_count, WantsRestructure(build_options));
+    memory_requirements->temporary_update_buffer_size = 0;  // refit needs no scratch
+    return RR_SUCCESS;
+}
+
+RRError rrGetSceneBuildMemoryRequirements(RRContext context, const RRSceneBuildInput* build_input,
+                                          const RRBuildOptions*, RRMemoryRequirements* memory_requirements) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrGetSceneBuildMemoryRequirements");
+    RR_REQUIRE(context && build_input && memory_requirements, "rrGetSceneBuildMemoryRequirements");
+    RR_REQUIRE(build_input->instance_count < (1u << 30), "rrGetSceneBuildMemoryRequirements");
+    memory_requirements->result_buffer_size = rr::TlasResultSize(build_input->instance_count);
+    memory_requirements->temporary_build_buffer_size = rr::TlasBuildScratchBytes(build_input->instance_count);
+    memory_requirements->temporary_update_buffer_size = 0;
+    return RR_SUCCESS;
+}
+
+RRError rrGetTraceMemoryRequirements(RRContext context, uint32_t ray_count, size_t* scratch_size) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrGetTraceMemoryRequirements");
+    // radeonrays.cpp:741 rejects ray_count == 0 as well.
+    RR_REQUIRE(context && scratch_size && ray_count, "rrGetTraceMemoryRequirements");
+    *scratch_size = rr::TraceScratchBytes(ray_count);
+    return RR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Builds and queries
+
+RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, const RRGeometryBuildInput* build_input,
+                           const RRBuildOptions* build_options, RRDevicePtr temporary_buffer,
+                           RRDevicePtr geometry_buffer, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdBuildGeometry");
+    RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildGeometry");
+    RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
+               "rrCmdBuildGeometry");
+    rr::MeshDesc m;
+    const RRError e = MeshFromInput(build_input, m, "rrCmdBuildGeometry");
+    if (e != RR_SUCCESS) return e;
+    RR_REQUIRE(geometry_buffer, "rrCmdBuildGeometry");
+    if (build_operation == RR_BUILD_OPERATION_BUILD) RR_REQUIRE(temporary_buffer, "rrCmdBuildGeometry");
+    return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        DeviceGuard g(c->device);
+        void* result = Addr(geometry_buffer);
+        void* scratch = Addr(temporary_buffer);
+        const bool restructure = WantsRestructure(build_options);
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelBlas;
+        }
+        if (build_operation == RR_BUILD_OPERATION_BUILD)
+            Record(AsStream(command_stream), [=](hipStream_t s) { rr::BuildBlas(m, restructure, scratch, result, s); });
+        else
+            Record(AsStream(command_stream), [=](hipStream_t s) { rr::UpdateBlas(m, result, s); });
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input, const RRBuildOptions*,
+                        RRDevicePtr temporary_buffer, RRDevicePtr scene_buffer, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdBuildScene");
+    RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildScene");
+    RR_REQUIRE(scene_buffer && temporary_buffer, "rrCmdBuildScene");
+    RR_REQUIRE(build_input->instance_count == 0 || build_input->instances, "rrCmdBuildScene");
+    RR_REQUIRE(build_input->instance_count < (1u << 30), "rrCmdBuildScene");
+    for (uint32_t i = 0; i < build_input->instance_count; ++i)
+        RR_REQUIRE(build_input->instances[i].geometry, "rrCmdBuildScene");
+    return Guarded("rrCmdBuildScene", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        CommandStream* cs = AsStream(command_stream);
+        DeviceGuard g(c->device);
+        const uint32_t count = build_input->instance_count;
+        void* result = Addr(scene_buffer);
+        void* scratch = Addr(temporary_buffer);
+        // Pack the instances now (the caller's array may go away after this
+        // call, as in intersector.cpp:218-247) into pinned memory that the
+        // recorded upload reads at execution time.
+        rr::InstanceUpload* host = nullptr;
+        if (count) {
+            Check(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(rr::InstanceUpload) * count, hipHostMallocDefault),
+                  "hipHostMalloc");
+            for (uint32_t i = 0; i < count; ++i) {
+                std::memcpy(host[i].xform.m, build_input->instances[i].transform, sizeof(rr::Transform));
+                host[i].blas = reinterpret_cast<uint64_t>(Addr(build_input->instances[i].geometry));
+                host[i].pad = 0;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelTlas;
+        }
+        auto cmd = [=](hipStream_t s) {
+            auto* dev_inst = reinterpret_cast<rr::InstanceUpload*>(static_cast<uint8_t*>(scratch) + rr::TlasUploadOffset());
+            if (count)
+                Check(hipMemcpyAsync(dev_inst, host, sizeof(rr::InstanceUpload) * count, hipMemcpyHostToDevice, s),
+                      "hipMemcpyAsync");
+            rr::BuildTlas(dev_inst, count, scratch, result, s);
+        };
+        if (cs->external) {
+            Record(cs, cmd);
+            if (host) {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->FreeAfter(cs->stream, host);
+            }
+        } else {
+            Record(cs, cmd);
+            if (host) cs->staging.push_back(host);
+        }
+        return RR_SUCCESS;
+    });
+}
+
+static RRError KindOf(Context* c, void* accel, rr::AccelKind& kind) {
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto it = c->kinds.find(reinterpret_cast<uintptr_t>(accel));
+        if (it != c->kinds.end()) {
+            kind = it->second;
+            return RR_SUCCESS;
+        }
+    }
+    // Built elsewhere: read the buffer's own header.
+    rr::AccelHeader h;
+    Check(hipMemcpy(&h, accel, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (h.magic != rr::kAccelMagic || (h.kind != rr::kAccelBlas && h.kind != rr::kAccelTlas))
+        return RR_ERROR_INVALID_PARAMETER;
+    kind = static_cast<rr::AccelKind>(h.kind);
+    return RR_SUCCESS;
+}
+
+RRError rrCmdIntersect(RRContext context, RRDevicePtr scene_buffer, RRIntersectQuery query, RRDevicePtr rays,
+                       uint32_t ray_count, RRDevicePtr indirect_ray_count, RRIntersectQueryOutput query_output,
+                       RRDevicePtr hits, RRDevicePtr scratch, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdIntersect");
+    RR_REQUIRE(context && scene_buffer && rays && hits && scratch && command_stream, "rrCmdIntersect");
+    RR_REQUIRE(query == RR_INTERSECT_QUERY_CLOSEST || query == RR_INTERSECT_QUERY_ANY, "rrCmdIntersect");
+    RR_REQUIRE(query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT || query_output == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
+                   query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
+               "rrCmdIntersect");
+    return Guarded("rrCmdIntersect", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        DeviceGuard g(c->device);
+        void* accel = Addr(scene_buffer);
+        rr::AccelKind kind;
+        const RRError e = KindOf(c, accel, kind);
+        if (e != RR_SUCCESS) return e;
+        const void* r = Addr(rays);
+        void* h = Addr(hits);
+        void* sc = Addr(scratch);
+        const uint32_t* ind = static_cast<const uint32_t*>(Addr(indirect_ray_count));
+        const auto q = query == RR_INTERSECT_QUERY_ANY ? rr::kQueryAny : rr::kQueryClosest;
+        const auto o = static_cast<rr::OutputKind>(query_output);
+        Record(AsStream(command_stream), [=](hipStream_t s) { rr::Trace(accel, kind, q, o, r, ray_count, ind, h, sc, s); });
+        return RR_SUCCESS;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Command streams and events
+
+RRError rrAllocateCommandStream(RRContext context, RRCommandStream* command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrAllocateCommandStream");
+    RR_REQUIRE(context && command_stream, "rrAllocateCommandStream");
+    return Guarded("rrAllocateCommandStream", [&]() -> RRError {
+        auto* cs = new CommandStream;
+        cs->ctx = AsCtx(context);
+        *command_stream = reinterpret_cast<RRCommandStream>(cs);
+        return RR_SUCCESS;
+    });
+}
+
+static void ReleaseStaging(Context* c, CommandStream* cs) {
+    if (cs->staging.empty()) return;
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (void* p : cs->staging) c->FreeAfter(c->stream, p);
+    cs->staging.clear();
+}
+
+RRError rrReleaseCommandStream(RRContext context, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseCommandStream");
+    RR_REQUIRE(context && command_stream, "rrReleaseCommandStream");
+    return Guarded("rrReleaseCommandStream", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        CommandStream* cs = AsStream(command_stream);
+        DeviceGuard g(c->device);
+        ReleaseStaging(c, cs);
+        delete cs;
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrSumbitCommandStream(RRContext context, RRCommandStream command_stream, RREvent wait_event, RREvent* out_event) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrSumbitCommandStream");
+    RR_REQUIRE(context && command_stream && out_event, "rrSumbitCommandStream");
+    return Guarded("rrSumbitCommandStream", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        CommandStream* cs = AsStream(command_stream);
+        DeviceGuard g(c->device);
+        const hipStream_t s = cs->external ? cs->stream : c->stream;
+        if (wait_event) Check(hipStreamWaitEvent(s, AsEvent(wait_event)->ev, 0), "hipStreamWaitEvent");
+        for (auto& cmd : cs->commands) cmd(s);
+        cs->commands.clear();
+        Check(hipGetLastError(), "kernel launch");
+        // Staging read by this submission is freed once it has run.
+        ReleaseStaging(c, cs);
+        Event* ev = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            c->Reap(false);
+            if (!c->event_pool.empty()) {
+                ev = c->event_pool.back();
+                c->event_pool.pop_back();
+            }
+        }
+        if (!ev) {
+            ev = new Event;
+            Check(hipEventCreateWithFlags(&ev->ev, hipEventDisableTiming), "hipEventCreate");
+        }
+        Check(hipEventRecord(ev->ev, s), "hipEventRecord");
+        *out_event = reinterpret_cast<RREvent>(ev);
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrReleaseEvent(RRContext context, RREvent event) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseEvent");
+    RR_REQUIRE(context && event, "rrReleaseEvent");
+    Context* c = AsCtx(context);
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->event_pool.push_back(AsEvent(event));
+    return RR_SUCCESS;
+}
+
+RRError rrWaitEvent(RRContext context, RREvent event) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrWaitEvent");
+    RR_REQUIRE(context && event, "rrWaitEvent");
+    return Guarded("rrWaitEvent", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        Check(hipEventSynchronize(AsEvent(event)->ev), "hipEventSynchronize");
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrGetCommandStreamFromHipStream(RRContext context, void* hip_stream, RRCommandStream* command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrGetCommandStreamFromHipStream");
+    RR_REQUIRE(context && command_stream, "rrGetCommandStreamFromHipStream");
+    return Guarded("rrGetCommandStreamFromHipStream", [&]() -> RRError {
+        auto* cs = new CommandStream;
+        cs->ctx = AsCtx(context);
+        cs->external = true;
+        cs->stream = static_cast<hipStream_t>(hip_stream);
+        *command_stream = reinterpret_cast<RRCommandStream>(cs);
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrReleaseExternalCommandStream(RRContext context, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseExternalCommandStream");
+    RR_REQUIRE(context && command_stream, "rrReleaseExternalCommandStream");
+    delete AsStream(command_stream);
+    return RR_SUCCESS;
+}
+
+void* rrGetHipStream(RRCommandStream command_stream) {
+    if (!command_stream) return nullptr;
+    CommandStream* cs = AsStream(command_stream);
+    return cs->external ? cs->stream : cs->ctx->stream;
+}
+
+// ---------------------------------------------------------------------------
+// Device pointers
+
+RRError rrGetDevicePtrFromHipPtr(RRContext context, void* device_address, size_t offset, RRDevicePtr* device_ptr) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrGetDevicePtrFromHipPtr");
+    RR_REQUIRE(context && device_address && device_ptr, "rrGetDevicePtrFromHipPtr");
+    return Guarded("rrGetDevicePtrFromHipPtr", [&]() -> RRError {
+        auto* p = new DevicePtr;
+        p->base = device_address;
+        p->offset = offset;
+        *device_ptr = reinterpret_cast<RRDevicePtr>(p);
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrReleaseDevicePtr(RRContext context, RRDevicePtr ptr) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseDevicePtr");
+    RR_REQUIRE(context && ptr, "rrReleaseDevicePtr");
+    return Guarded("rrReleaseDevicePtr", [&]() -> RRError {
+        DevicePtr* p = AsPtr(ptr);
+        if (p->owned) {
+            DeviceGuard g(AsCtx(context)->device);
+            (void)hipStreamSynchronize(AsCtx(context)->stream);
+            if (p->mapped) std::free(p->mapped);
+            (void)hipFree(p->base);
+        }
+        delete p;
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrAllocateDeviceBuffer(RRContext context, size_t size, RRDevicePtr* device_ptr) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrAllocateDeviceBuffer(%zu)", size);
+    RR_REQUIRE(context && size && device_ptr, "rrAllocateDeviceBuffer");
+    return Guarded("rrAllocateDeviceBuffer", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        void* mem = nullptr;
+        Check(hipMalloc(&mem, size), "hipMalloc");
+        auto* p = new DevicePtr;
+        p->base = mem;
+        p->owned = true;
+        p->size = size;
+        *device_ptr = reinterpret_cast<RRDevicePtr>(p);
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrMapDevicePtr(RRContext context, RRDevicePtr device_ptr, void** mapping_ptr) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrMapDevicePtr");
+    RR_REQUIRE(context && device_ptr && mapping_ptr, "rrMapDevicePtr");
+    DevicePtr* p = AsPtr(device_ptr);
+    RR_REQUIRE(p->owned, "rrMapDevicePtr");
+    return Guarded("rrMapDevicePtr", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        DeviceGuard g(c->device);
+        if (!p->mapped) {
+            p->mapped = std::malloc(p->size);
+            if (!p->mapped) return RR_ERROR_OUT_OF_HOST_MEMORY;
+        }
+        Check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        Check(hipMemcpy(p->mapped, p->base, p->size, hipMemcpyDeviceToHost), "hipMemcpy");
+        *mapping_ptr = p->mapped;
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrUnmapDevicePtr(RRContext context, RRDevicePtr device_ptr, void** mapping_ptr) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrUnmapDevicePtr");
+    RR_REQUIRE(context && device_ptr && mapping_ptr, "rrUnmapDevicePtr");
+    DevicePtr* p = AsPtr(device_ptr);
+    RR_REQUIRE(p->owned && p->mapped, "rrUnmapDevicePtr");
+    return Guarded("rrUnmapDevicePtr", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        Check(hipMemcpy(p->base, p->mapped, p->size, hipMemcpyHostToDevice), "hipMemcpy");
+        std::free(p->mapped);
+        p->mapped = nullptr;
+        *mapping_ptr = nullptr;
+        return RR_SUCCESS;
+    });
+}
+
+void* rrGetHipAddress(RRDevicePtr device_ptr) { return Addr(device_ptr); }
+
+// ---------------------------------------------------------------------------
+// Extensions
+
+RRError rrxGetSortMemoryRequirements(RRContext context, uint32_t n, size_t* scratch_size) {
+    RR_REQUIRE(context && scratch_size, "rrxGetSortMemoryRequirements");
+    RR_REQUIRE(n < (1u << 30), "rrxGetSortMemoryRequirements");
+    *scratch_size = rr::SortScratchBytes(n);
+    return RR_SUCCESS;
+}
+
+RRError rrxCmdSortPairs(RRContext context, RRDevicePtr keys, RRDevicePtr values, RRDevicePtr keys_tmp,
+                        RRDevicePtr values_tmp, uint32_t n, uint32_t key_bits, RRDevicePtr scratch,
+                        RRCommandStream command_stream) {
+    RR_REQUIRE(context && keys && values && keys_tmp && values_tmp && scratch && command_stream, "rrxCmdSortPairs");
+    RR_REQUIRE(n < (1u << 30) && key_bits >= 1 && key_bits <= 32, "rrxCmdSortPairs");
+    return Guarded("rrxCmdSortPairs", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        auto* k = static_cast<uint32_t*>(Addr(keys));
+        auto* v = static_cast<uint32_t*>(Addr(values));
+        auto* kt = static_cast<uint32_t*>(Addr(keys_tmp));
+        auto* vt = static_cast<uint32_t*>(Addr(values_tmp));
+        void* sc = Addr(scratch);
+        Record(AsStream(command_stream), [=](hipStream_t s) {
+            const rr::SortScratch ss = rr::CarveSortScratch(sc, n);
+            const bool in_place = rr::RadixSortPairs(k, v, kt, vt, n, (int)key_bits, ss, false, s);
+            if (!in_place) {
+                Check(hipMemcpyAsync(k, kt, size_t(n) * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                Check(hipMemcpyAsync(v, vt, size_t(n) * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+            }
+        });
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrxCmdIntersectWithStats(RRContext context, RRDevicePtr scene_buffer, RRIntersectQuery query, RRDevicePtr rays,
+                                 uint32_t ray_count, RRDevicePtr hits, RRDevicePtr scratch, RRDevicePtr stats,
+                                 RRCommandStream command_stream) {
+    RR_REQUIRE(context && scene_buffer && rays && hits && scratch && stats && command_stream, "rrxCmdIntersectWithStats");
+    return Guarded("rrxCmdIntersectWithStats", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        DeviceGuard g(c->device);
+        void* accel = Addr(scene_buffer);
+        rr::AccelKind kind;
+        const RRError e = KindOf(c, accel, kind);
+        if (e != RR_SUCCESS) return e;
+        const void* r = Addr(rays);
+        void* h = Addr(hits);
+        void* sc = Addr(scratch);
+        auto* st = static_cast<uint32_t*>(Addr(stats));
+        const auto q = query == RR_INTERSECT_QUERY_ANY ? rr::kQueryAny : rr::kQueryClosest;
+        Record(AsStream(command_stream), [=](hipStream_t s) { rr::TraceWithStats(accel, kind, q, r, ray_count, h, sc, st, s); });
+        return RR_SUCCESS;
+    });
+}
+
+}  // extern "C"

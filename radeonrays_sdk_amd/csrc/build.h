@@ -1,0 +1,51 @@
+// build.h — host launchers for acceleration-structure builds.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rr_layout.h"
+
+namespace rr {
+
+// A triangle mesh in device memory (RRTriangleMeshPrimitive, radeonrays.h:182-192).
+struct MeshDesc {
+    const void* vertices = nullptr;
+    uint32_t    vertex_count = 0;
+    uint32_t    vertex_stride = 12;
+    const void* indices = nullptr;
+    uint32_t    index16 = 0;      // 1: uint16 indices, 0: uint32
+    uint32_t    tri_count = 0;
+};
+
+// One instance as uploaded for a TLAS build: forward object-to-world
+// transform and the device address of the instance's BLAS buffer.
+struct InstanceUpload {
+    Transform xform;
+    uint64_t  blas;
+    uint64_t  pad;
+};
+static_assert(sizeof(InstanceUpload) == 64, "InstanceUpload must be 64 bytes");
+
+size_t BlasBuildScratchBytes(uint32_t tris, bool restructure);
+size_t TlasBuildScratchBytes(uint32_t instances);
+
+// LBVH build of one mesh into `result` (BlasResultSize bytes), optionally
+// followed by treelet restructuring.  All work is enqueued on `stream`.
+void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* result, hipStream_t stream);
+
+// Refit of a BLAS built from a mesh with the same triangles and indices.
+void UpdateBlas(const MeshDesc& mesh, void* result, hipStream_t stream);
+
+// TLAS build over `count` instances whose descriptions are already in
+// device memory at `instances` (typically the head of `scratch`).
+void BuildTlas(const InstanceUpload* instances, uint32_t count, void* scratch, void* result, hipStream_t stream);
+
+// Byte offset inside the TLAS scratch where the instance upload goes.
+inline size_t TlasUploadOffset() { return 0; }
+
+// Treelet restructuring of a built BLAS in place (three rounds, minimum
+// treelet-root sizes 64, 128, 256 primitives).
+void RestructureBlas(uint32_t tris, void* scratch, void* result, hipStream_t stream);
+
+}  // namespace rr

@@ -1,0 +1,813 @@
+// build.hip — LBVH construction, refit and TLAS build for gfx950.
+//
+// Behavioural spec: the reference's Vulkan LBVH path
+// (vlk/hlbvh_builder.cpp:159-351 and its kernels lbvh_init_mesh.comp,
+// lbvh_calc_mesh_aabb.comp, lbvh_calc_morton_codes_mesh.comp,
+// lbvh_emit_hierarchy_mesh.comp, lbvh_fit_aabb_mesh.comp; the scene twins for
+// the TLAS).  The launch structure is this library's own:
+//   init (header, bounds, sort tickets) -> mesh bounds (wave64 reductions,
+//   one set of ordered-uint atomics per block) -> Morton codes fused with the
+//   radix histogram -> onesweep sort -> Karras emission -> refit, where each
+//   finished node writes its box straight into its parent's child slot and
+//   the parity of the parent's `update` counter tells the second arriver to
+//   continue (no reset pass is needed before an update).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+
+#include "build.h"
+#include "rr_device.h"
+#include "sort.h"
+
+namespace rr {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kMaxGrid = 4096;
+
+inline uint32_t Blocks(size_t n, int per = kBlock) { return (uint32_t)std::max<size_t>(1, (n + per - 1) / per); }
+
+struct MeshView {
+    const uint8_t* vertices;
+    const void*    indices;
+    uint32_t       stride;
+    uint32_t       index16;
+    uint32_t       vertex_count;
+    uint32_t       tri_count;
+};
+
+MeshView ViewOf(const MeshDesc& m) {
+    return MeshView{static_cast<const uint8_t*>(m.vertices), m.indices, m.vertex_stride, m.index16, m.vertex_count,
+                    m.tri_count};
+}
+
+// Fetch triangle t.  Indices past the vertex array are clamped to its last
+// vertex so that a bad index buffer cannot read out of bounds.
+__device__ __forceinline__ void LoadTriangle(const MeshView& m, uint32_t t, float v[9]) {
+    uint32_t i0, i1, i2;
+    if (m.index16) {
+        const uint16_t* ix = static_cast<const uint16_t*>(m.indices) + 3 * (size_t)t;
+        i0 = ix[0]; i1 = ix[1]; i2 = ix[2];
+    } else {
+        const uint32_t* ix = static_cast<const uint32_t*>(m.indices) + 3 * (size_t)t;
+        i0 = ix[0]; i1 = ix[1]; i2 = ix[2];
+    }
+    const uint32_t last = m.vertex_count - 1;
+    i0 = min(i0, last); i1 = min(i1, last); i2 = min(i2, last);
+    const float* p0 = reinterpret_cast<const float*>(m.vertices + (size_t)i0 * m.stride);
+    const float* p1 = reinterpret_cast<const float*>(m.vertices + (size_t)i1 * m.stride);
+    const float* p2 = reinterpret_cast<const float*>(m.vertices + (size_t)i2 * m.stride);
+    v[0] = p0[0]; v[1] = p0[1]; v[2] = p0[2];
+    v[3] = p1[0]; v[4] = p1[1]; v[5] = p1[2];
+    v[6] = p2[0]; v[7] = p2[1]; v[8] = p2[2];
+}
+
+// Scratch block at the head of every build scratch buffer.
+struct BuildHead {
+    uint32_t bounds[8];   // ordered-uint min xyz, pad, max xyz, pad
+    uint32_t misc[56];
+};
+
+struct BlasScratchLayout {
+    size_t head, keys, vals, keys_tmp, vals_tmp, sort, total;
+};
+
+BlasScratchLayout LayoutBlas(uint32_t n) {
+    BlasScratchLayout l;
+    size_t o = 0;
+    l.head = o; o += 256;
+    l.keys = o; o += AlignUp(size_t(n) * 4, 256);
+    l.vals = o; o += AlignUp(size_t(n) * 4, 256);
+    l.keys_tmp = o; o += AlignUp(size_t(n) * 4, 256);
+    l.vals_tmp = o; o += AlignUp(size_t(n) * 4, 256);
+    l.sort = o; o += SortScratchBytes(n);
+    l.total = o;
+    return l;
+}
+
+// Treelet restructuring scratch (reuses the key/value region of the LBVH
+// scratch, which is dead once the hierarchy is emitted).
+struct RestructureLayout {
+    size_t counter, roots, counts, total;
+};
+RestructureLayout LayoutRestructure(uint32_t n) {
+    RestructureLayout l;
+    size_t o = 0;
+    l.counter = o; o += 256;
+    l.roots = o; o += AlignUp(size_t(n) * 4, 256);
+    l.counts = o; o += AlignUp(size_t(NodeCount(n)) * 4, 256);
+    l.total = o;
+    return l;
+}
+
+template <typename T>
+T* At(void* base, size_t off) {
+    return reinterpret_cast<T*>(static_cast<uint8_t*>(base) + off);
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+
+__global__ __launch_bounds__(kBlock) void InitBuildKernel(AccelHeader* hdr, uint32_t kind, uint32_t prims,
+                                                          uint64_t nodes_off, uint64_t xf_off, uint64_t table_off,
+                                                          uint32_t flags, BuildHead* head, uint32_t* sort_hist,
+                                                          uint32_t* sort_tickets) {
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        const uint32_t serial = hdr->magic == kAccelMagic ? hdr->build_serial + 1 : 1;
+        AccelHeader h = {};
+        h.magic = kAccelMagic;
+        h.kind = kind;
+        h.prim_count = prims;
+        h.node_count = NodeCount(prims);
+        h.nodes_offset = nodes_off;
+        h.transforms_offset = xf_off;
+        h.blas_table_offset = table_off;
+        h.flags = flags;
+        h.build_serial = serial;
+        *hdr = h;
+    }
+    if (t < 8) head->bounds[t] = (t < 4) ? 0xFFFFFFFFu : 0u;
+    for (uint32_t i = t; i < kSortMaxPasses * kSortRadix; i += kBlock) sort_hist[i] = 0;
+    if (t < kSortMaxPasses) sort_tickets[t] = 0;
+}
+
+// Block min/max of ordered-uint coordinates, then 6 device atomics per
+// block (lbvh_calc_mesh_aabb.comp:121-179 reduces in a 128-wide LDS tree
+// with atomics per loop iteration).
+__device__ __forceinline__ void ReduceBoundsToGlobal(uint32_t mn[3], uint32_t mx[3], uint32_t* bounds) {
+    __shared__ uint32_t s_red[kBlock / 64][6];
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = WaveMin(mn[a]);
+        mx[a] = WaveMax(mx[a]);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) { s_red[wave][a] = mn[a]; s_red[wave][3 + a] = mx[a]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t a = threadIdx.x;
+        uint32_t r = s_red[0][a];
+        for (int w = 1; w < kBlock / 64; ++w) r = a < 3 ? min(r, s_red[w][a]) : max(r, s_red[w][a]);
+        if (a < 3) atomicMin(&bounds[a], r);
+        else atomicMax(&bounds[4 + (a - 3)], r);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void MeshBoundsKernel(MeshView m, BuildHead* head) {
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < m.tri_count; t += gridDim.x * kBlock) {
+        float v[9];
+        LoadTriangle(m, t, v);
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t o = FloatToOrdered(v[3 * k + a]);
+                mn[a] = min(mn[a], o);
+                mx[a] = max(mx[a], o);
+            }
+    }
+    ReduceBoundsToGlobal(mn, mx, head->bounds);
+}
+
+struct SceneBounds {
+    float lo[3], ext[3];
+};
+__device__ __forceinline__ SceneBounds ReadBounds(const BuildHead* head) {
+    SceneBounds b;
+    for (int a = 0; a < 3; ++a) {
+        const float lo = OrderedToFloat(head->bounds[a]);
+        const float hi = OrderedToFloat(head->bounds[4 + a]);
+        b.lo[a] = lo;
+        b.ext[a] = hi - lo;
+    }
+    return b;
+}
+
+// Morton code of each primitive's box centroid (lbvh_calc_morton_codes_mesh.comp:78-130),
+// fused with the sort's digit histogram and the clearing of its look-back words.
+template <typename CentroidFn>
+__device__ __forceinline__ void MortonAndHistogram(uint32_t n, const BuildHead* head, CentroidFn centroid,
+                                                   uint32_t* keys, uint32_t* vals, int passes, uint32_t* sort_hist,
+                                                   uint32_t* sort_status, size_t status_words) {
+    __shared__ uint32_t s_h[kSortMaxPasses][kSortRadix];
+    for (uint32_t i = threadIdx.x; i < kSortMaxPasses * kSortRadix; i += kBlock) (&s_h[0][0])[i] = 0;
+    const SceneBounds b = ReadBounds(head);
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t key = 0;
+    if (valid) {
+        float c[3];
+        centroid(i, c);
+        key = Morton30(UnitCoord(c[0], b.lo[0], b.ext[0]), UnitCoord(c[1], b.lo[1], b.ext[1]),
+                       UnitCoord(c[2], b.lo[2], b.ext[2]));
+        keys[i] = key;
+        vals[i] = i;
+    }
+    AccumulateDigits(key, valid, s_h, passes);
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < (uint32_t)passes * kSortRadix; d += kBlock) {
+        const uint32_t c = (&s_h[0][0])[d];
+        if (c) atomicAdd(&sort_hist[d], c);
+    }
+    for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += (size_t)gridDim.x * kBlock)
+        sort_status[w] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void MeshMortonKernel(MeshView m, const BuildHead* head, uint32_t* keys,
+                                                           uint32_t* vals, int passes, uint32_t* sort_hist,
+                                                           uint32_t* sort_status, size_t status_words) {
+    auto centroid = [&](uint32_t t, float c[3]) {
+        float v[9];
+        LoadTriangle(m, t, v);
+        for (int a = 0; a < 3; ++a) {
+            const float lo = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+            const float hi = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+            c[a] = (lo + hi) * 0.5f;
+        }
+    };
+    MortonAndHistogram(m.tri_count, head, centroid, keys, vals, passes, sort_hist, sort_status, status_words);
+}
+
+// Karras 2012 binary radix tree over sorted codes
+// (lbvh_emit_hierarchy_mesh.comp:85-217).  delta(i, j) is the common prefix
+// length of codes i and j, with 32 + clz(i ^ j) for equal codes and -1 for
+// j outside [0, n).
+__device__ __forceinline__ int Delta(const uint32_t* __restrict__ codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = codes[i], b = codes[j];
+    return a != b ? __clz(a ^ b) : 32 + __clz((uint32_t)(i ^ j));
+}
+
+__global__ __launch_bounds__(kBlock) void EmitHierarchyKernel(const uint32_t* __restrict__ codes,
+                                                              const uint32_t* __restrict__ prims, uint32_t n,
+                                                              BvhNode* __restrict__ nodes) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t leaf = n - 1 + i;
+    nodes[leaf].child0 = kInvalid;
+    nodes[leaf].child1 = prims[i];
+    nodes[leaf].update = 0;
+    if (n == 1) {
+        nodes[0].parent = kInvalid;
+        return;
+    }
+    if (i >= n - 1) return;
+    const int N = (int)n, I = (int)i;
+    // Direction and far end of node i's key range.
+    const int d = (Delta(codes, N, I, I + 1) - Delta(codes, N, I, I - 1)) > 0 ? 1 : -1;
+    const int dmin = Delta(codes, N, I, I - d);
+    int lmax = 2;
+    while (Delta(codes, N, I, I + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (Delta(codes, N, I, I + (l + t) * d) > dmin) l += t;
+    const int j = I + l * d;
+    const int first = min(I, j), last = max(I, j);
+    // Split: last position that shares more than the range's common prefix with `first`.
+    const int common = Delta(codes, N, first, last);
+    int lo = first, hi = last;
+    while (hi > lo + 1) {
+        const int mid = (lo + hi) / 2;
+        if (Delta(codes, N, first, mid) > common) lo = mid;
+        else hi = mid;
+    }
+    const int split = lo;
+    const uint32_t left = (split == first) ? (n - 1 + (uint32_t)split) : (uint32_t)split;
+    const uint32_t right = (split + 1 == last) ? (n - 1 + (uint32_t)split + 1) : (uint32_t)split + 1;
+    nodes[i].child0 = left;
+    nodes[i].child1 = right;
+    nodes[i].update = 0;
+    nodes[left].parent = i;
+    nodes[right].parent = i;
+    if (i == 0) nodes[0].parent = kInvalid;
+}
+
+// Bottom-up refit shared by BLAS build/update and TLAS build
+// (lbvh_fit_aabb_mesh.comp:122-205, lbvh_fit_aabb_scene.comp:98-167).  Each
+// finished node stores its box into its parent's child slot, then bumps the
+// parent's arrival counter (agent scope, acq_rel); an odd previous value
+// means the sibling already stored its box, so this thread merges both and
+// climbs on.
+__device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t child, uint32_t p,
+                                           float lo[3], float hi[3]) {
+    // Guards: a parent index outside the node array or more levels than any
+    // tree of `total` nodes can have stops the climb instead of faulting.
+    for (uint32_t level = 0; p != kInvalid && p < total && level < total; ++level) {
+        BvhNode* pn = nodes + p;
+        const uint32_t c0 = pn->child0;
+        float* slot_lo = c0 == child ? pn->a0_min_or_v0 : pn->a1_min_or_v2;
+        float* slot_hi = c0 == child ? pn->a0_max_or_v1 : pn->a1_max_or_v3;
+        float* other_lo = c0 == child ? pn->a1_min_or_v2 : pn->a0_min_or_v0;
+        float* other_hi = c0 == child ? pn->a1_max_or_v3 : pn->a0_max_or_v1;
+        slot_lo[0] = lo[0]; slot_lo[1] = lo[1]; slot_lo[2] = lo[2];
+        slot_hi[0] = hi[0]; slot_hi[1] = hi[1]; slot_hi[2] = hi[2];
+        const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 1u) == 0) return;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], other_lo[a]);
+            hi[a] = fmaxf(hi[a], other_hi[a]);
+        }
+        child = p;
+        p = pn->parent;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void RefitMeshKernel(MeshView m, BvhNode* __restrict__ nodes) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = m.tri_count;
+    if (i >= n) return;
+    const uint32_t leaf = n - 1 + i;
+    const uint32_t prim = nodes[leaf].child1;
+    const uint32_t parent = nodes[leaf].parent;
+    float v[9];
+    LoadTriangle(m, prim, v);
+    float4* q = reinterpret_cast<float4*>(nodes + leaf);
+    q[0] = make_float4(v[0], v[1], v[2], __uint_as_float(kInvalid));
+    q[1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
+    q[2] = make_float4(v[6], v[7], v[8], __uint_as_float(parent));
+    q[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+        hi[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+    }
+    ClimbRefit(nodes, 2 * n - 1, leaf, parent, lo, hi);
+}
+
+
+// ---------------------------------------------------------------------------
+// Treelet restructuring (Karras & Aila, "Fast Parallel Construction of
+// High-Quality Bounding Volume Hierarchies", HPG 2013), following
+// init_primitive_count.comp, find_treelet_roots.comp and
+// restructure_bvh.comp:149-542 with restructure_hlbvh.cpp:133-278.
+
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletMasks = 1 << kTreeletLeaves;
+constexpr float kCostInternal = 1.2f;  // C_INT, restructure_bvh.comp:38
+
+__device__ __forceinline__ float SurfaceArea(const float lo[3], const float hi[3]) {
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    float s = ex * ez;
+    s = s + ey * ex;
+    s = s + ez * ey;
+    return 2.0f * s;
+}
+
+// Box of a node as the reference's GetNodeAabb (restructure_bvh.comp:71-89):
+// the four child-box corners of an internal node, the three vertices of a leaf.
+__device__ __forceinline__ void NodeBox(const BvhNode* __restrict__ nodes, uint32_t idx, uint32_t n_internal, float lo[3],
+                                        float hi[3]) {
+    const BvhNode& nd = nodes[idx];
+    for (int a = 0; a < 3; ++a) {
+        float l = fminf(fminf(nd.a0_min_or_v0[a], nd.a0_max_or_v1[a]), nd.a1_min_or_v2[a]);
+        float h = fmaxf(fmaxf(nd.a0_min_or_v0[a], nd.a0_max_or_v1[a]), nd.a1_min_or_v2[a]);
+        if (idx < n_internal) {
+            l = fminf(l, nd.a1_max_or_v3[a]);
+            h = fmaxf(h, nd.a1_max_or_v3[a]);
+        }
+        lo[a] = l;
+        hi[a] = h;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void InitTreeletCountsKernel(uint32_t n, uint32_t* counts, uint32_t* counter) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n - 1) counts[i] = 0;
+    if (i == 0) *counter = 0;
+}
+
+// Climb from each leaf, accumulating primitive counts; the first node whose
+// subtree reaches `min_prims` becomes a treelet root (find_treelet_roots.comp:63-98).
+__global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* __restrict__ nodes, uint32_t n,
+                                                                 uint32_t min_prims, uint32_t* counts, uint32_t* counter,
+                                                                 uint32_t* roots) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t count = 1;
+    uint32_t idx = nodes[n - 1 + i].parent;
+    const uint32_t total = 2 * n - 1;
+    for (uint32_t level = 0; idx < total && level < total; ++level) {
+        const uint32_t old = __hip_atomic_exchange(&counts[idx], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        count += old;
+        if (old == 0) return;
+        if (count >= min_prims) {
+            roots[atomicAdd(counter, 1u)] = idx;
+            return;
+        }
+        idx = nodes[idx].parent;
+    }
+}
+
+// One wave per treelet root.  Lane 0 does the pointer-chasing parts (forming
+// the treelet, rewiring it, climbing); the 128 subset areas and the dynamic
+// programme over subsets run across the wave.
+__global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, uint32_t n, const uint32_t* counter,
+                                                                const uint32_t* __restrict__ roots, uint32_t* counts) {
+    if (blockIdx.x >= *counter) return;
+    __shared__ uint32_t s_leaf[kTreeletLeaves];
+    __shared__ uint32_t s_inner[kTreeletLeaves - 1];
+    __shared__ float s_box[kTreeletLeaves][6];
+    __shared__ float s_area[kTreeletMasks];
+    __shared__ float s_cost[kTreeletMasks];
+    __shared__ uint32_t s_part[kTreeletMasks];
+    __shared__ uint32_t s_root;
+    __shared__ uint32_t s_go;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n_int = n - 1;
+    if (lane == 0) s_root = roots[blockIdx.x];
+    __syncthreads();
+    // Each pass moves one level up, so a tree of n leaves needs fewer than n passes.
+    for (uint32_t pass = 0; pass < n; ++pass) {
+        // 1. Form a treelet: repeatedly open the treelet leaf with the largest
+        //    surface area (restructure_bvh.comp:170-218).
+        if (lane == 0) {
+            const uint32_t root = s_root;
+            float area[kTreeletLeaves];
+            float lo[3], hi[3];
+            s_inner[0] = root;
+            s_leaf[0] = nodes[root].child0;
+            s_leaf[1] = nodes[root].child1;
+            bool ok = s_leaf[0] < 2 * n - 1 && s_leaf[1] < 2 * n - 1;
+            for (int k = 0; k < 2 && ok; ++k) {
+                NodeBox(nodes, s_leaf[k], n_int, lo, hi);
+                area[k] = SurfaceArea(lo, hi);
+            }
+            for (int size = 2; ok && size < kTreeletLeaves; ++size) {
+                float largest = 0.0f;
+                int sel = -1;
+                for (int k = 0; k < size; ++k) {
+                    if (s_leaf[k] < n_int && (largest == 0.0f || area[k] > largest)) {
+                        largest = area[k];
+                        sel = k;
+                    }
+                }
+                if (sel < 0) { ok = false; break; }
+                const uint32_t opened = s_leaf[sel];
+                s_inner[size - 1] = opened;
+                const uint32_t c0 = nodes[opened].child0, c1 = nodes[opened].child1;
+                if (c0 >= 2 * n - 1 || c1 >= 2 * n - 1) { ok = false; break; }
+                s_leaf[sel] = c0;
+                s_leaf[size] = c1;
+                NodeBox(nodes, c0, n_int, lo, hi);
+                area[sel] = SurfaceArea(lo, hi);
+                NodeBox(nodes, c1, n_int, lo, hi);
+                area[size] = SurfaceArea(lo, hi);
+            }
+            if (ok) {
+                for (int k = 0; k < kTreeletLeaves; ++k) {
+                    NodeBox(nodes, s_leaf[k], n_int, lo, hi);
+                    for (int a = 0; a < 3; ++a) { s_box[k][a] = lo[a]; s_box[k][3 + a] = hi[a]; }
+                }
+            }
+            s_go = ok ? 1u : 0u;
+        }
+        __syncthreads();
+        if (s_go) {
+            // 2. Surface area of every subset of the 7 treelet leaves.
+            for (uint32_t m = lane; m < kTreeletMasks; m += 64) {
+                float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                for (int k = 0; k < kTreeletLeaves; ++k)
+                    if (m & (1u << k))
+                        for (int a = 0; a < 3; ++a) {
+                            lo[a] = fminf(lo[a], s_box[k][a]);
+                            hi[a] = fmaxf(hi[a], s_box[k][3 + a]);
+                        }
+                s_area[m] = m ? SurfaceArea(lo, hi) : 0.0f;
+                s_cost[m] = __popc(m) == 1 ? kCostInternal * s_area[m] : 0.0f;
+                s_part[m] = 0;
+            }
+            __syncthreads();
+            // 3. Optimal partition of each subset, by increasing size.  The
+            //    candidates of subset s are the non-empty subsets p of s
+            //    without its lowest leaf, in increasing order; the first
+            //    cheapest wins.  Orientation as restructure_bvh.comp:395-397
+            //    (sizes 2-5 flipped for even sizes, 6 and 7 unflipped).
+            for (int bits = 2; bits <= kTreeletLeaves; ++bits) {
+                for (uint32_t s = lane; s < kTreeletMasks; s += 64) {
+                    if (__popc(s) != bits) continue;
+                    const uint32_t delta = (s - 1) & s;
+                    uint32_t p = (0u - delta) & s;
+                    float best_cost = FLT_MAX;
+                    uint32_t best = 0;
+                    do {
+                        const float c = s_cost[p] + s_cost[s ^ p];
+                        if (best == 0 || c < best_cost) {
+                            best_cost = c;
+                            best = p;
+                        }
+                        p = (p - delta) & s;
+                    } while (p != 0);
+                    s_cost[s] = kCostInternal * s_area[s] + best_cost;
+                    s_part[s] = (bits <= 5 && (bits & 1) == 0) ? (s ^ best) : best;
+                }
+                __syncthreads();
+            }
+            // 4. Rewire the treelet top-down from the optimal partitions and
+            //    write every internal node's child boxes (the union of the
+            //    leaves on each side) (restructure_bvh.comp:418-494).
+            if (lane == 0) {
+                uint32_t stack_mask[kTreeletLeaves], stack_node[kTreeletLeaves];
+                int sp = 0, allocated = 1;
+                stack_mask[sp] = kTreeletMasks - 1;
+                stack_node[sp] = s_inner[0];
+                ++sp;
+                while (sp > 0) {
+                    --sp;
+                    const uint32_t m = stack_mask[sp], node = stack_node[sp];
+                    const uint32_t side[2] = {s_part[m], m ^ s_part[m]};
+                    uint32_t child[2];
+                    for (int c = 0; c < 2; ++c) {
+                        if (__popc(side[c]) > 1) {
+                            child[c] = s_inner[allocated++];
+                            stack_mask[sp] = side[c];
+                            stack_node[sp] = child[c];
+                            ++sp;
+                        } else {
+                            child[c] = s_leaf[__ffs(side[c]) - 1];
+                        }
+                    }
+                    float box[2][6];
+                    for (int c = 0; c < 2; ++c) {
+                        for (int a = 0; a < 3; ++a) { box[c][a] = FLT_MAX; box[c][3 + a] = -FLT_MAX; }
+                        for (int k = 0; k < kTreeletLeaves; ++k)
+                            if (side[c] & (1u << k))
+                                for (int a = 0; a < 3; ++a) {
+                                    box[c][a] = fminf(box[c][a], s_box[k][a]);
+                                    box[c][3 + a] = fmaxf(box[c][3 + a], s_box[k][3 + a]);
+                                }
+                    }
+                    BvhNode& nd = nodes[node];
+                    for (int a = 0; a < 3; ++a) {
+                        nd.a0_min_or_v0[a] = box[0][a];
+                        nd.a0_max_or_v1[a] = box[0][3 + a];
+                        nd.a1_min_or_v2[a] = box[1][a];
+                        nd.a1_max_or_v3[a] = box[1][3 + a];
+                    }
+                    nd.child0 = child[0];
+                    nd.child1 = child[1];
+                    nodes[child[0]].parent = node;
+                    nodes[child[1]].parent = node;
+                }
+            }
+        }
+        // 5. Climb: the last of the two subtrees to finish below the parent
+        //    refits it and restructures there next (restructure_bvh.comp:497-540).
+        if (lane == 0) {
+            const uint32_t parent = nodes[s_root].parent;
+            uint32_t go = 0;
+            if (parent < n_int) {
+                const uint32_t old =
+                    __hip_atomic_fetch_add(&counts[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                if (old != 0) {
+                    float lo[3], hi[3];
+                    BvhNode& pn = nodes[parent];
+                    NodeBox(nodes, pn.child0, n_int, lo, hi);
+                    for (int a = 0; a < 3; ++a) { pn.a0_min_or_v0[a] = lo[a]; pn.a0_max_or_v1[a] = hi[a]; }
+                    NodeBox(nodes, pn.child1, n_int, lo, hi);
+                    for (int a = 0; a < 3; ++a) { pn.a1_min_or_v2[a] = lo[a]; pn.a1_max_or_v3[a] = hi[a]; }
+                    s_root = parent;
+                    go = 1;
+                }
+            }
+            s_go = go;
+        }
+        __syncthreads();
+        if (!s_go) return;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// TLAS (lbvh_init_scene.comp, lbvh_calc_scene_aabb.comp:126-165,
+// lbvh_calc_morton_codes_scene.comp:83-114, lbvh_fit_aabb_scene.comp:98-167).
+
+// World boxes of instances are widened by this fraction of the magnitude of
+// the terms that produced them, so that rounding in the corner transforms
+// and in the ray's object-space transform can never cull a hit instance.
+constexpr float kInstanceBoxWiden = 1.52587890625e-05f;  // 2^-16
+
+struct TlasScratchLayout {
+    size_t upload, head, boxes, keys, vals, keys_tmp, vals_tmp, sort, total;
+};
+TlasScratchLayout LayoutTlas(uint32_t n) {
+    TlasScratchLayout l;
+    size_t o = 0;
+    l.upload = o; o += AlignUp(size_t(n) * sizeof(InstanceUpload), 256);
+    l.head = o; o += 256;
+    l.boxes = o; o += AlignUp(size_t(n) * 32, 256);
+    l.keys = o; o += AlignUp(size_t(n) * 4, 256);
+    l.vals = o; o += AlignUp(size_t(n) * 4, 256);
+    l.keys_tmp = o; o += AlignUp(size_t(n) * 4, 256);
+    l.vals_tmp = o; o += AlignUp(size_t(n) * 4, 256);
+    l.sort = o; o += SortScratchBytes(n);
+    l.total = o;
+    return l;
+}
+
+// Inverse of a 3x4 affine transform: adjugate of the 3x3 part over its
+// determinant, then the translation mapped back.  (The reference inverts the
+// padded 4x4 with GLSL inverse(), common.h:326-342.)
+__device__ __forceinline__ void InverseAffine(const float (*m)[4], float (*r)[4]) {
+    const float c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const float c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    const float c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const float det = (m[0][0] * c00 + m[0][1] * c01) + m[0][2] * c02;
+    const float inv = 1.0f / det;
+    r[0][0] = c00 * inv;
+    r[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv;
+    r[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv;
+    r[1][0] = c01 * inv;
+    r[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv;
+    r[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv;
+    r[2][0] = c02 * inv;
+    r[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv;
+    r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv;
+    for (int a = 0; a < 3; ++a) r[a][3] = -((r[a][0] * m[0][3] + r[a][1] * m[1][3]) + r[a][2] * m[2][3]);
+}
+
+// Box of a BLAS's root: empty for an empty BLAS, the triangle's box for a
+// one-triangle BLAS, else the union of the root's two child boxes.
+__device__ __forceinline__ bool BlasRootBox(const uint8_t* blas, float lo[3], float hi[3]) {
+    const AccelHeader* h = reinterpret_cast<const AccelHeader*>(blas);
+    if (h->node_count == 0) return false;
+    const BvhNode& r = *reinterpret_cast<const BvhNode*>(blas + h->nodes_offset);
+    for (int a = 0; a < 3; ++a) {
+        if (h->node_count == 1) {
+            lo[a] = fminf(fminf(r.a0_min_or_v0[a], r.a0_max_or_v1[a]), r.a1_min_or_v2[a]);
+            hi[a] = fmaxf(fmaxf(r.a0_min_or_v0[a], r.a0_max_or_v1[a]), r.a1_min_or_v2[a]);
+        } else {
+            lo[a] = fminf(r.a0_min_or_v0[a], r.a1_min_or_v2[a]);
+            hi[a] = fmaxf(r.a0_max_or_v1[a], r.a1_max_or_v3[a]);
+        }
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void TlasInstanceKernel(const InstanceUpload* __restrict__ inst, uint32_t n,
+                                                             Transform* __restrict__ xforms, uint64_t* __restrict__ table,
+                                                             float* __restrict__ boxes, BuildHead* head) {
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const InstanceUpload u = inst[i];
+        Transform inv;
+        InverseAffine(u.xform.m, inv.m);
+        xforms[2 * (size_t)i] = inv;
+        xforms[2 * (size_t)i + 1] = u.xform;
+        table[i] = u.blas;
+        float blo[3], bhi[3];
+        float wlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, whi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (BlasRootBox(reinterpret_cast<const uint8_t*>(u.blas), blo, bhi)) {
+            float mag[3] = {0.0f, 0.0f, 0.0f};
+            for (int c = 0; c < 8; ++c) {
+                const float x = (c & 4) ? bhi[0] : blo[0];
+                const float y = (c & 2) ? bhi[1] : blo[1];
+                const float z = (c & 1) ? bhi[2] : blo[2];
+                for (int a = 0; a < 3; ++a) {
+                    const float* m = u.xform.m[a];
+                    const float p = ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
+                    const float g = ((fabsf(m[0] * x) + fabsf(m[1] * y)) + fabsf(m[2] * z)) + fabsf(m[3]);
+                    wlo[a] = fminf(wlo[a], p);
+                    whi[a] = fmaxf(whi[a], p);
+                    mag[a] = fmaxf(mag[a], g);
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                const float e = mag[a] * kInstanceBoxWiden;
+                wlo[a] = wlo[a] - e;
+                whi[a] = whi[a] + e;
+                mn[a] = FloatToOrdered(wlo[a]);
+                mx[a] = FloatToOrdered(whi[a]);
+            }
+        }
+        float4* b = reinterpret_cast<float4*>(boxes) + 2 * (size_t)i;
+        b[0] = make_float4(wlo[0], wlo[1], wlo[2], 0.0f);
+        b[1] = make_float4(whi[0], whi[1], whi[2], 0.0f);
+    }
+    ReduceBoundsToGlobal(mn, mx, head->bounds);
+}
+
+__global__ __launch_bounds__(kBlock) void TlasMortonKernel(const float* __restrict__ boxes, uint32_t n,
+                                                           const BuildHead* head, uint32_t* keys, uint32_t* vals,
+                                                           int passes, uint32_t* sort_hist, uint32_t* sort_status,
+                                                           size_t status_words) {
+    auto centroid = [&](uint32_t t, float c[3]) {
+        const float4 lo = reinterpret_cast<const float4*>(boxes)[2 * (size_t)t];
+        const float4 hi = reinterpret_cast<const float4*>(boxes)[2 * (size_t)t + 1];
+        c[0] = (lo.x + hi.x) * 0.5f;
+        c[1] = (lo.y + hi.y) * 0.5f;
+        c[2] = (lo.z + hi.z) * 0.5f;
+    };
+    MortonAndHistogram(n, head, centroid, keys, vals, passes, sort_hist, sort_status, status_words);
+}
+
+__global__ __launch_bounds__(kBlock) void RefitSceneKernel(const float* __restrict__ boxes, uint32_t n,
+                                                           BvhNode* __restrict__ nodes) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t leaf = n - 1 + i;
+    const uint32_t id = nodes[leaf].child1;
+    const uint32_t parent = nodes[leaf].parent;
+    const float4 blo = reinterpret_cast<const float4*>(boxes)[2 * (size_t)id];
+    const float4 bhi = reinterpret_cast<const float4*>(boxes)[2 * (size_t)id + 1];
+    // A TLAS leaf stores the instance's world box twice (lbvh_fit_aabb_scene.comp:125-128).
+    float4* q = reinterpret_cast<float4*>(nodes + leaf);
+    q[0] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(kInvalid));
+    q[1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(id));
+    q[2] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(parent));
+    q[3] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(0u));
+    float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
+    ClimbRefit(nodes, 2 * n - 1, leaf, parent, lo, hi);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+
+size_t BlasBuildScratchBytes(uint32_t tris, bool restructure) {
+    const size_t lbvh = LayoutBlas(tris).total;
+    const size_t rs = restructure ? LayoutRestructure(tris).total : 0;
+    return std::max(lbvh, rs);
+}
+
+void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* result, hipStream_t s) {
+    const uint32_t n = mesh.tri_count;
+    AccelHeader* hdr = static_cast<AccelHeader*>(result);
+    BvhNode* nodes = At<BvhNode>(result, sizeof(AccelHeader));
+    const BlasScratchLayout L = LayoutBlas(n);
+    BuildHead* head = At<BuildHead>(scratch, L.head);
+    const SortScratch sc = CarveSortScratch(At<void>(scratch, L.sort), n);
+    const bool do_restructure = restructure && n >= 64;
+    InitBuildKernel<<<1, kBlock, 0, s>>>(hdr, kAccelBlas, n, sizeof(AccelHeader), 0, 0,
+                                         do_restructure ? kAccelFlagRestructured : 0u, head, sc.hist, sc.tickets);
+    if (n == 0) return;
+    const MeshView m = ViewOf(mesh);
+    MeshBoundsKernel<<<std::min(kMaxGrid, Blocks(n)), kBlock, 0, s>>>(m, head);
+    uint32_t* keys = At<uint32_t>(scratch, L.keys);
+    uint32_t* vals = At<uint32_t>(scratch, L.vals);
+    const int passes = SortPasses(30);
+    MeshMortonKernel<<<Blocks(n), kBlock, 0, s>>>(m, head, keys, vals, passes, sc.hist, sc.status, sc.status_words);
+    const bool in_place = RadixSortPairs(keys, vals, At<uint32_t>(scratch, L.keys_tmp), At<uint32_t>(scratch, L.vals_tmp),
+                                         n, 30, sc, true, s);
+    const uint32_t* sk = in_place ? keys : At<uint32_t>(scratch, L.keys_tmp);
+    const uint32_t* sv = in_place ? vals : At<uint32_t>(scratch, L.vals_tmp);
+    EmitHierarchyKernel<<<Blocks(n), kBlock, 0, s>>>(sk, sv, n, nodes);
+    RefitMeshKernel<<<Blocks(n), kBlock, 0, s>>>(m, nodes);
+    if (do_restructure) RestructureBlas(n, scratch, result, s);
+}
+
+void UpdateBlas(const MeshDesc& mesh, void* result, hipStream_t s) {
+    const uint32_t n = mesh.tri_count;
+    if (n == 0) return;
+    BvhNode* nodes = At<BvhNode>(result, sizeof(AccelHeader));
+    RefitMeshKernel<<<Blocks(n), kBlock, 0, s>>>(ViewOf(mesh), nodes);
+}
+
+void RestructureBlas(uint32_t n, void* scratch, void* result, hipStream_t s) {
+    if (n < 64) return;
+    BvhNode* nodes = At<BvhNode>(result, sizeof(AccelHeader));
+    const RestructureLayout L = LayoutRestructure(n);
+    uint32_t* counter = At<uint32_t>(scratch, L.counter);
+    uint32_t* roots = At<uint32_t>(scratch, L.roots);
+    uint32_t* counts = At<uint32_t>(scratch, L.counts);
+    uint32_t min_prims = 64;  // restructure_hlbvh.cpp:34-35,151,276: 64, 128, 256
+    for (int round = 0; round < 3; ++round, min_prims *= 2) {
+        InitTreeletCountsKernel<<<Blocks(n), kBlock, 0, s>>>(n, counts, counter);
+        FindTreeletRootsKernel<<<Blocks(n), kBlock, 0, s>>>(nodes, n, min_prims, counts, counter, roots);
+        RestructureTreeletsKernel<<<Blocks(n, (int)min_prims), 64, 0, s>>>(nodes, n, counter, roots, counts);
+    }
+}
+
+size_t TlasBuildScratchBytes(uint32_t instances) { return LayoutTlas(instances).total; }
+
+void BuildTlas(const InstanceUpload* instances, uint32_t n, void* scratch, void* result, hipStream_t s) {
+    AccelHeader* hdr = static_cast<AccelHeader*>(result);
+    const uint64_t nodes_off = sizeof(AccelHeader);
+    const uint64_t xf_off = nodes_off + TlasNodesBytes(n);
+    const uint64_t table_off = xf_off + TlasTransformsBytes(n);
+    BvhNode* nodes = At<BvhNode>(result, nodes_off);
+    const TlasScratchLayout L = LayoutTlas(n);
+    BuildHead* head = At<BuildHead>(scratch, L.head);
+    const SortScratch sc = CarveSortScratch(At<void>(scratch, L.sort), n);
+    InitBuildKernel<<<1, kBlock, 0, s>>>(hdr, kAccelTlas, n, nodes_off, xf_off, table_off, 0u, head, sc.hist, sc.tickets);
+    if (n == 0) return;
+    float* boxes = At<float>(scratch, L.boxes);
+    TlasInstanceKernel<<<Blocks(n), kBlock, 0, s>>>(instances, n, At<Transform>(result, xf_off),
+                                                    At<uint64_t>(result, table_off), boxes, head);
+    uint32_t* keys = At<uint32_t>(scratch, L.keys);
+    uint32_t* vals = At<uint32_t>(scratch, L.vals);
+    TlasMortonKernel<<<Blocks(n), kBlock, 0, s>>>(boxes, n, head, keys, vals, SortPasses(30), sc.hist, sc.status,
+                                                  sc.status_words);
+    const bool in_place = RadixSortPairs(keys, vals, At<uint32_t>(scratch, L.keys_tmp), At<uint32_t>(scratch, L.vals_tmp),
+                                         n, 30, sc, true, s);
+    const uint32_t* sk = in_place ? keys : At<uint32_t>(scratch, L.keys_tmp);
+    const uint32_t* sv = in_place ? vals : At<uint32_t>(scratch, L.vals_tmp);
+    EmitHierarchyKernel<<<Blocks(n), kBlock, 0, s>>>(sk, sv, n, nodes);
+    RefitSceneKernel<<<Blocks(n), kBlock, 0, s>>>(boxes, n, nodes);
+}
+
+}  // namespace rr

@@ -1,0 +1,192 @@
+// rr_device.h — device-side arithmetic shared by the build and trace kernels.
+//
+// Every floating-point expression here is written out operation by
+// operation and the library is compiled with -ffp-contract=off, so results
+// are a fixed function of the inputs (IEEE-754 binary32, round to nearest),
+// reproducible bit for bit by a CPU that evaluates the same sequence.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rr_layout.h"
+
+namespace rr {
+
+#define RR_DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// Order-preserving float <-> uint mapping used for atomic min/max of box
+// coordinates (vlk/kernels/common.h:68-90): positive floats get the sign bit
+// set, negative floats are bit-inverted, so unsigned order == float order.
+RR_DEV uint32_t FloatToOrdered(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+RR_DEV float OrderedToFloat(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// ---------------------------------------------------------------------------
+// 30-bit Morton code (vlk/kernels/common.h:251-268): each coordinate in
+// [0,1] is scaled by 1024, clamped to [0,1023], truncated, and its 10 bits
+// interleaved as x2 y1 z0.
+RR_DEV uint32_t SpreadBits10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+RR_DEV uint32_t QuantizeUnit(float p) {
+    float q = p * 1024.0f;
+    q = fminf(fmaxf(q, 0.0f), 1023.0f);
+    return (uint32_t)q;
+}
+RR_DEV uint32_t Morton30(float px, float py, float pz) {
+    return (SpreadBits10(QuantizeUnit(px)) << 2) | (SpreadBits10(QuantizeUnit(py)) << 1) |
+           SpreadBits10(QuantizeUnit(pz));
+}
+// Position of centroid c inside [lo, lo+extent]; a zero-extent axis maps to
+// 0 instead of the reference's 0/0 (lbvh_calc_morton_codes_mesh.comp:119-123).
+RR_DEV float UnitCoord(float c, float lo, float extent) { return extent > 0.0f ? (c - lo) / extent : 0.0f; }
+
+// ---------------------------------------------------------------------------
+// Ray set-up shared by the slab and triangle tests.
+struct RayState {
+    float ox, oy, oz;     // origin
+    float dx, dy, dz;     // direction
+    float tmin;           // lower bound of the accepted interval
+    float ix, iy, iz;     // reciprocal direction for slab tests
+    float sx, sy, sz;     // watertight shear constants
+    int   kx, ky, kz;     // watertight axis permutation
+};
+
+// Relative widening of slab intervals: slab distances (b - o) * (1/d) carry
+// at most ~3 ulp of relative error, so widening by 2^-21 keeps every box
+// whose exact interval is non-empty.
+constexpr float kSlabWiden = 4.76837158203125e-07f;   // 2^-21
+// Relative slack on the current closest distance when culling boxes, so a
+// candidate at exactly the closest distance (tie) or within the triangle
+// test's rounding of it is never culled.
+constexpr float kLimitWiden = 1.52587890625e-05f;     // 2^-16
+// |d| below this is replaced by +/-kTinyDir before taking the reciprocal
+// (vlk/kernels/common.h:172-183 uses 1e-5; a tiny clamp keeps the slab test
+// conservative for nearly axis-parallel rays).
+constexpr float kTinyDir = 1e-20f;
+
+RR_DEV float SafeRcp(float d) {
+    float a = fabsf(d) > kTinyDir ? d : (d < 0.0f ? -kTinyDir : kTinyDir);
+    return 1.0f / a;
+}
+
+RR_DEV float Perm(int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); }
+
+RR_DEV void SetupRay(RayState& r, float ox, float oy, float oz, float tmin, float dx, float dy, float dz) {
+    r.ox = ox; r.oy = oy; r.oz = oz; r.tmin = tmin;
+    r.dx = dx; r.dy = dy; r.dz = dz;
+    r.ix = SafeRcp(dx); r.iy = SafeRcp(dy); r.iz = SafeRcp(dz);
+    // Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013:
+    // kz = dimension of largest |d| (first on ties), kx/ky the next two in
+    // cyclic order, swapped when d[kz] < 0 to keep the winding.
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dkz = Perm(kz, dx, dy, dz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.sx = Perm(kx, dx, dy, dz) / dkz;
+    r.sy = Perm(ky, dx, dy, dz) / dkz;
+    r.sz = 1.0f / dkz;
+}
+
+// Conservative slab test of one box.  Returns the widened entry distance in
+// t0 and whether [t0, t1] is non-empty, where t1 is clipped to `limit`.
+RR_DEV bool SlabTest(const RayState& r, float limit, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+                     float& t0) {
+    float lx = (bx0 - r.ox) * r.ix, hx = (bx1 - r.ox) * r.ix;
+    float ly = (by0 - r.oy) * r.iy, hy = (by1 - r.oy) * r.iy;
+    float lz = (bz0 - r.oz) * r.iz, hz = (bz1 - r.oz) * r.iz;
+    float nx = fminf(lx, hx), fx = fmaxf(lx, hx);
+    float ny = fminf(ly, hy), fy = fmaxf(ly, hy);
+    float nz = fminf(lz, hz), fz = fmaxf(lz, hz);
+    float tn = fmaxf(fmaxf(nx, ny), nz);
+    float tf = fminf(fminf(fx, fy), fz);
+    tn = fmaf(-kSlabWiden, fabsf(tn), tn);
+    tf = fmaf(kSlabWiden, fabsf(tf), tf);
+    float a = fmaxf(tn, r.tmin);
+    float b = fminf(tf, limit);
+    t0 = a;
+    return a <= b;
+}
+
+RR_DEV float WidenLimit(float t) { return fmaf(kLimitWiden, fabsf(t), t); }
+
+// Watertight ray/triangle test (Woop et al. 2013, with the paper's double
+// precision fallback for zero edge functions, no backface culling).  On a
+// hit returns true with the barycentric numerators and denominator: the hit
+// is u = V/det (weight of v1), v = W/det (weight of v2), t = T/det.
+struct TriHit {
+    float V, W, det, t;
+};
+
+RR_DEV bool WatertightTest(const RayState& r, float v0x, float v0y, float v0z, float v1x, float v1y, float v1z,
+                           float v2x, float v2y, float v2z, TriHit& h) {
+    const float Ax0 = v0x - r.ox, Ay0 = v0y - r.oy, Az0 = v0z - r.oz;
+    const float Ax1 = v1x - r.ox, Ay1 = v1y - r.oy, Az1 = v1z - r.oz;
+    const float Ax2 = v2x - r.ox, Ay2 = v2y - r.oy, Az2 = v2z - r.oz;
+    const float akx = Perm(r.kx, Ax0, Ay0, Az0), aky = Perm(r.ky, Ax0, Ay0, Az0), akz = Perm(r.kz, Ax0, Ay0, Az0);
+    const float bkx = Perm(r.kx, Ax1, Ay1, Az1), bky = Perm(r.ky, Ax1, Ay1, Az1), bkz = Perm(r.kz, Ax1, Ay1, Az1);
+    const float ckx = Perm(r.kx, Ax2, Ay2, Az2), cky = Perm(r.ky, Ax2, Ay2, Az2), ckz = Perm(r.kz, Ax2, Ay2, Az2);
+    const float Ax = akx - r.sx * akz, Ay = aky - r.sy * akz;
+    const float Bx = bkx - r.sx * bkz, By = bky - r.sy * bkz;
+    const float Cx = ckx - r.sx * ckz, Cy = cky - r.sy * ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.sz * akz, Bz = r.sz * bkz, Cz = r.sz * ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    h.V = V; h.W = W; h.det = det;
+    h.t = T / det;
+    return true;
+}
+
+// Closest-hit ordering: (t, inst, prim) lexicographic, so equal distances
+// resolve to the lowest ids whatever order the BVH visits them in.
+RR_DEV bool CloserHit(float t, uint32_t inst, uint32_t prim, float bt, uint32_t binst, uint32_t bprim) {
+    return t < bt || (t == bt && (inst < binst || (inst == binst && prim < bprim)));
+}
+
+// ---------------------------------------------------------------------------
+// 64-bit wave helpers (wave64 on CDNA).
+RR_DEV uint64_t LaneMaskLt() {
+    const uint32_t lane = __lane_id();
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+template <typename T>
+RR_DEV T WaveMin(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <typename T>
+RR_DEV T WaveMax(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Node loads as four 16-byte vector loads.
+RR_DEV void LoadNode(const BvhNode* __restrict__ nodes, uint32_t i, float4& q0, float4& q1, float4& q2, float4& q3) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
+}
+
+}  // namespace rr

@@ -1,0 +1,70 @@
+// sort.h — host interface of the onesweep LSD radix sort of (uint32 key,
+// uint32 value) pairs.  Replaces the reference's 8-pass 4-bit sort
+// (vlk/radix_sort.cpp:157-338 with radix_sort_group_histograms.comp,
+// scan_exclusive_add*.comp and radix_sort_scatter_keys.comp): one upfront
+// histogram of every digit, then one kernel per 8-bit digit that ranks a
+// tile in LDS, finds its global offsets by decoupled look-back and scatters.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace rr {
+
+constexpr int kSortRadixBits = 8;
+constexpr int kSortRadix = 1 << kSortRadixBits;
+constexpr int kSortMaxPasses = 4;
+
+// Tile shape of one pass kernel.  Small inputs use small tiles so that the
+// grid still spreads over all CUs.
+struct SortShape {
+    int block;   // threads per tile
+    int items;   // keys per thread
+    int tile() const { return block * items; }
+};
+SortShape ChooseSortShape(uint32_t n);
+
+// Scratch needed to sort n pairs: histograms, tile tickets and look-back
+// status words.  Everything in it is (re)initialised by the sort itself.
+size_t SortScratchBytes(uint32_t n);
+
+// Number of 8-bit passes for keys whose bits above key_bits are zero.
+inline int SortPasses(int key_bits) { return (key_bits + kSortRadixBits - 1) / kSortRadixBits; }
+
+struct SortScratch {
+    uint32_t* hist;     // [kSortMaxPasses][256] global digit counts
+    uint32_t* tickets;  // [kSortMaxPasses] tile tickets
+    uint32_t* status;   // [kSortMaxPasses][tiles][256] look-back words
+    size_t status_words;
+};
+SortScratch CarveSortScratch(void* base, uint32_t n);
+
+// Sort keys/vals in place (ping-ponging through keys_tmp/vals_tmp) by the
+// low key_bits bits; stable.  When hist_ready is true the caller has already
+// cleared hist/tickets, accumulated the digit histograms into hist and
+// cleared the status words (the build fuses that into its Morton kernel).
+// Returns true if the result ended in keys/vals, false if in the tmp arrays.
+bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
+                    const SortScratch& sc, bool hist_ready, hipStream_t stream);
+
+// Launch helpers exposed for the build's fused histogram.
+void LaunchSortClear(const SortScratch& sc, hipStream_t stream);
+
+// Accumulate the 8-bit digits of `key` for `passes` passes into LDS
+// histograms.  A wave whose valid keys all share a digit adds once.
+__device__ __forceinline__ void AccumulateDigits(uint32_t key, bool valid, uint32_t (*s_h)[kSortRadix], int passes) {
+    const uint64_t act = __ballot(valid);
+    const uint32_t lane = __lane_id();
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t d = (key >> (p * kSortRadixBits)) & (kSortRadix - 1);
+        const uint32_t d0 = __shfl(d, 0, 64);
+        const bool same = __all(!valid || d == d0);
+        if (same) {
+            if (lane == 0 && valid) atomicAdd(&s_h[p][d0], (uint32_t)__popcll(act));
+        } else if (valid) {
+            atomicAdd(&s_h[p][d], 1u);
+        }
+    }
+}
+
+}  // namespace rr

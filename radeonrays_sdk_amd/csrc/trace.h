@@ -1,0 +1,33 @@
+// trace.h — host interface of the ray traversal kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rr_layout.h"
+
+namespace rr {
+
+// Per-lane traversal stack: kLdsStackEntries in LDS, kSpillEntries more in
+// the trace scratch buffer (the reference keeps 16 in LDS and 64 per ray in
+// scratch, vlk/kernels/isect.comp:26-27, geometry_trace.cpp:169).
+constexpr int kLdsStackEntries = 16;
+constexpr int kSpillEntries = 64;
+
+size_t TraceScratchBytes(uint32_t ray_count);
+
+enum QueryKind : int { kQueryClosest = 0, kQueryAny = 1 };
+enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };
+
+// Trace ray_count rays (or min(ray_count, *indirect) when indirect is not
+// null) against the BLAS or TLAS at `accel`.
+void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output, const void* rays, uint32_t ray_count,
+           const uint32_t* indirect, void* hits, void* scratch, hipStream_t stream);
+
+// Same traversal with per-ray visit counters written to `stats`
+// (uint32 x4 per ray: internal nodes visited, leaves visited, box tests,
+// triangle tests), for the roofline's algorithmic byte count.
+void TraceWithStats(const void* accel, AccelKind kind, QueryKind query, const void* rays, uint32_t ray_count, void* hits,
+                    void* scratch, uint32_t* stats, hipStream_t stream);
+
+}  // namespace rr

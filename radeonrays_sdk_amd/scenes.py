@@ -1,0 +1,170 @@
+"""Seeded synthetic workloads for the build-and-intersect path.
+
+Each generator returns numpy arrays (float32 vertices [V, 3], int32 triangle
+indices [T, 3], float32 rays [R, 8] laid out as RRRay: origin, min_t,
+direction, max_t).  Seeds are 0x52520000 + workload index; randomness comes
+from numpy's PCG64 generator, normals in float64 rounded to float32.
+
+Workloads (the survey's configs 0-4):
+  soup_camera      65,536-triangle soup, 1024x1024 pinhole camera rays
+  heightfield      724x724-vertex fBm heightfield (1,045,458 triangles),
+                   4,194,304 rays from a slab above it, uniform directions
+  instanced        64 soups x 4,096 instances, secondary rays
+  batched          8,192 small soups (batched build/refit)
+  serving          32 resident scenes and a Poisson request stream
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+SEED_BASE = 0x52520000
+
+
+def rng_for(workload: int, stream: int = 0) -> np.random.Generator:
+    return np.random.default_rng([SEED_BASE + workload, stream])
+
+
+# --- geometry ----------------------------------------------------------------
+def triangle_soup(n_tris: int, rng: np.random.Generator, sigma: float = 0.01):
+    """Centroids uniform in [0,1]^3, each vertex = centroid + N(0, sigma^2) per axis."""
+    c = rng.uniform(0.0, 1.0, size=(n_tris, 1, 3))
+    v = (c + rng.normal(0.0, sigma, size=(n_tris, 3, 3))).astype(np.float32).reshape(-1, 3)
+    idx = np.arange(3 * n_tris, dtype=np.int32).reshape(n_tris, 3)
+    return v, idx
+
+
+def _hash2(ix, iz, salt):
+    """Integer lattice hash -> uniform float in [0, 1)."""
+    h = (ix.astype(np.uint32) * np.uint32(0x8DA6B343)) ^ (iz.astype(np.uint32) * np.uint32(0xD8163841)) ^ np.uint32(salt)
+    h ^= h >> np.uint32(16)
+    h *= np.uint32(0x7FEB352D)
+    h ^= h >> np.uint32(15)
+    h *= np.uint32(0x846CA68B)
+    h ^= h >> np.uint32(16)
+    return (h >> np.uint32(8)).astype(np.float64) / float(1 << 24)
+
+
+def fbm_heights(res: int, seed: int, octaves: int = 6, base_freq: float = 4.0, lo: float = 0.0, hi: float = 0.2):
+    """fBm value noise on a res x res grid over [0,1]^2: lattice hash values,
+    smoothstep interpolation, lacunarity 2, gain 0.5, rescaled to [lo, hi]."""
+    x = np.linspace(0.0, 1.0, res)
+    X, Z = np.meshgrid(x, x, indexing="xy")
+    total = np.zeros_like(X)
+    amp, freq = 1.0, base_freq
+    with np.errstate(over="ignore"):
+        for o in range(octaves):
+            fx, fz = X * freq, Z * freq
+            ix, iz = np.floor(fx).astype(np.int64), np.floor(fz).astype(np.int64)
+            tx, tz = fx - ix, fz - iz
+            sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+            salt = (seed * 0x9E3779B1 + o * 0x85EBCA77) & 0xFFFFFFFF
+            v00, v10 = _hash2(ix, iz, salt), _hash2(ix + 1, iz, salt)
+            v01, v11 = _hash2(ix, iz + 1, salt), _hash2(ix + 1, iz + 1, salt)
+            total += amp * ((v00 * (1 - sx) + v10 * sx) * (1 - sz) + (v01 * (1 - sx) + v11 * sx) * sz)
+            amp *= 0.5
+            freq *= 2.0
+    t = (total - total.min()) / max(total.max() - total.min(), 1e-12)
+    return lo + (hi - lo) * t
+
+
+def heightfield_mesh(res: int = 724, seed: int = SEED_BASE + 1):
+    """res x res vertices on [0,1]^2 in x,z with fBm heights y; 2 triangles per cell."""
+    h = fbm_heights(res, seed)
+    x = np.linspace(0.0, 1.0, res)
+    X, Z = np.meshgrid(x, x, indexing="xy")
+    v = np.stack([X, h, Z], axis=-1).astype(np.float32).reshape(-1, 3)
+    i = np.arange(res - 1)
+    I, J = np.meshgrid(i, i, indexing="xy")  # I along x, J along z
+    a = (J * res + I).ravel()
+    b = a + 1
+    c = a + res
+    d = c + 1
+    tris = np.empty((2 * a.size, 3), dtype=np.int32)
+    tris[0::2] = np.stack([a, b, c], axis=1)
+    tris[1::2] = np.stack([b, d, c], axis=1)
+    return v, tris
+
+
+# --- rays --------------------------------------------------------------------
+def make_rays(origins, dirs, tmin=0.0, tmax=1e30):
+    n = origins.shape[0]
+    r = np.empty((n, 8), dtype=np.float32)
+    r[:, 0:3] = origins
+    r[:, 3] = tmin
+    r[:, 4:7] = dirs
+    r[:, 7] = tmax
+    return r
+
+
+def unit_sphere(n, rng):
+    d = rng.normal(size=(n, 3))
+    return (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+
+
+def camera_rays(width, height, eye, fov_deg=45.0, tmin=0.0, tmax=1e30):
+    """Pinhole camera at `eye` looking along +z with +y up, vertical FOV fov_deg, pixel centres."""
+    t = math.tan(math.radians(fov_deg) * 0.5)
+    aspect = width / height
+    px = (np.arange(width) + 0.5) / width * 2 - 1
+    py = 1 - (np.arange(height) + 0.5) / height * 2
+    PX, PY = np.meshgrid(px * t * aspect, py * t, indexing="xy")
+    d = np.stack([PX, PY, np.ones_like(PX)], axis=-1).reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.broadcast_to(np.asarray(eye, dtype=np.float64), d.shape)
+    return make_rays(o.astype(np.float32), d.astype(np.float32), tmin, tmax)
+
+
+# --- workloads ---------------------------------------------------------------
+@dataclass
+class MeshWorkload:
+    name: str
+    vertices: np.ndarray
+    indices: np.ndarray
+    rays: np.ndarray
+
+
+def soup_camera(n_tris: int = 65536, width: int = 1024, height: int = 1024) -> MeshWorkload:
+    """Config 0: triangle soup, primary rays from (0.5, 0.5, -1.5) along +z, 45 deg FOV, t in [0, 1e30]."""
+    rng = rng_for(0)
+    v, i = triangle_soup(n_tris, rng)
+    rays = camera_rays(width, height, (0.5, 0.5, -1.5))
+    return MeshWorkload("soup_camera", v, i, rays)
+
+
+def heightfield(res: int = 724, n_rays: int = 4 * 1024 * 1024) -> MeshWorkload:
+    """Config 1: fBm heightfield; origins U([0,1]x[0.25,0.5]x[0,1]), directions uniform on S^2."""
+    v, i = heightfield_mesh(res)
+    rng = rng_for(1)
+    o = rng.uniform([0.0, 0.25, 0.0], [1.0, 0.5, 1.0], size=(n_rays, 3)).astype(np.float32)
+    rays = make_rays(o, unit_sphere(n_rays, rng))
+    return MeshWorkload("heightfield", v, i, rays)
+
+
+def perturbed_heights(vertices: np.ndarray, frame: int, sigma: float = 1e-3) -> np.ndarray:
+    """Refit frame `frame`: every vertex height = base + N(0, sigma^2), fresh per frame."""
+    rng = rng_for(1, 1000 + frame)
+    out = vertices.copy()
+    out[:, 1] = (vertices[:, 1].astype(np.float64) + rng.normal(0.0, sigma, size=vertices.shape[0])).astype(np.float32)
+    return out
+
+
+def log_uniform_ints(rng, n, lo, hi):
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), size=n)).astype(np.int64)
+
+
+def random_transforms(rng, n, scale=(0.5, 2.0), trans=50.0):
+    """Uniform rotation (Shoemake quaternions), uniform scale, translation U[-trans, trans]^3 -> [n, 3, 4]."""
+    u1, u2, u3 = rng.uniform(size=(3, n))
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    x, y, z, w = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], axis=1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], axis=1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], axis=1)], axis=1)
+    s = rng.uniform(scale[0], scale[1], size=(n, 1, 1))
+    t = rng.uniform(-trans, trans, size=(n, 3, 1))
+    return np.concatenate([R * s, t], axis=2).astype(np.float32)

