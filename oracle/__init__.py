@@ -45,6 +45,7 @@ def load():
             "orc_trace": ([P, P, P, c_uint32, c_int, P, P, c_int], None),
             "orc_brute_force": ([P, P, P, c_uint32, P, c_int], None),
             "orc_triangle_fp64": ([P, P, P, P, c_double, POINTER(c_double), POINTER(c_double), POINTER(c_double)], c_int),
+            "orc_treelet_dp": ([P, P, P], None),
             "orc_is_valid": ([P], c_int),
             "orc_check_consistency": ([P], c_int),
             "orc_sah": ([P], c_float),
@@ -57,6 +58,24 @@ def load():
             f.restype = res
         _lib = lib
     return _lib
+
+
+def check_accel(buf: np.ndarray) -> None:
+    """Raise unless `buf` holds a well-formed acceleration-structure header
+    whose node array (and TLAS tables) lie inside the buffer."""
+    if buf.dtype != np.uint8 or buf.size < 128:
+        raise ValueError("accel buffer must be uint8 with a 128-byte header")
+    hdr = buf[:128].view(np.uint32)
+    magic, kind, prims, count = (int(x) for x in hdr[:4])
+    offs = buf[16:40].view(np.uint64)
+    if magic != 0x53415252 or kind not in (1, 2):
+        raise ValueError(f"bad header magic/kind {magic:#x}/{kind}")
+    if count != (2 * prims - 1 if prims else 0):
+        raise ValueError(f"node_count {count} inconsistent with prim_count {prims}")
+    if int(offs[0]) + 64 * count > buf.size:
+        raise ValueError("node array runs past the buffer")
+    if kind == 2 and (int(offs[1]) + 96 * prims > buf.size or int(offs[2]) + 8 * prims > buf.size):
+        raise ValueError("TLAS tables run past the buffer")
 
 
 def _p(a):
@@ -135,13 +154,16 @@ def _blas_ptrs(accel: np.ndarray, blas_map):
     if hdr[1] != 2:
         return None, ()
     n = int(hdr[2])
-    off = int(accel[40:48].view(np.uint64)[0])
+    off = int(accel[32:40].view(np.uint64)[0])  # blas_table_offset
     addrs = accel[off:off + 8 * n].view(np.uint64)
     bufs = [blas_map[int(a)] for a in addrs]
+    for b in bufs:
+        check_accel(b)
     return (c_void_p * max(n, 1))(*[b.ctypes.data for b in bufs]), bufs
 
 
 def trace(accel: np.ndarray, rays, any_hit=False, blas_map=None, stats=False, threads=0):
+    check_accel(accel)
     rays = _c(rays, np.float32)
     n = rays.shape[0]
     hits = np.zeros(n, HIT_DTYPE)
@@ -152,6 +174,7 @@ def trace(accel: np.ndarray, rays, any_hit=False, blas_map=None, stats=False, th
 
 
 def brute_force(accel: np.ndarray, rays, blas_map=None, threads=0):
+    check_accel(accel)
     rays = _c(rays, np.float32)
     hits = np.zeros(rays.shape[0], HIT_DTYPE)
     ptrs, keep = _blas_ptrs(accel, blas_map or {})
@@ -166,19 +189,31 @@ def triangle_fp64(ray, v0, v1, v2, eps=1e-9):
     return bool(ok), t.value, u.value, v.value
 
 
+def treelet_dp(boxes):
+    """(cost[128], part[128]) of the treelet dynamic programme over 7 leaf boxes [7, 6]."""
+    b = _c(boxes, np.float32).reshape(7, 6)
+    cost, part = np.zeros(128, np.float32), np.zeros(128, np.uint32)
+    load().orc_treelet_dp(_p(b), _p(cost), _p(part))
+    return cost, part
+
+
 def is_valid(accel) -> bool:
+    check_accel(accel)
     return bool(load().orc_is_valid(_p(accel)))
 
 
 def check_consistency(accel) -> bool:
+    check_accel(accel)
     return bool(load().orc_check_consistency(_p(accel)))
 
 
 def sah(accel) -> float:
+    check_accel(accel)
     return float(load().orc_sah(_p(accel)))
 
 
 def height(accel) -> int:
+    check_accel(accel)
     return int(load().orc_height(_p(accel)))
 
 
@@ -191,6 +226,7 @@ def write_analyzer_dump(accel, rays, width, height_, bvh_path, rays_path, cfg_pa
 
 def nodes_of(accel: np.ndarray) -> np.ndarray:
     """Node array of a buffer as uint32 [count, 16] (float fields viewable via .view(np.float32))."""
+    check_accel(accel)
     hdr = accel[:128].view(np.uint32)
     off = int(accel[16:24].view(np.uint64)[0])
     count = int(hdr[3])

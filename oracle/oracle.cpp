@@ -260,6 +260,50 @@ Box NodeBox(const Node* nodes, uint32_t x, uint32_t nint) {
     return b;
 }
 
+// Optimal partition of every subset of the 7 treelet leaves by increasing
+// size (restructure_bvh.comp:224-414): candidates of subset s are the
+// non-empty subsets p of s without its lowest leaf, in increasing order, the
+// first cheapest wins; cost = C_INT * area + best split.  Orientation as
+// restructure_bvh.comp:395-397 (sizes 2-5 flipped for even sizes).
+void TreeletDp(const Box lb[kLeaves], float sa[kMasks], float cost[kMasks], uint32_t part[kMasks]) {
+    auto union_of = [&](uint32_t m) {
+        Box b;
+        for (int a = 0; a < 3; ++a) { b.lo[a] = FLT_MAX; b.hi[a] = -FLT_MAX; }
+        for (int k = 0; k < kLeaves; ++k)
+            if (m & (1u << k))
+                for (int a = 0; a < 3; ++a) {
+                    b.lo[a] = Min(b.lo[a], lb[k].lo[a]);
+                    b.hi[a] = Max(b.hi[a], lb[k].hi[a]);
+                }
+        return b;
+    };
+    for (uint32_t m = 0; m < kMasks; ++m) {
+        Box b = union_of(m);
+        sa[m] = m ? Area(b.lo, b.hi) : 0.0f;
+        cost[m] = __builtin_popcount(m) == 1 ? kCInt * sa[m] : 0.0f;
+        part[m] = 0;
+    }
+    for (int bits = 2; bits <= kLeaves; ++bits) {
+        for (uint32_t s = 0; s < kMasks; ++s) {
+            if (__builtin_popcount(s) != bits) continue;
+            const uint32_t delta = (s - 1) & s;
+            uint32_t p = (0u - delta) & s;
+            float best_cost = FLT_MAX;
+            uint32_t best = 0;
+            do {
+                const float c = cost[p] + cost[s ^ p];
+                if (best == 0 || c < best_cost) {
+                    best_cost = c;
+                    best = p;
+                }
+                p = (p - delta) & s;
+            } while (p != 0);
+            cost[s] = kCInt * sa[s] + best_cost;
+            part[s] = (bits <= 5 && (bits & 1) == 0) ? (s ^ best) : best;
+        }
+    }
+}
+
 void RestructureTreelet(Node* nodes, uint32_t nint, uint32_t root) {
     uint32_t leaf[kLeaves], inner[kLeaves - 1];
     float area[kLeaves];
@@ -304,31 +348,7 @@ void RestructureTreelet(Node* nodes, uint32_t nint, uint32_t root) {
                 }
         return b;
     };
-    for (uint32_t m = 0; m < kMasks; ++m) {
-        Box b = union_of(m);
-        sa[m] = m ? Area(b.lo, b.hi) : 0.0f;
-        cost[m] = __builtin_popcount(m) == 1 ? kCInt * sa[m] : 0.0f;
-        part[m] = 0;
-    }
-    for (int bits = 2; bits <= kLeaves; ++bits) {
-        for (uint32_t s = 0; s < kMasks; ++s) {
-            if (__builtin_popcount(s) != bits) continue;
-            const uint32_t delta = (s - 1) & s;
-            uint32_t p = (0u - delta) & s;
-            float best_cost = FLT_MAX;
-            uint32_t best = 0;
-            do {
-                const float c = cost[p] + cost[s ^ p];
-                if (best == 0 || c < best_cost) {
-                    best_cost = c;
-                    best = p;
-                }
-                p = (p - delta) & s;
-            } while (p != 0);
-            cost[s] = kCInt * sa[s] + best_cost;
-            part[s] = (bits <= 5 && (bits & 1) == 0) ? (s ^ best) : best;
-        }
-    }
+    TreeletDp(lb, sa, cost, part);
     // Rewire top-down.
     uint32_t stack_mask[kLeaves], stack_node[kLeaves];
     int sp = 0, allocated = 1;
@@ -429,16 +449,15 @@ struct RayState {
     int k[3];
 };
 
-constexpr float kSlabWiden = 4.76837158203125e-07f;  // 2^-21
-constexpr float kLimitWiden = 1.52587890625e-05f;    // 2^-16
-constexpr float kTiny = 1e-20f;
+constexpr float kWidenDown = 0.999999523162841796875f;  // 1 - 2^-21
+constexpr float kWidenUp = 1.000000476837158203125f;    // 1 + 2^-21
+constexpr float kLimitWiden = 1.52587890625e-05f;       // 2^-16
 
 void Setup(RayState& r, const float o[3], float tmin, const float d[3]) {
     for (int a = 0; a < 3; ++a) {
         r.o[a] = o[a];
         r.d[a] = d[a];
-        const float c = std::fabs(d[a]) > kTiny ? d[a] : (d[a] < 0.0f ? -kTiny : kTiny);
-        r.id[a] = 1.0f / c;
+        r.id[a] = 1.0f / d[a];  // +-inf for a zero component (the reference clamps |d| to 1e-5, common.h:172-183)
     }
     r.tmin = tmin;
     const float ax = std::fabs(d[0]), ay = std::fabs(d[1]), az = std::fabs(d[2]);
@@ -453,19 +472,22 @@ void Setup(RayState& r, const float o[3], float tmin, const float d[3]) {
     r.S[2] = 1.0f / d[kz];
 }
 
+// Slab test with near/far planes chosen by the direction's sign; NaN plane
+// distances (origin on the plane of a parallel axis) drop out of the IEEE
+// max/min.  Widened by 2^-21 relative.
 bool Slab(const RayState& r, float limit, const float lo[3], const float hi[3], float& t0) {
-    float tn = -FLT_MAX, tf = FLT_MAX;
     float n3[3], f3[3];
     for (int a = 0; a < 3; ++a) {
         const float l = (lo[a] - r.o[a]) * r.id[a];
         const float h = (hi[a] - r.o[a]) * r.id[a];
-        n3[a] = Min(l, h);
-        f3[a] = Max(l, h);
+        const bool neg = r.id[a] < 0.0f;
+        n3[a] = neg ? h : l;
+        f3[a] = neg ? l : h;
     }
-    tn = Max(Max(n3[0], n3[1]), n3[2]);
-    tf = Min(Min(f3[0], f3[1]), f3[2]);
-    tn = std::fma(-kSlabWiden, std::fabs(tn), tn);
-    tf = std::fma(kSlabWiden, std::fabs(tf), tf);
+    float tn = Max(Max(n3[0], n3[1]), n3[2]);
+    float tf = Min(Min(f3[0], f3[1]), f3[2]);
+    tn = tn * (tn > 0.0f ? kWidenDown : kWidenUp);
+    tf = tf * (tf > 0.0f ? kWidenUp : kWidenDown);
     const float a = Max(tn, r.tmin);
     const float b = Min(tf, limit);
     t0 = a;
@@ -930,6 +952,17 @@ int orc_triangle_fp64(const float* ray, const float* v0, const float* v1, const 
     *u = uu;
     *v = vv;
     return (uu >= -eps && vv >= -eps && uu + vv <= 1.0 + eps) ? 1 : 0;
+}
+
+void orc_treelet_dp(const float* boxes, float* cost, uint32_t* part) {
+    Box lb[kLeaves];
+    for (int k = 0; k < kLeaves; ++k)
+        for (int a = 0; a < 3; ++a) {
+            lb[k].lo[a] = boxes[6 * k + a];
+            lb[k].hi[a] = boxes[6 * k + 3 + a];
+        }
+    float sa[kMasks];
+    TreeletDp(lb, sa, cost, part);
 }
 
 int orc_is_valid(const void* accel) {

@@ -52,6 +52,10 @@ void orc_refit(const float* vertices, uint32_t stride_floats, const uint32_t* in
 /* Treelet restructuring, three rounds with minimum root sizes 64/128/256 (a9). */
 void orc_restructure(uint32_t tris, void* nodes);
 
+/* Dynamic programme of one 7-leaf treelet: boxes are 7 x (lo xyz, hi xyz);
+ * writes the optimal cost and split of each of the 128 leaf subsets. */
+void orc_treelet_dp(const float* boxes, float* cost, uint32_t* part);
+
 /* Full BLAS build into `buffer` (orc_blas_bytes(tris) bytes). */
 size_t orc_blas_bytes(uint32_t tris);
 void orc_build_blas(const float* vertices, uint32_t stride_floats, const uint32_t* indices, uint32_t tris,

@@ -106,7 +106,7 @@ class Context:
         return rr.RRBuildOptions(0 if restructure else rr.RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD, None)
 
     def geometry_requirements(self, tri_count: int, vertex_count: int = 3, restructure: bool = False):
-        prim = rr.RRTriangleMeshPrimitive(c_void_p(1), vertex_count, 12, c_void_p(1), tri_count, rr.RR_INDEX_TYPE_UINT32)
+        prim = rr.RRTriangleMeshPrimitive(None, vertex_count, 12, None, tri_count, rr.RR_INDEX_TYPE_UINT32)
         inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, 1, ctypes.cast(ctypes.pointer(prim), c_void_p))
         req = rr.RRMemoryRequirements()
         opts = self._options(restructure)

@@ -193,19 +193,23 @@ RRError Guarded(const char* name, F&& f) {
     } while (0)
 
 // Mesh description from a build input (GetTriangleMeshBuildInfo, radeonrays.cpp:59-74).
-RRError MeshFromInput(const RRGeometryBuildInput* in, rr::MeshDesc& m, const char* name) {
+// With resolve == false only the counts are read (memory requirements never
+// look at the device pointers, as in the reference).
+RRError MeshFromInput(const RRGeometryBuildInput* in, rr::MeshDesc& m, const char* name, bool resolve) {
     if (in->primitive_type != RR_PRIMITIVE_TYPE_TRIANGLE_MESH) {
         RR_LOG(RR_LOG_LEVEL_ERROR, "%s: build input type not supported", name);
         return RR_ERROR_NOT_IMPLEMENTED;
     }
     RR_REQUIRE(in->primitive_count == 1 && in->triangle_mesh_primitives, name);
     const RRTriangleMeshPrimitive& p = in->triangle_mesh_primitives[0];
+    RR_REQUIRE(p.index_type == RR_INDEX_TYPE_UINT32 || p.index_type == RR_INDEX_TYPE_UINT16, name);
+    RR_REQUIRE(p.triangle_count < (1u << 30), name);
+    m.tri_count = p.triangle_count;
+    if (!resolve) return RR_SUCCESS;
     if (p.triangle_count > 0) {
         RR_REQUIRE(p.vertices && p.triangle_indices && p.vertex_count > 0, name);
         RR_REQUIRE(p.vertex_stride >= 12 && p.vertex_stride % 4 == 0, name);
     }
-    RR_REQUIRE(p.index_type == RR_INDEX_TYPE_UINT32 || p.index_type == RR_INDEX_TYPE_UINT16, name);
-    RR_REQUIRE(p.triangle_count < (1u << 30), name);
     m.vertices = Addr(p.vertices);
     m.vertex_count = p.vertex_count;
     m.vertex_stride = p.vertex_stride;
@@ -311,7 +315,7 @@ RRError rrGetGeometryBuildMemoryRequirements(RRContext context, const RRGeometry
     RR_LOG(RR_LOG_LEVEL_INFO, "rrGetGeometryBuildMemoryRequirements");
     RR_REQUIRE(context && build_input && memory_requirements, "rrGetGeometryBuildMemoryRequirements");
     rr::MeshDesc m;
-    const RRError e = MeshFromInput(build_input, m, "rrGetGeometryBuildMemoryRequirements");
+    const RRError e = MeshFromInput(build_input, m, "rrGetGeometryBuildMemoryRequirements", false);
     if (e != RR_SUCCESS) return e;
     memory_requirements->result_buffer_size = rr::BlasResultSize(m.tri_count);
     memory_requirements->temporary_build_buffer_size = rr::BlasBuildScratchBytes(m.tri_count, WantsRestructure(build_options));
@@ -349,7 +353,7 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
     RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
                "rrCmdBuildGeometry");
     rr::MeshDesc m;
-    const RRError e = MeshFromInput(build_input, m, "rrCmdBuildGeometry");
+    const RRError e = MeshFromInput(build_input, m, "rrCmdBuildGeometry", true);
     if (e != RR_SUCCESS) return e;
     RR_REQUIRE(geometry_buffer, "rrCmdBuildGeometry");
     if (build_operation == RR_BUILD_OPERATION_BUILD) RR_REQUIRE(temporary_buffer, "rrCmdBuildGeometry");

@@ -63,28 +63,23 @@ struct RayState {
 
 // Relative widening of slab intervals: slab distances (b - o) * (1/d) carry
 // at most ~3 ulp of relative error, so widening by 2^-21 keeps every box
-// whose exact interval is non-empty.
-constexpr float kSlabWiden = 4.76837158203125e-07f;   // 2^-21
+// whose exact interval is non-empty.  Widening multiplies, so infinite
+// distances stay infinite.
+constexpr float kSlabWidenDown = 0.999999523162841796875f;  // 1 - 2^-21
+constexpr float kSlabWidenUp = 1.000000476837158203125f;    // 1 + 2^-21
 // Relative slack on the current closest distance when culling boxes, so a
 // candidate at exactly the closest distance (tie) or within the triangle
 // test's rounding of it is never culled.
-constexpr float kLimitWiden = 1.52587890625e-05f;     // 2^-16
-// |d| below this is replaced by +/-kTinyDir before taking the reciprocal
-// (vlk/kernels/common.h:172-183 uses 1e-5; a tiny clamp keeps the slab test
-// conservative for nearly axis-parallel rays).
-constexpr float kTinyDir = 1e-20f;
-
-RR_DEV float SafeRcp(float d) {
-    float a = fabsf(d) > kTinyDir ? d : (d < 0.0f ? -kTinyDir : kTinyDir);
-    return 1.0f / a;
-}
+constexpr float kLimitWiden = 1.52587890625e-05f;  // 2^-16
 
 RR_DEV float Perm(int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); }
 
 RR_DEV void SetupRay(RayState& r, float ox, float oy, float oz, float tmin, float dx, float dy, float dz) {
     r.ox = ox; r.oy = oy; r.oz = oz; r.tmin = tmin;
     r.dx = dx; r.dy = dy; r.dz = dz;
-    r.ix = SafeRcp(dx); r.iy = SafeRcp(dy); r.iz = SafeRcp(dz);
+    // Exact reciprocals: a zero component gives +-inf (sign of the zero), so
+    // an axis-parallel ray has an unbounded slab on that axis.
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
     // Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013:
     // kz = dimension of largest |d| (first on ties), kx/ky the next two in
     // cyclic order, swapped when d[kz] < 0 to keep the winding.
@@ -100,22 +95,27 @@ RR_DEV void SetupRay(RayState& r, float ox, float oy, float oz, float tmin, floa
     r.sz = 1.0f / dkz;
 }
 
-// Conservative slab test of one box.  Returns the widened entry distance in
-// t0 and whether [t0, t1] is non-empty, where t1 is clipped to `limit`.
+// Conservative slab test of one box.  Near and far planes are chosen by the
+// sign of the reciprocal direction.  An origin lying exactly on a plane of
+// an axis the ray is parallel to gives 0 * inf = NaN for that plane; the
+// IEEE max/min below ignore NaN operands, which makes that plane unbounded,
+// as it is for the exact ray.  Returns the widened entry distance in t0 and
+// whether [t0, t1] is non-empty, where t1 is clipped to `limit`.
 RR_DEV bool SlabTest(const RayState& r, float limit, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
                      float& t0) {
-    float lx = (bx0 - r.ox) * r.ix, hx = (bx1 - r.ox) * r.ix;
-    float ly = (by0 - r.oy) * r.iy, hy = (by1 - r.oy) * r.iy;
-    float lz = (bz0 - r.oz) * r.iz, hz = (bz1 - r.oz) * r.iz;
-    float nx = fminf(lx, hx), fx = fmaxf(lx, hx);
-    float ny = fminf(ly, hy), fy = fmaxf(ly, hy);
-    float nz = fminf(lz, hz), fz = fmaxf(lz, hz);
+    const float lx = (bx0 - r.ox) * r.ix, hx = (bx1 - r.ox) * r.ix;
+    const float ly = (by0 - r.oy) * r.iy, hy = (by1 - r.oy) * r.iy;
+    const float lz = (bz0 - r.oz) * r.iz, hz = (bz1 - r.oz) * r.iz;
+    const bool sx = r.ix < 0.0f, sy = r.iy < 0.0f, sz = r.iz < 0.0f;
+    const float nx = sx ? hx : lx, fx = sx ? lx : hx;
+    const float ny = sy ? hy : ly, fy = sy ? ly : hy;
+    const float nz = sz ? hz : lz, fz = sz ? lz : hz;
     float tn = fmaxf(fmaxf(nx, ny), nz);
     float tf = fminf(fminf(fx, fy), fz);
-    tn = fmaf(-kSlabWiden, fabsf(tn), tn);
-    tf = fmaf(kSlabWiden, fabsf(tf), tf);
-    float a = fmaxf(tn, r.tmin);
-    float b = fminf(tf, limit);
+    tn = tn * (tn > 0.0f ? kSlabWidenDown : kSlabWidenUp);
+    tf = tf * (tf > 0.0f ? kSlabWidenUp : kSlabWidenDown);
+    const float a = fmaxf(tn, r.tmin);
+    const float b = fminf(tf, limit);
     t0 = a;
     return a <= b;
 }

@@ -1,0 +1,140 @@
+"""CPU checks of the C ABI: the library loads, exports exactly what the
+headers declare, and rejects bad arguments before touching the GPU (the
+reference's shim checks pointers first, radeonrays.cpp:252-256,424-428)."""
+import ctypes
+import re
+from ctypes import byref, c_size_t, c_void_p
+from pathlib import Path
+
+import pytest
+
+from radeonrays_sdk_amd import rr
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include").glob("radeonrays*.h"))
+# A context handle that is never dereferenced by the paths under test.
+FAKE_CTX = c_void_p(0x1000)
+
+
+def declared_symbols():
+    names = set()
+    for h in HEADERS:
+        text = h.read_text()
+        names |= set(re.findall(r"RR_API\s+[\w\s\*]+?\b(rrx?\w+)\s*\(", text))
+    return names
+
+
+def test_headers_declare_the_reference_entry_points():
+    names = declared_symbols()
+    reference_abi = {  # radeonrays.h:276-473
+        "rrCreateContext", "rrDestroyContext", "rrSetLogLevel", "rrSetLogFile", "rrCmdBuildGeometry",
+        "rrGetGeometryBuildMemoryRequirements", "rrCmdBuildScene", "rrGetSceneBuildMemoryRequirements",
+        "rrCmdIntersect", "rrGetTraceMemoryRequirements", "rrAllocateCommandStream", "rrReleaseCommandStream",
+        "rrSumbitCommandStream", "rrReleaseEvent", "rrWaitEvent", "rrReleaseDevicePtr",
+        "rrReleaseExternalCommandStream"}
+    assert reference_abi <= names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = rr.load()
+    names = declared_symbols()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(rr._SIGNATURES) == names  # the binding covers exactly the headers
+
+
+def test_struct_layouts_match_the_abi():
+    # Sizes as compiled from include/radeonrays.h by gcc on x86-64.
+    assert ctypes.sizeof(rr.RRTriangleMeshPrimitive) == 32
+    assert ctypes.sizeof(rr.RRGeometryBuildInput) == 16
+    assert ctypes.sizeof(rr.RRBuildOptions) == 16
+    assert ctypes.sizeof(rr.RRInstance) == 56
+    assert ctypes.sizeof(rr.RRSceneBuildInput) == 16
+    assert ctypes.sizeof(rr.RRMemoryRequirements) == 24
+
+
+def _mesh_input(tris, ptype=rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, count=1):
+    prim = rr.RRTriangleMeshPrimitive(None, 3 * tris, 12, None, tris, rr.RR_INDEX_TYPE_UINT32)
+    inp = rr.RRGeometryBuildInput(ptype, count, ctypes.cast(ctypes.pointer(prim), c_void_p))
+    return prim, inp
+
+
+def test_null_arguments_are_invalid_parameter():
+    lib = rr.load()
+    prim, inp = _mesh_input(10)
+    req = rr.RRMemoryRequirements()
+    E = rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrCreateContext(rr.RR_API_VERSION, rr.RR_API_HIP, None) == E
+    assert lib.rrDestroyContext(None) == E
+    assert lib.rrGetGeometryBuildMemoryRequirements(None, byref(inp), None, byref(req)) == E
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, None, None, byref(req)) == E
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, None) == E
+    assert lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, None, None, None, None, c_void_p(1)) == E
+    assert lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(inp), None, None, None, None) == E
+    assert lib.rrCmdBuildScene(FAKE_CTX, None, None, None, None, c_void_p(1)) == E
+    assert lib.rrCmdIntersect(FAKE_CTX, None, 0, None, 1, None, 0, None, None, None) == E
+    size = c_size_t()
+    # radeonrays.cpp:741: zero rays is an invalid parameter too.
+    assert lib.rrGetTraceMemoryRequirements(FAKE_CTX, 0, byref(size)) == E
+    assert lib.rrGetTraceMemoryRequirements(FAKE_CTX, 1, None) == E
+    assert lib.rrAllocateCommandStream(None, None) == E
+    assert lib.rrSumbitCommandStream(FAKE_CTX, None, None, None) == E
+    assert lib.rrReleaseEvent(FAKE_CTX, None) == E
+    assert lib.rrWaitEvent(FAKE_CTX, None) == E
+    assert lib.rrReleaseDevicePtr(FAKE_CTX, None) == E
+    assert lib.rrReleaseExternalCommandStream(FAKE_CTX, None) == E
+    assert lib.rrGetDevicePtrFromHipPtr(FAKE_CTX, None, 0, None) == E
+    assert lib.rrSetLogFile(None) == E
+
+
+def test_aabb_list_is_not_implemented():
+    # radeonrays.cpp:286-289.
+    lib = rr.load()
+    prim, inp = _mesh_input(10, ptype=rr.RR_PRIMITIVE_TYPE_AABB_LIST)
+    req = rr.RRMemoryRequirements()
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, byref(req)) == rr.RR_ERROR_NOT_IMPLEMENTED
+    assert (lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(inp), None, c_void_p(1), c_void_p(1),
+                                   c_void_p(1)) == rr.RR_ERROR_NOT_IMPLEMENTED)
+
+
+def test_log_level_validation():
+    lib = rr.load()
+    assert lib.rrSetLogLevel(0) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrSetLogLevel(9) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrSetLogLevel(rr.RR_LOG_LEVEL_WARN) == rr.RR_SUCCESS
+
+
+def test_vulkan_and_dx_contexts_are_unsupported():
+    lib = rr.load()
+    ctx = c_void_p()
+    assert lib.rrCreateContext(rr.RR_API_VERSION, rr.RR_API_VK, byref(ctx)) == rr.RR_ERROR_UNSUPPORTED_API
+    assert lib.rrCreateContext(rr.RR_API_VERSION, rr.RR_API_DX, byref(ctx)) == rr.RR_ERROR_UNSUPPORTED_API
+
+
+@pytest.mark.parametrize("tris", [0, 1, 2, 1000, 1045458])
+def test_memory_requirements(tris):
+    lib = rr.load()
+    for flags, restructure in ((rr.RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD, False), (0, True)):
+        prim, inp = _mesh_input(tris)
+        opts = rr.RRBuildOptions(flags, None)
+        req = rr.RRMemoryRequirements()
+        assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), byref(opts), byref(req)) == rr.RR_SUCCESS
+        nodes = 2 * tris - 1 if tris else 0
+        # 128-byte header + (2N-1) 64-byte nodes (hlbvh_builder.cpp:376-377), 256-byte aligned.
+        assert req.result_buffer_size == 128 + (nodes * 64 + 255) // 256 * 256
+        assert req.temporary_update_buffer_size == 0
+        assert req.temporary_build_buffer_size >= 16 * tris
+    scene = rr.RRSceneBuildInput(None, 4096)
+    req = rr.RRMemoryRequirements()
+    assert lib.rrGetSceneBuildMemoryRequirements(FAKE_CTX, byref(scene), None, byref(req)) == rr.RR_SUCCESS
+    assert req.result_buffer_size >= 128 + 8191 * 64 + 8192 * 48 + 4096 * 8
+    size = c_size_t()
+    assert lib.rrGetTraceMemoryRequirements(FAKE_CTX, 1000, byref(size)) == rr.RR_SUCCESS
+    assert size.value >= 1000 * 256
+
+
+def test_multiple_meshes_per_build_rejected():
+    lib = rr.load()
+    prim, inp = _mesh_input(10, count=2)
+    req = rr.RRMemoryRequirements()
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, byref(req)) == rr.RR_ERROR_INVALID_PARAMETER

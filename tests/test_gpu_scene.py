@@ -16,7 +16,7 @@ def _scene(ctx, dev, n_meshes, n_inst, tris_lo, tris_hi, seed):
     counts = scenes.log_uniform_ints(rng, n_meshes, tris_lo, tris_hi)
     blases, meshes = [], []
     for c in counts:
-        v, i = scenes.triangle_soup(int(c), rng)
+        v, i = scenes.triangle_soup(int(c), rng, sigma=0.08)
         meshes.append((v, i))
         blases.append(ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True))
     pick = rng.integers(0, n_meshes, size=n_inst)
@@ -55,7 +55,7 @@ def test_tlas_build_and_trace(ctx, dev, n_inst):
             bf = oracle.brute_force(buf, rays[:3000], blas_map=host)
             ok, msg = hits_equal({k: a[:3000] for k, a in gpu.items()}, bf)
             assert ok, "brute force: " + msg
-            assert (gpu["inst_id"] != RR_INVALID).mean() > 0.05
+            assert (gpu["inst_id"] != RR_INVALID).mean() > 0.01
     ids = ctx.intersect(tlas, drays, "closest", "instance")
     torch.cuda.synchronize()
     r = oracle.trace(buf, rays, blas_map=host)
