@@ -1,0 +1,211 @@
+"""Benchmark of the build-and-intersect hot path on one MI355X.
+
+Workload (one "step"): the 1,045,458-triangle fBm heightfield BLAS (724x724
+vertices, LBVH + treelet restructuring) queried with 4,194,304 incoherent rays
+(origins in [0,1]x[0.25,0.5]x[0,1], directions uniform on the sphere), once
+closest-hit and once any-hit, FULL_HIT output, through rrCmdIntersect.
+Inputs are resident in HBM before timing starts.  value = rays traced per
+second over the whole step, in Mrays/s.
+
+Also reported: build/refit/sort timings, the traversal kernel's roofline
+(algorithmic bytes from per-ray visit counters, HIP-event kernel times) and
+the CPU baseline (the oracle's traversal on the host cores).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "incoherent Mrays/s @1M tris + served p50/p99 latency vs plain per-call path"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 64
+RAY_BYTES = 32
+HIT_BYTES = 16
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def median_ms(starts, ends):
+    return float(np.median([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+
+
+def timed(fn, reps, warm=2):
+    import torch
+
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s, e in ev:
+        s.record()
+        fn()
+        e.record()
+    torch.cuda.synchronize()
+    return median_ms([s for s, _ in ev], [e for _, e in ev])
+
+
+def cpu_baseline(buf, rays, threads, budget_s=20.0):
+    """Oracle traversal (closest + any) of the same BVH on the host cores."""
+    import oracle
+
+    n = rays.shape[0]
+    t0 = time.perf_counter()
+    oracle.trace(buf, rays[: n // 8], threads=threads)  # warm-up
+    reps, total_rays, elapsed = 0, 0, 0.0
+    start = time.perf_counter()
+    while reps < 3 and (time.perf_counter() - t0) < budget_s:
+        oracle.trace(buf, rays, any_hit=False, threads=threads)
+        oracle.trace(buf, rays, any_hit=True, threads=threads)
+        reps += 1
+        total_rays += 2 * n
+    elapsed = time.perf_counter() - start
+    return {"value": total_rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} x (closest + any) over all {n} rays of the same BVH, oracle/oracle.cpp OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip build/refit/sort timings")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(rays would shard across GPUs with a replicated BVH; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from radeonrays_sdk_amd import api, scenes
+
+    dev = torch.device("cuda", 0)
+    log("generating heightfield workload")
+    w = scenes.heightfield()
+    n_rays = w.rays.shape[0]
+    ctx = api.Context(0)
+    v = torch.from_numpy(w.vertices).to(dev)
+    idx = torch.from_numpy(w.indices).to(dev)
+    rays = torch.from_numpy(w.rays).to(dev)
+    req = ctx.geometry_requirements(idx.shape[0], v.shape[0], True)
+    scratch = ctx.empty(req.temporary_build_buffer_size)
+    result = ctx.empty(req.result_buffer_size)
+    log("building BVH")
+    accel = ctx.build_geometry(v, idx, restructure=True, scratch=scratch, result=result)
+    hits_c = ctx.empty(n_rays * HIT_BYTES)
+    hits_a = ctx.empty(n_rays * HIT_BYTES)
+    tscratch = ctx.trace_scratch(n_rays)
+    torch.cuda.synchronize()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        ctx.intersect(accel, rays, "closest", "full", hits=hits_c, scratch=tscratch)
+        if ev is not None:
+            ev[1].record()
+        ctx.intersect(accel, rays, "any", "full", hits=hits_a, scratch=tscratch)
+        if ev is not None:
+            ev[2].record()
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    value = 2 * n_rays * args.steps / wall / 1e6
+    closest_ms = float(np.median([e[0].elapsed_time(e[1]) for e in events]))
+    any_ms = float(np.median([e[1].elapsed_time(e[2]) for e in events]))
+
+    # Algorithmic bytes of the closest-hit traversal: rays in, hits out, and
+    # one 64-byte node per inner node or leaf visited (visit counts from the
+    # counting variant of the same kernel, outside the timed region).
+    _, stats = ctx.intersect_with_stats(accel, rays, "closest")
+    st = stats.cpu().numpy().astype(np.int64)
+    visits = int(st[:, 0].sum() + st[:, 1].sum())
+    alg_bytes = n_rays * (RAY_BYTES + HIT_BYTES) + NODE_BYTES * visits
+    achieved = alg_bytes / (closest_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "heightfield_fbm_724x724_1045458tris_4194304rays_closest+any",
+                   "triangles": int(idx.shape[0]), "rays_per_query": n_rays, "queries_per_step": ["closest", "any"],
+                   "output": "FULL_HIT", "build": "lbvh+treelet"},
+        "closest_mrays_s": round(n_rays / closest_ms / 1e3, 2),
+        "any_mrays_s": round(n_rays / any_ms / 1e3, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "TraceKernel<closest, FULL_HIT, single level>", "kernel_ms": round(closest_ms, 4),
+                     "algorithmic_bytes": int(alg_bytes),
+                     "visits_per_ray": round(visits / n_rays, 3)},
+    }
+
+    if not args.no_extras:
+        log("build / refit / sort timings")
+        out["build_ms"] = round(timed(lambda: ctx.build_geometry(v, idx, True, scratch=scratch, result=result), 5), 3)
+        out["build_fast_ms"] = round(timed(lambda: ctx.build_geometry(v, idx, False, scratch=scratch, result=result), 5), 3)
+        v2 = torch.from_numpy(scenes.perturbed_heights(w.vertices, 0)).to(dev)
+        refit_ms = timed(lambda: ctx.update_geometry(accel, v2, idx), 8)
+        n = idx.shape[0]
+        out["refit_ms"] = round(refit_ms, 4)
+        out["refit_gbs"] = round((n * 112 + (n - 1) * 192) / (refit_ms * 1e-3) / 1e9, 1)
+        keys = torch.randint(0, 1 << 30, (16 << 20,), dtype=torch.int32, device=dev)
+        vals = torch.arange(16 << 20, dtype=torch.int32, device=dev)
+        k2, v2_ = keys.clone(), vals.clone()
+        tmp = (torch.empty_like(keys), torch.empty_like(vals))
+
+        def sort_once():
+            k2.copy_(keys)
+            v2_.copy_(vals)
+            ctx.sort_pairs(k2, v2_, 30, tmp=tmp)
+
+        copy_ms = timed(lambda: (k2.copy_(keys), v2_.copy_(vals)), 5)
+        sort_ms = timed(sort_once, 5) - copy_ms
+        sort_gbs = 68 * (16 << 20) / (sort_ms * 1e-3) / 1e9
+        out["sort16m_ms"] = round(sort_ms, 4)
+        out["sort16m_gbs"] = round(sort_gbs, 1)
+        out["sort16m_hbm_frac"] = round(sort_gbs / HBM_PEAK_GBS, 4)
+
+    if not args.no_cpu:
+        log("cpu baseline")
+        threads = min(16, os.cpu_count() or 1)
+        buf = accel.buffer.cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(buf, w.rays, threads)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -63,7 +63,7 @@ class Context:
         handles = []
         try:
             for a in addrs:
-                if a is None:
+                if not a:  # None, or the null address of an empty tensor
                     handles.append(None)
                     continue
                 p = c_void_p()

@@ -55,7 +55,7 @@ def test_tlas_build_and_trace(ctx, dev, n_inst):
             bf = oracle.brute_force(buf, rays[:3000], blas_map=host)
             ok, msg = hits_equal({k: a[:3000] for k, a in gpu.items()}, bf)
             assert ok, "brute force: " + msg
-            assert (gpu["inst_id"] != RR_INVALID).mean() > 0.01
+            assert (gpu["inst_id"] != RR_INVALID).sum() > 50
     ids = ctx.intersect(tlas, drays, "closest", "instance")
     torch.cuda.synchronize()
     r = oracle.trace(buf, rays, blas_map=host)
