@@ -86,7 +86,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(rays would shard across GPUs with a replicated BVH; see DESIGN.md).", file=sys.stderr)
+              "(rays would shard across GPUs over a replicated BVH).", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -14,10 +14,10 @@
 // Numerics follow the library's contract: binary32 operations in a fixed
 // order (built with -ffp-contract=off), a watertight triangle test (Woop,
 // Benthin, Wald, JCGT 2(1) 2013) and (t, inst, prim) lexicographic ordering of
-// closest hits.  Parity unpinned: none of these functions is pinned by a
-// reference-held golden vector (the reference has none for hits); each is
-// pinned by the closed forms, invariants, brute force and the compiled
-// bvh_analyzer checks listed in tests/.
+// closest hits.  The reference holds no golden vectors for this path; every
+// function here is pinned instead by closed forms, brute force, invariants,
+// FP64 and the reference's own bvh_analyzer compiled from its sources (see
+// tests/test_oracle.py).
 #include "oracle.h"
 
 #include <omp.h>

@@ -288,9 +288,23 @@ __global__ __launch_bounds__(kBlock) void EmitHierarchyKernel(const uint32_t* __
 // Bottom-up refit shared by BLAS build/update and TLAS build
 // (lbvh_fit_aabb_mesh.comp:122-205, lbvh_fit_aabb_scene.comp:98-167).  Each
 // finished node stores its box into its parent's child slot, then bumps the
-// parent's arrival counter (agent scope, acq_rel); an odd previous value
-// means the sibling already stored its box, so this thread merges both and
-// climbs on.
+// parent's arrival counter; an odd previous value means the sibling already
+// stored its box, so this thread merges both and climbs on.
+//
+// Cross-workgroup hand-off without an L2 write-back per step: the slot is
+// written with agent-scope (write-through) stores, the wave waits for them to
+// complete (vmcnt(0)) before the counter's atomic add, and the second
+// arriver reads the sibling's slot with agent-scope loads, which do not hit
+// stale cache lines.  Both threads' accesses to a slot are agent-scope
+// atomics, so the counter orders them.
+__device__ __forceinline__ void StoreAgent(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float LoadAgent(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void WaitStores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t child, uint32_t p,
                                            float lo[3], float hi[3]) {
     // Guards: a parent index outside the node array or more levels than any
@@ -300,15 +314,19 @@ __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t
         const uint32_t c0 = pn->child0;
         float* slot_lo = c0 == child ? pn->a0_min_or_v0 : pn->a1_min_or_v2;
         float* slot_hi = c0 == child ? pn->a0_max_or_v1 : pn->a1_max_or_v3;
-        float* other_lo = c0 == child ? pn->a1_min_or_v2 : pn->a0_min_or_v0;
-        float* other_hi = c0 == child ? pn->a1_max_or_v3 : pn->a0_max_or_v1;
-        slot_lo[0] = lo[0]; slot_lo[1] = lo[1]; slot_lo[2] = lo[2];
-        slot_hi[0] = hi[0]; slot_hi[1] = hi[1]; slot_hi[2] = hi[2];
-        const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if ((old & 1u) == 0) return;
+        const float* other_lo = c0 == child ? pn->a1_min_or_v2 : pn->a0_min_or_v0;
+        const float* other_hi = c0 == child ? pn->a1_max_or_v3 : pn->a0_max_or_v1;
         for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], other_lo[a]);
-            hi[a] = fmaxf(hi[a], other_hi[a]);
+            StoreAgent(slot_lo + a, lo[a]);
+            StoreAgent(slot_hi + a, hi[a]);
+        }
+        WaitStores();
+        const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 1u) == 0) return;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], LoadAgent(other_lo + a));
+            hi[a] = fmaxf(hi[a], LoadAgent(other_hi + a));
         }
         child = p;
         p = pn->parent;
