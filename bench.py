@@ -57,6 +57,18 @@ def timed(fn, reps, warm=2):
     return median_ms([s for s, _ in ev], [e for _, e in ev])
 
 
+def profiled_traffic(key="traffic_bytes_closest"):
+    """HBM bytes per traversal launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (calibrated against a copy of known size), from profiles/heightfield_trace_pmc.txt."""
+    p = ROOT / "profiles" / "heightfield_trace_pmc.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith(key + ":"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(buf, rays, threads, budget_s=20.0):
     """Oracle traversal (closest + any) of the same BVH on the host cores."""
     import oracle
@@ -83,6 +95,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip build/refit/sort timings")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -166,8 +180,8 @@ def main():
         "closest_mrays_s": round(n_rays / closest_ms / 1e3, 2),
         "any_mrays_s": round(n_rays / any_ms / 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "TraceKernel<closest, FULL_HIT, single level>", "kernel_ms": round(closest_ms, 4),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(),
+                     "kernel": "TracePersistentKernel<closest, FULL_HIT>", "kernel_ms": round(closest_ms, 4),
                      "algorithmic_bytes": int(alg_bytes),
                      "visits_per_ray": round(visits / n_rays, 3)},
     }
@@ -197,6 +211,14 @@ def main():
         out["sort16m_ms"] = round(sort_ms, 4)
         out["sort16m_gbs"] = round(sort_gbs, 1)
         out["sort16m_hbm_frac"] = round(sort_gbs / HBM_PEAK_GBS, 4)
+
+    if args.calibrate_copy_mb:
+        src = torch.ones(args.calibrate_copy_mb << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = int(src.numel() * 4)
 
     if not args.no_cpu:
         log("cpu baseline")

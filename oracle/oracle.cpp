@@ -449,9 +449,8 @@ struct RayState {
     int k[3];
 };
 
-constexpr float kWidenDown = 0.999999523162841796875f;  // 1 - 2^-21
-constexpr float kWidenUp = 1.000000476837158203125f;    // 1 + 2^-21
-constexpr float kLimitWiden = 1.52587890625e-05f;       // 2^-16
+constexpr float kSlabWiden = 9.5367431640625e-07f;  // 2^-20
+constexpr float kLimitWiden = 1.52587890625e-05f;  // 2^-16
 
 void Setup(RayState& r, const float o[3], float tmin, const float d[3]) {
     for (int a = 0; a < 3; ++a) {
@@ -467,14 +466,14 @@ void Setup(RayState& r, const float o[3], float tmin, const float d[3]) {
     r.k[0] = kx;
     r.k[1] = ky;
     r.k[2] = kz;
-    r.S[0] = d[kx] / d[kz];
-    r.S[1] = d[ky] / d[kz];
-    r.S[2] = 1.0f / d[kz];
+    r.S[2] = r.id[kz];  // 1/d[kz], shared with the slab test
+    r.S[0] = d[kx] * r.S[2];
+    r.S[1] = d[ky] * r.S[2];
 }
 
 // Slab test with near/far planes chosen by the direction's sign; NaN plane
 // distances (origin on the plane of a parallel axis) drop out of the IEEE
-// max/min.  Widened by 2^-21 relative.
+// max/min.  The box is kept when entry <= exit + 2^-20 |exit|.
 bool Slab(const RayState& r, float limit, const float lo[3], const float hi[3], float& t0) {
     float n3[3], f3[3];
     for (int a = 0; a < 3; ++a) {
@@ -484,14 +483,10 @@ bool Slab(const RayState& r, float limit, const float lo[3], const float hi[3], 
         n3[a] = neg ? h : l;
         f3[a] = neg ? l : h;
     }
-    float tn = Max(Max(n3[0], n3[1]), n3[2]);
-    float tf = Min(Min(f3[0], f3[1]), f3[2]);
-    tn = tn * (tn > 0.0f ? kWidenDown : kWidenUp);
-    tf = tf * (tf > 0.0f ? kWidenUp : kWidenDown);
-    const float a = Max(tn, r.tmin);
-    const float b = Min(tf, limit);
+    const float a = Max(Max(Max(n3[0], n3[1]), n3[2]), r.tmin);
+    const float b = Min(Min(Min(f3[0], f3[1]), f3[2]), limit);
     t0 = a;
-    return a <= b;
+    return a <= std::fma(kSlabWiden, std::fabs(b), b);
 }
 
 float Widen(float t) { return std::fma(kLimitWiden, std::fabs(t), t); }

@@ -61,12 +61,11 @@ struct RayState {
     int   kx, ky, kz;     // watertight axis permutation
 };
 
-// Relative widening of slab intervals: slab distances (b - o) * (1/d) carry
-// at most ~3 ulp of relative error, so widening by 2^-21 keeps every box
-// whose exact interval is non-empty.  Widening multiplies, so infinite
-// distances stay infinite.
-constexpr float kSlabWidenDown = 0.999999523162841796875f;  // 1 - 2^-21
-constexpr float kSlabWidenUp = 1.000000476837158203125f;    // 1 + 2^-21
+// Conservative widening of slab intervals: slab distances (b - o) * (1/d)
+// carry at most ~3 ulp of relative error and their signs are exact, so a box
+// is kept when entry <= exit + 2^-20 |exit| (entry and exit clipped to the
+// ray's interval first).
+constexpr float kSlabWiden = 9.5367431640625e-07f;  // 2^-20
 // Relative slack on the current closest distance when culling boxes, so a
 // candidate at exactly the closest distance (tie) or within the triangle
 // test's rounding of it is never culled.
@@ -87,20 +86,21 @@ RR_DEV void SetupRay(RayState& r, float ox, float oy, float oz, float tmin, floa
     int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
     int ky = kx == 2 ? 0 : kx + 1;
-    float dkz = Perm(kz, dx, dy, dz);
+    const float dkz = Perm(kz, dx, dy, dz);
     if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
     r.kx = kx; r.ky = ky; r.kz = kz;
-    r.sx = Perm(kx, dx, dy, dz) / dkz;
-    r.sy = Perm(ky, dx, dy, dz) / dkz;
-    r.sz = 1.0f / dkz;
+    // Shear constants from the reciprocal already computed for the slab test.
+    r.sz = Perm(kz, r.ix, r.iy, r.iz);
+    r.sx = Perm(kx, dx, dy, dz) * r.sz;
+    r.sy = Perm(ky, dx, dy, dz) * r.sz;
 }
 
 // Conservative slab test of one box.  Near and far planes are chosen by the
 // sign of the reciprocal direction.  An origin lying exactly on a plane of
 // an axis the ray is parallel to gives 0 * inf = NaN for that plane; the
 // IEEE max/min below ignore NaN operands, which makes that plane unbounded,
-// as it is for the exact ray.  Returns the widened entry distance in t0 and
-// whether [t0, t1] is non-empty, where t1 is clipped to `limit`.
+// as it is for the exact ray.  Returns the entry distance in t0 and whether
+// the box is kept; the exit is clipped to `limit`.
 RR_DEV bool SlabTest(const RayState& r, float limit, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
                      float& t0) {
     const float lx = (bx0 - r.ox) * r.ix, hx = (bx1 - r.ox) * r.ix;
@@ -110,14 +110,10 @@ RR_DEV bool SlabTest(const RayState& r, float limit, float bx0, float by0, float
     const float nx = sx ? hx : lx, fx = sx ? lx : hx;
     const float ny = sy ? hy : ly, fy = sy ? ly : hy;
     const float nz = sz ? hz : lz, fz = sz ? lz : hz;
-    float tn = fmaxf(fmaxf(nx, ny), nz);
-    float tf = fminf(fminf(fx, fy), fz);
-    tn = tn * (tn > 0.0f ? kSlabWidenDown : kSlabWidenUp);
-    tf = tf * (tf > 0.0f ? kSlabWidenUp : kSlabWidenDown);
-    const float a = fmaxf(tn, r.tmin);
-    const float b = fminf(tf, limit);
+    const float a = fmaxf(fmaxf(fmaxf(nx, ny), nz), r.tmin);
+    const float b = fminf(fminf(fminf(fx, fy), fz), limit);
     t0 = a;
-    return a <= b;
+    return a <= fmaf(kSlabWiden, fabsf(b), b);
 }
 
 RR_DEV float WidenLimit(float t) { return fmaf(kLimitWiden, fabsf(t), t); }

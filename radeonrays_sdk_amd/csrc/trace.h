@@ -16,6 +16,19 @@ constexpr int kSpillEntries = 64;
 
 size_t TraceScratchBytes(uint32_t ray_count);
 
+// Single-level queries run the persistent kernel unless the mode is plain
+// (one lane per ray, the exact depth-first loop); RR_TRACE_MODE=plain selects
+// it at start-up.
+enum TraceModeKind : int { kTraceModePersistent = 0, kTraceModePlain = 1 };
+int TraceMode();
+void SetTraceMode(int mode);
+
+// Persistent kernel: a wave goes back to fetch rays for its idle lanes once
+// fewer than this many lanes are still traversing (RR_TRACE_MIN_BUSY).
+constexpr uint32_t kDefaultMinBusy = 32;
+uint32_t MinBusyLanes();
+void SetMinBusyLanes(uint32_t lanes);
+
 enum QueryKind : int { kQueryClosest = 0, kQueryAny = 1 };
 enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };
 

@@ -19,11 +19,22 @@
 #include "rr_device.h"
 #include "trace.h"
 
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
 namespace rr {
 
 namespace {
 
 constexpr int kBlock = 256;
+
+// Trace scratch: kQueues ray-queue counters, each on its own 128-byte line,
+// then kSpillEntries stack slots per lane.
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kQueueStride = 32;
+constexpr uint32_t kQueueHeadWords = kQueues * kQueueStride;
+constexpr uint32_t kFetch = 64;
 
 struct RayIn {
     float4 a;  // origin.xyz, min_t
@@ -56,9 +67,14 @@ struct LaneStack {
     }
     __device__ __forceinline__ uint32_t pop() {
         --sp;
-        if (sp >= gbase) return lds[sp & (kLdsStackEntries - 1)][threadIdx.x];
-        gbase = sp;
-        return spill[sp];
+        // Read the LDS slot unconditionally (a plain ds_read) and replace it
+        // from scratch only when the entry was spilled.
+        uint32_t v = lds[sp & (kLdsStackEntries - 1)][threadIdx.x];
+        if (sp < gbase) {
+            gbase = sp;
+            v = spill[sp];
+        }
+        return v;
     }
 };
 
@@ -94,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     float limit = WidenLimit(tmax);
     uint32_t n_internal_visits = 0, n_leaf_visits = 0, n_box_tests = 0, n_tri_tests = 0;
 
-    LaneStack st{s_stack, scratch + (size_t)i * kSpillEntries, 0, 0};
+    LaneStack st{s_stack, scratch + kQueueHeadWords + (size_t)i * kSpillEntries, 0, 0};
     const BvhNode* nodes = top_nodes;
     uint32_t node_count = top_count;  // of the level being traversed (bounds every node index)
     uint32_t inst = kInvalid;  // kInvalid while in the top level
@@ -193,6 +209,194 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     }
 }
 
+// Write the query's output record for ray i.
+template <int OUT, bool TWO_LEVEL>
+__device__ __forceinline__ void WriteHit(void* __restrict__ hits, uint32_t i, bool hit, float u, float v,
+                                         uint32_t inst, uint32_t prim, float t, float tmax) {
+    if (OUT == kOutFullHit) {
+        reinterpret_cast<float4*>(hits)[i] = hit ? make_float4(u, v, __uint_as_float(inst), __uint_as_float(prim))
+                                                 : make_float4(0.0f, 0.0f, __uint_as_float(kInvalid), __uint_as_float(kInvalid));
+    } else if (OUT == kOutInstanceId) {
+        reinterpret_cast<uint32_t*>(hits)[i] = hit ? (TWO_LEVEL ? inst : prim) : kInvalid;
+    } else {
+        float4* o = reinterpret_cast<float4*>(hits) + 2 * (size_t)i;
+        o[0] = hit ? make_float4(u, v, __uint_as_float(inst), __uint_as_float(prim))
+                   : make_float4(0.0f, 0.0f, __uint_as_float(kInvalid), __uint_as_float(kInvalid));
+        o[1] = make_float4(hit ? t : tmax, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent single-level traversal.
+//
+// The grid holds only as many waves as the GPU keeps resident.  Rays are split
+// into kQueues contiguous ranges, one per XCD (workgroups are dispatched to
+// the XCDs round-robin), so that neighbouring rays share an L2; each wave
+// takes 64 ray indices at a time from its range with one atomic and hands
+// them to lanes as their rays finish, moving on to the other ranges once its
+// own is empty.  Each lane runs a while-while loop (Aila & Laine, "Understanding
+// the Efficiency of Ray Traversal on GPUs", HPG 2009): inner nodes until every
+// busy lane holds a postponed leaf, then the leaves.  Leaves are still tested
+// in depth-first order, so closest and any hits are those of the plain
+// kernel; only the number of boxes tested differs.
+
+__global__ void QueueInitKernel(uint32_t* counters, uint32_t ray_count, const uint32_t* __restrict__ indirect) {
+    uint32_t n = ray_count;
+    if (indirect) n = min(n, *indirect);
+    const uint32_t part = (n + kQueues - 1) / kQueues;
+    const uint32_t k = threadIdx.x;
+    if (k < kQueues) counters[k * kQueueStride] = min(k * part, n);
+}
+
+template <bool ANY, int OUT>
+__global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* __restrict__ accel,
+                                                                const float4* __restrict__ rays, uint32_t ray_count,
+                                                                const uint32_t* __restrict__ indirect,
+                                                                void* __restrict__ hits, uint32_t* __restrict__ scratch,
+                                                                uint32_t min_busy) {
+    __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
+    uint32_t n = ray_count;
+    if (indirect) n = min(n, *indirect);
+    const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
+    const uint32_t total = hdr->node_count;
+    const uint32_t n_int = hdr->prim_count ? hdr->prim_count - 1 : 0;
+    const BvhNode* nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
+    uint32_t* counters = scratch;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = LaneMaskLt();
+    const uint32_t part = (n + kQueues - 1) / kQueues;
+
+    // Wave-uniform queue state.
+    uint32_t q = blockIdx.x % kQueues, tries = 0, q_next = 0, q_end = 0;
+    bool drained = n == 0;
+
+    // Lane state.
+    uint32_t rid = kInvalid;
+    RayState r;
+    float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
+    uint32_t best_prim = kInvalid;
+    uint32_t node = kInvalid, leaf = kInvalid;
+    LaneStack st{s_stack, scratch + kQueueHeadWords + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries, 0, 0};
+
+    for (;;) {
+        // Refill idle lanes.
+        const uint64_t idle = __ballot(rid == kInvalid);
+        if (idle && !drained) {
+            if (q_next == q_end) {
+                for (;;) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(&counters[q * kQueueStride], kFetch);
+                    b = __shfl(b, 0, 64);
+                    const uint32_t end = min((q + 1) * part, n);
+                    if (b < end) {
+                        q_next = b;
+                        q_end = min(b + kFetch, end);
+                        break;
+                    }
+                    if (++tries == kQueues) {
+                        drained = true;
+                        break;
+                    }
+                    q = q + 1 == kQueues ? 0 : q + 1;
+                }
+            }
+            const uint32_t take = min((uint32_t)__popcll(idle), q_end - q_next);
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (rid == kInvalid && rank < take) {
+                rid = q_next + rank;
+                const RayIn ray = LoadRay(rays, rid);
+                SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
+                tmax = ray.b.w;
+                best_t = tmax;
+                best_prim = kInvalid;
+                bV = 0.0f; bW = 0.0f; bdet = 1.0f;
+                limit = WidenLimit(tmax);
+                st.sp = 0;
+                st.gbase = 0;
+                node = total ? 0u : kInvalid;
+                leaf = kInvalid;
+            }
+            q_next += take;
+        }
+        if (__ballot(rid != kInvalid) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (rid == kInvalid) continue;
+
+        // Traverse until the ray ends or fewer than min_busy lanes are still
+        // traversing, then go back and hand new rays to the idle lanes.
+        for (;;) {
+            // Phase A: inner nodes until every traversing lane has postponed a leaf.
+            while (node < n_int) {
+                float4 q0, q1, q2, q3;
+                LoadNode(nodes, node, q0, q1, q2, q3);
+                const uint32_t c0 = __float_as_uint(q0.w);
+                const uint32_t c1 = __float_as_uint(q1.w);
+                float ta, tb;
+                const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
+                const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
+                if (h0 || h1) {
+                    const bool c1first = h1 && (!h0 || ta > tb);
+                    if (h0 && h1) st.push(c1first ? c0 : c1);
+                    node = c1first ? c1 : c0;
+                } else {
+                    node = st.empty() ? kInvalid : st.pop();
+                }
+                if (node >= total && node != kInvalid) node = kInvalid;  // corrupt index: end the ray
+                if (node != kInvalid && node >= n_int && leaf == kInvalid) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                }
+                // Leave for the leaf phase once every descending lane holds a
+                // leaf, or once too few lanes are still descending.
+                if (!__any(leaf == kInvalid) || (uint32_t)__popcll(__ballot(1)) < min_busy) break;
+            }
+            // Phase B: the postponed leaf, then any leaves that come next in order.
+            if (leaf == kInvalid && node != kInvalid && node >= n_int) {
+                leaf = node;
+                node = st.empty() ? kInvalid : st.pop();
+            }
+            bool finished = false;
+            while (leaf != kInvalid) {
+                float4 q0, q1, q2, q3;
+                LoadNode(nodes, leaf, q0, q1, q2, q3);
+                const uint32_t prim = __float_as_uint(q1.w);
+                TriHit h;
+                if (WatertightTest(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) && h.t >= r.tmin &&
+                    h.t <= best_t) {
+                    if (ANY) {
+                        best_t = h.t; best_prim = prim;
+                        bV = h.V; bW = h.W; bdet = h.det;
+                        finished = true;
+                        break;
+                    }
+                    if (CloserHit(h.t, 0u, prim, best_t, 0u, best_prim)) {
+                        best_t = h.t; best_prim = prim;
+                        bV = h.V; bW = h.W; bdet = h.det;
+                        limit = WidenLimit(best_t);
+                    }
+                }
+                leaf = kInvalid;
+                if (node != kInvalid && node >= n_int && node < total) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                }
+            }
+            if (finished || (node == kInvalid && leaf == kInvalid)) {
+                const bool hit = best_prim != kInvalid;
+                WriteHit<OUT, false>(hits, rid, hit, hit ? bV / bdet : 0.0f, hit ? bW / bdet : 0.0f, 0u, best_prim,
+                                     best_t, tmax);
+                rid = kInvalid;
+                node = kInvalid;
+                leaf = kInvalid;
+                break;
+            }
+            if ((uint32_t)__popcll(__ballot(1)) < min_busy) break;
+        }
+    }
+}
+
 template <bool ANY, bool TWO_LEVEL, bool STATS>
 void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, const float4* rays, uint32_t count,
                const uint32_t* indirect, void* hits, uint32_t* scratch, uint32_t* stats) {
@@ -211,16 +415,82 @@ void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, cons
 
 }  // namespace
 
-size_t TraceScratchBytes(uint32_t ray_count) { return AlignUp((size_t)ray_count * kSpillEntries * 4, 256); }
+size_t TraceScratchBytes(uint32_t ray_count) {
+    return AlignUp((size_t)kQueueHeadWords * 4 + (size_t)ray_count * kSpillEntries * 4, 256);
+}
+
+static uint32_t g_min_busy = 0;
+uint32_t MinBusyLanes() {
+    if (!g_min_busy) {
+        const char* e = std::getenv("RR_TRACE_MIN_BUSY");
+        const int v = e ? std::atoi(e) : 0;
+        g_min_busy = (v >= 1 && v <= 64) ? (uint32_t)v : kDefaultMinBusy;
+    }
+    return g_min_busy;
+}
+void SetMinBusyLanes(uint32_t v) { g_min_busy = (v >= 1 && v <= 64) ? v : kDefaultMinBusy; }
+
+static int g_trace_mode = -1;
+int TraceMode() {
+    if (g_trace_mode < 0) {
+        const char* e = std::getenv("RR_TRACE_MODE");
+        g_trace_mode = (e && std::strcmp(e, "plain") == 0) ? kTraceModePlain : kTraceModePersistent;
+    }
+    return g_trace_mode;
+}
+void SetTraceMode(int mode) { g_trace_mode = mode; }
+
+namespace {
+
+// Resident waves of the persistent kernel (queried once per device).
+uint32_t PersistentBlocks() {
+    static uint32_t blocks[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!blocks[dev]) {
+        int per_cu = 0, cus = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, TracePersistentKernel<false, kOutFullHit>, kBlock, 0);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        blocks[dev] = (uint32_t)std::max(1, per_cu * std::max(1, cus));
+    }
+    return blocks[dev];
+}
+
+template <bool ANY>
+void LaunchPersistent(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
+                      const uint32_t* ind, void* hits, uint32_t* sc) {
+    QueueInitKernel<<<1, 64, 0, s>>>(sc, n, ind);
+    switch (out) {
+        case kOutFullHit:
+            TracePersistentKernel<ANY, kOutFullHit><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, MinBusyLanes());
+            break;
+        case kOutInstanceId:
+            TracePersistentKernel<ANY, kOutInstanceId><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, MinBusyLanes());
+            break;
+        default:
+            TracePersistentKernel<ANY, kOutFullHitT><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, MinBusyLanes());
+            break;
+    }
+}
+
+}  // namespace
 
 void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output, const void* rays, uint32_t ray_count,
            const uint32_t* indirect, void* hits, void* scratch, hipStream_t s) {
     if (ray_count == 0) return;
-    const dim3 g((ray_count + kBlock - 1) / kBlock);
     const uint8_t* a = static_cast<const uint8_t*>(accel);
     const float4* r = static_cast<const float4*>(rays);
     uint32_t* sc = static_cast<uint32_t*>(scratch);
     const bool two = kind == kAccelTlas;
+    if (!two && TraceMode() != kTraceModePlain) {
+        // Never more lanes than rays, so per-lane spill stacks fit the scratch.
+        const uint32_t blocks = std::min(PersistentBlocks(), (ray_count + kBlock - 1) / kBlock);
+        if (query == kQueryAny) LaunchPersistent<true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        else LaunchPersistent<false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        return;
+    }
+    const dim3 g((ray_count + kBlock - 1) / kBlock);
     if (query == kQueryAny) {
         if (two) LaunchOut<true, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
         else LaunchOut<true, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);

@@ -179,3 +179,20 @@ def test_sort_pairs(ctx, dev, n):
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(k.cpu().numpy().view(np.uint32), keys[order])
     assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])
+
+
+@pytest.mark.parametrize("query", ["closest", "any"])
+def test_visit_counters_match_oracle_stats(ctx, dev, query):
+    say(f"plain-kernel hits and TraversalStats vs oracle, {query}")
+    import torch
+
+    w = scenes.heightfield(res=256, n_rays=200000)
+    accel, buf = _build(ctx, dev, w.vertices, w.indices, True)
+    hits, stats = ctx.intersect_with_stats(accel, to_dev(w.rays, dev), query)
+    torch.cuda.synchronize()
+    ref, rst = oracle.trace(buf, w.rays, any_hit=(query == "any"), stats=True)
+    ok, msg = hits_equal(api.decode_hits(hits, "full_t"), ref)
+    assert ok, msg
+    st = stats.cpu().numpy().view(np.uint32)
+    for k, f in enumerate(("internal_nodes", "leaves", "box_tests", "tri_tests")):
+        assert np.array_equal(st[:, k], rst[f]), f
