@@ -88,6 +88,82 @@ def cpu_baseline(buf, rays, threads, budget_s=20.0):
             "sample": f"{reps} x (closest + any) over all {n} rays of the same BVH, oracle/oracle.cpp OpenMP"}
 
 
+def serving_bench(ctx, dev, n_requests, log):
+    """The serving stream: 32 resident scenes (log-uniform 10k-2M triangles,
+    soups and heightfields), requests of log-uniform 1k-1M rays sliced from
+    camera / sphere ray pools, 70% closest / 30% any-hit, FULL_HIT output.
+    The plain path (one rrCmdIntersect + submit + wait per request) and the
+    batched server run the same requests; both are first run closed-loop
+    (everything released at once) for their capacity, then the plain path at
+    70% of its capacity in requests/s with Poisson arrivals, and the batched
+    server over a sweep of Poisson rates.  The comparison is the highest
+    batched throughput whose p99 latency is at most the plain path's p99."""
+    import torch
+
+    from radeonrays_sdk_amd import scenes, serving
+
+    log("serving: generating 32 scenes")
+    sc = scenes.serving_scenes()
+    accels = []
+    for s in sc:
+        accels.append(ctx.build_geometry(torch.from_numpy(s.vertices).to(dev), torch.from_numpy(s.indices).to(dev),
+                                         restructure=True))
+    pool_rays = 1 << 20
+    pools = [torch.from_numpy(p).to(dev) for p in scenes.serving_ray_pools(pool_rays)]
+    reqs = scenes.serving_requests(n_requests, sc, pool_rays)
+    counts = np.array([q.count for q in reqs], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(counts * HIT_BYTES)])
+    arena = ctx.empty(int(offs[-1]))
+    served = [serving.Request(accels[q.scene], pools[q.pool], q.offset * RAY_BYTES, q.count, q.query, "full",
+                              arena[int(offs[k]):int(offs[k + 1])]) for k, q in enumerate(reqs)]
+    torch.cuda.synchronize()
+    resident = sum(a.buffer.numel() for a in accels) + sum(p.numel() * 4 for p in pools)
+    budget = (200 << 30) - resident
+
+    def run(mode, qps=None, stream=0):
+        with serving.Server(ctx, mode=mode, max_inflight=3, max_batch_requests=256, max_batch_rays=4 << 20,
+                            memory_budget=budget) as srv:
+            rel = None
+            if qps is not None:
+                t0 = srv.now() + 5_000_000
+                rel = [t0 + int(a) for a in scenes.poisson_arrivals(len(served), qps, stream=stream)]
+            first = srv.submit(served, rel)
+            srv.drain()
+            t = srv.timings(first, len(served))
+        out = serving.latency_summary(t, counts)
+        if qps is not None:
+            out["qps"] = round(qps, 1)
+        return out
+
+    log("serving: closed-loop capacity")
+    run("plain")  # warm-up
+    run("batched")
+    plain_cap = run("plain")
+    batched_cap = run("batched")
+    cap_qps = plain_cap["requests"] / plain_cap["span_s"]
+    log(f"serving: plain capacity {plain_cap['mrays_s']} Mrays/s ({cap_qps:.0f} req/s), "
+        f"batched {batched_cap['mrays_s']} Mrays/s")
+    plain_load = run("plain", 0.7 * cap_qps, stream=1)
+    sweep = []
+    for f in (0.7, 1.0, 1.4, 2.0, 2.8, 4.0, 5.6):
+        r = run("batched", f * cap_qps, stream=1)
+        sweep.append(r)
+        log(f"serving: batched at {r['qps']} req/s -> {r['mrays_s']} Mrays/s p50 {r['p50_ms']} p99 {r['p99_ms']} ms")
+        if r["mrays_s"] < 0.8 * r["qps"] * counts.mean() / 1e6:
+            break  # past capacity: the backlog only grows
+    ok = [r for r in sweep if r["p99_ms"] <= plain_load["p99_ms"]]
+    best = max(ok, key=lambda r: r["mrays_s"]) if ok else None
+    return {
+        "workload": f"32 resident scenes ({sum(s.indices.shape[0] for s in sc)} tris, log-uniform 10k-2M), "
+                    f"{n_requests} requests of log-uniform 1k-1M rays, 70% closest / 30% any, Poisson arrivals",
+        "mean_rays_per_request": round(float(counts.mean()), 1),
+        "plain_capacity": plain_cap, "batched_capacity": batched_cap,
+        "plain_at_70pct_load": plain_load, "batched_sweep": sweep,
+        "batched_at_plain_p99": best,
+        "batched_over_plain_mrays_at_equal_p99": round(best["mrays_s"] / plain_load["mrays_s"], 3) if best else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +171,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip build/refit/sort timings")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream comparison")
+    ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -211,6 +289,9 @@ def main():
         out["sort16m_ms"] = round(sort_ms, 4)
         out["sort16m_gbs"] = round(sort_gbs, 1)
         out["sort16m_hbm_frac"] = round(sort_gbs / HBM_PEAK_GBS, 4)
+
+    if not args.no_serving:
+        out["serving"] = serving_bench(ctx, dev, args.serving_requests, log)
 
     if args.calibrate_copy_mb:
         src = torch.ones(args.calibrate_copy_mb << 18, dtype=torch.float32, device=dev)

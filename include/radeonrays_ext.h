@@ -46,6 +46,98 @@ RR_API RRError rrxCmdIntersectWithStats(RRContext        context,
                                         RRDevicePtr      stats,
                                         RRCommandStream  command_stream);
 
+/* One query of a batch: the arguments of one rrCmdIntersect call
+ * (radeonrays.h:391-400) without the indirect ray count. */
+typedef struct RRxIntersectRequest
+{
+    RRDevicePtr            scene_buffer;  /* geometry (BLAS) or scene (TLAS) result buffer */
+    RRDevicePtr            rays;          /* ray_count RRRay records */
+    RRDevicePtr            hits;          /* ray_count records of query_output's type */
+    uint32_t               ray_count;
+    RRIntersectQuery       query;
+    RRIntersectQueryOutput query_output;
+    uint32_t               padding;
+} RRxIntersectRequest;
+
+/* Scratch bytes for rrxCmdIntersectBatch over request_count requests holding
+ * total_ray_count rays, none with more than max_request_rays rays. */
+RR_API RRError rrxGetIntersectBatchMemoryRequirements(RRContext context,
+                                                      uint32_t  request_count,
+                                                      uint64_t  total_ray_count,
+                                                      uint32_t  max_request_rays,
+                                                      size_t*   scratch_size);
+
+/* Run request_count queries as one submission.  Every geometry (BLAS)
+ * request shares one persistent traversal launch whose waves take 64-ray
+ * chunks of any request, so small requests fill the GPU together; scene
+ * (TLAS) requests follow, one launch each.  Each request's hits are those an
+ * rrCmdIntersect call with the same arguments writes.  No counterpart in the
+ * reference API (its rrCmdIntersect runs one query per dispatch,
+ * vlk/intersector.cpp:269-339).  The request array is copied during the
+ * call. */
+RR_API RRError rrxCmdIntersectBatch(RRContext                  context,
+                                    const RRxIntersectRequest* requests,
+                                    uint32_t                   request_count,
+                                    RRDevicePtr                scratch,
+                                    RRCommandStream            command_stream);
+
+/* ------------------------------------------------------------------------
+ * Serving loop: a host thread that admits queued requests under a memory
+ * budget and dispatches them to the GPU.  In batched mode every dispatch is
+ * one rrxCmdIntersectBatch over all admitted requests (up to the batch
+ * limits) on one of max_inflight streams, so requests that arrive while
+ * others run are coalesced into the next launch and launches overlap.  In
+ * plain mode every request is one rrCmdIntersect + rrSumbitCommandStream +
+ * rrWaitEvent, one after another: the per-call path, for comparison.
+ * Times are nanoseconds of the server's steady clock (rrxServerNow). */
+typedef struct _RRxServer* RRxServer;
+
+typedef enum RRxServeMode
+{
+    RRX_SERVE_BATCHED = 0,
+    RRX_SERVE_PLAIN = 1
+} RRxServeMode;
+
+typedef struct RRxServerOptions
+{
+    RRxServeMode mode;
+    uint32_t     max_inflight;        /* concurrent launches (batched mode), >= 1 */
+    uint32_t     max_batch_requests;  /* requests per launch */
+    uint32_t     max_batch_rays;      /* rays per launch; also the largest request */
+    uint64_t     memory_budget;       /* bytes of admitted ray + hit buffers */
+} RRxServerOptions;
+
+typedef struct RRxServedTiming
+{
+    uint64_t arrival_ns;   /* max(release time, submission time) */
+    uint64_t dispatch_ns;  /* launch enqueued */
+    uint64_t complete_ns;  /* completion seen by the server */
+    uint32_t batch_id;     /* dispatch the request went out in */
+    uint32_t batch_size;   /* requests in that dispatch */
+} RRxServedTiming;
+
+RR_API RRError rrxCreateServer(RRContext context, const RRxServerOptions* options, RRxServer* server);
+RR_API RRError rrxDestroyServer(RRxServer server);
+RR_API uint64_t rrxServerNow(RRxServer server);
+
+/* Queue count requests; request i is admitted no earlier than release_ns[i]
+ * (null: now).  Tickets first_ticket .. first_ticket + count - 1 name them.
+ * Requests larger than max_batch_rays or the memory budget are rejected. */
+RR_API RRError rrxServerSubmit(RRxServer                  server,
+                               const RRxIntersectRequest* requests,
+                               const uint64_t*            release_ns,
+                               uint32_t                   count,
+                               uint64_t*                  first_ticket);
+
+/* Block until every request submitted so far has completed. */
+RR_API RRError rrxServerDrain(RRxServer server);
+
+/* Timings of tickets [first_ticket, first_ticket + count), completed ones only. */
+RR_API RRError rrxServerGetTimings(RRxServer server, uint64_t first_ticket, uint32_t count, RRxServedTiming* timings);
+
+/* Peak bytes admitted at once since creation. */
+RR_API RRError rrxServerGetPeakAdmitted(RRxServer server, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,6 +195,34 @@ class Context:
                      "rrxCmdIntersectWithStats")
         return hits, stats
 
+    def batch_scratch(self, request_count: int, total_rays: int, max_rays: int):
+        size = c_size_t()
+        rr.check(self.lib.rrxGetIntersectBatchMemoryRequirements(self.ctx, request_count, total_rays, max_rays,
+                                                                 byref(size)),
+                 "rrxGetIntersectBatchMemoryRequirements")
+        return self.empty(size.value)
+
+    def intersect_batch(self, requests, scratch=None):
+        """rrxCmdIntersectBatch over requests = [(accel, rays, query, output[, hits])]; returns the hit buffers."""
+        outs, addrs = [], []
+        for req in requests:
+            accel, rays, query, output = req[:4]
+            n = int(rays.shape[0])
+            hits = req[4] if len(req) > 4 and req[4] is not None else self.empty(n * _HIT_BYTES[output])
+            outs.append(hits)
+            addrs += [accel.address, rays.data_ptr() if n else None, hits.data_ptr()]
+        total = sum(int(r[1].shape[0]) for r in requests)
+        largest = max([int(r[1].shape[0]) for r in requests] + [1])
+        scratch = scratch if scratch is not None else self.batch_scratch(len(requests), total, largest)
+        with self.ptrs(*addrs, scratch.data_ptr()) as hs:
+            arr = (rr.RRxIntersectRequest * max(len(requests), 1))()
+            for k, req in enumerate(requests):
+                a, r, h = hs[3 * k: 3 * k + 3]
+                arr[k] = rr.RRxIntersectRequest(a, r, h, int(req[1].shape[0]), _QUERY[req[2]], _OUTPUT[req[3]], 0)
+            rr.check(self.lib.rrxCmdIntersectBatch(self.ctx, arr, len(requests), hs[-1], self.cs),
+                     "rrxCmdIntersectBatch")
+        return outs
+
     def sort_pairs(self, keys, values, key_bits: int = 32, tmp=None, scratch=None):
         """Stable radix sort of uint32 (stored as int32) key/value tensors in place."""
         torch = _torch()

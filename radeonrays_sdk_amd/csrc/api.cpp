@@ -13,6 +13,8 @@
 // wrapped from a client hipStream_t enqueues immediately on that stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -21,12 +23,14 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/radeonrays.h"
 #include "../../include/radeonrays_ext.h"
 #include "../../include/radeonrays_hip.h"
+#include "api_internal.h"
 #include "build.h"
 #include "rr_layout.h"
 #include "sort.h"
@@ -115,13 +119,53 @@ struct Context {
     std::unordered_map<uintptr_t, rr::AccelKind> kinds;  // result buffer -> BLAS/TLAS
     std::vector<PendingFree> pending;                      // pinned staging to free once done
     std::vector<Event*> event_pool;
+    // Pinned staging blocks kept for reuse (hipHostMalloc costs far more than
+    // the small uploads it serves), at most kPinnedPoolBytes in all.
+    static constexpr size_t kPinnedPoolBytes = size_t(64) << 20;
+    std::unordered_map<void*, size_t> pinned_size;
+    std::vector<void*> pinned_free;
+    size_t pinned_free_bytes = 0;
+
+    // Pinned host block of at least `bytes` (caller holds mu).
+    void* AcquirePinned(size_t bytes) {
+        Reap(false);
+        for (size_t k = 0; k < pinned_free.size(); ++k) {
+            void* p = pinned_free[k];
+            const size_t sz = pinned_size[p];
+            if (sz >= bytes) {
+                pinned_free.erase(pinned_free.begin() + k);
+                pinned_free_bytes -= sz;
+                return p;
+            }
+        }
+        const size_t sz = std::max<size_t>(bytes, 4096);
+        void* p = nullptr;
+        Check(hipHostMalloc(&p, sz, hipHostMallocDefault), "hipHostMalloc");
+        pinned_size[p] = sz;
+        return p;
+    }
+    void ReleasePinned(void* p) {
+        auto it = pinned_size.find(p);
+        if (it == pinned_size.end() || pinned_free_bytes + it->second > kPinnedPoolBytes) {
+            if (it != pinned_size.end()) pinned_size.erase(it);
+            (void)hipHostFree(p);
+            return;
+        }
+        pinned_free.push_back(p);
+        pinned_free_bytes += it->second;
+    }
+    void FreePool() {
+        for (void* p : pinned_free) (void)hipHostFree(p);
+        pinned_free.clear();
+        pinned_free_bytes = 0;
+    }
 
     void Reap(bool wait_all) {
         std::vector<PendingFree> keep;
         for (auto& p : pending) {
             const hipError_t q = wait_all ? hipEventSynchronize(p.ev) : hipEventQuery(p.ev);
             if (q == hipSuccess) {
-                (void)hipHostFree(p.host);
+                ReleasePinned(p.host);
                 (void)hipEventDestroy(p.ev);
             } else {
                 keep.push_back(p);
@@ -219,6 +263,15 @@ RRError MeshFromInput(const RRGeometryBuildInput* in, rr::MeshDesc& m, const cha
     return RR_SUCCESS;
 }
 
+// How long rrWaitEvent polls before it blocks (RR_WAIT_SPIN_US, default 2000).
+long WaitSpinMicros() {
+    static const long us = [] {
+        const char* e = std::getenv("RR_WAIT_SPIN_US");
+        return e ? std::max(0L, std::atol(e)) : 2000L;
+    }();
+    return us;
+}
+
 bool WantsRestructure(const RRBuildOptions* o) {
     // intersector.cpp:116,170: restructure only with options that do not ask for a fast build.
     return o && (o->build_flags & RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD) == 0;
@@ -276,6 +329,7 @@ RRError rrDestroyContext(RRContext context) {
         DeviceGuard g(c->device);
         (void)hipStreamSynchronize(c->stream);
         c->Reap(true);
+        c->FreePool();
         for (Event* e : c->event_pool) {
             (void)hipEventDestroy(e->ev);
             delete e;
@@ -396,8 +450,10 @@ RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input,
         // recorded upload reads at execution time.
         rr::InstanceUpload* host = nullptr;
         if (count) {
-            Check(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(rr::InstanceUpload) * count, hipHostMallocDefault),
-                  "hipHostMalloc");
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                host = static_cast<rr::InstanceUpload*>(c->AcquirePinned(sizeof(rr::InstanceUpload) * count));
+            }
             for (uint32_t i = 0; i < count; ++i) {
                 std::memcpy(host[i].xform.m, build_input->instances[i].transform, sizeof(rr::Transform));
                 host[i].blas = reinterpret_cast<uint64_t>(Addr(build_input->instances[i].geometry));
@@ -555,7 +611,18 @@ RRError rrWaitEvent(RRContext context, RREvent event) {
     RR_REQUIRE(context && event, "rrWaitEvent");
     return Guarded("rrWaitEvent", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
-        Check(hipEventSynchronize(AsEvent(event)->ev), "hipEventSynchronize");
+        // Poll first (short queries finish in tens of microseconds, less than
+        // a blocking wait takes to wake up), then block.
+        const hipEvent_t ev = AsEvent(event)->ev;
+        const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(WaitSpinMicros());
+        for (;;) {
+            const hipError_t q = hipEventQuery(ev);
+            if (q == hipSuccess) return RR_SUCCESS;
+            if (q != hipErrorNotReady) Check(q, "hipEventQuery");
+            if (std::chrono::steady_clock::now() >= until) break;
+            std::this_thread::yield();
+        }
+        Check(hipEventSynchronize(ev), "hipEventSynchronize");
         return RR_SUCCESS;
     });
 }
@@ -724,4 +791,108 @@ RRError rrxCmdIntersectWithStats(RRContext context, RRDevicePtr scene_buffer, RR
     });
 }
 
+RRError rrxGetIntersectBatchMemoryRequirements(RRContext context, uint32_t request_count, uint64_t total_ray_count,
+                                               uint32_t max_request_rays, size_t* scratch_size) {
+    RR_REQUIRE(context && scratch_size, "rrxGetIntersectBatchMemoryRequirements");
+    RR_REQUIRE(total_ray_count < (uint64_t(1) << 37), "rrxGetIntersectBatchMemoryRequirements");
+    return Guarded("rrxGetIntersectBatchMemoryRequirements", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        // Geometry requests share the batch launch; a scene request runs the
+        // per-ray-stack kernel on the same scratch afterwards.
+        *scratch_size = std::max(rr::TraceBatchScratchBytes(request_count, total_ray_count),
+                                 rr::TraceScratchBytes(max_request_rays));
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* requests, uint32_t request_count,
+                             RRDevicePtr scratch, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrxCmdIntersectBatch");
+    RR_REQUIRE(context && scratch && command_stream, "rrxCmdIntersectBatch");
+    RR_REQUIRE(request_count == 0 || requests, "rrxCmdIntersectBatch");
+    for (uint32_t i = 0; i < request_count; ++i) {
+        const RRxIntersectRequest& q = requests[i];
+        RR_REQUIRE(q.scene_buffer && (q.ray_count == 0 || (q.rays && q.hits)), "rrxCmdIntersectBatch");
+        RR_REQUIRE(q.query == RR_INTERSECT_QUERY_CLOSEST || q.query == RR_INTERSECT_QUERY_ANY, "rrxCmdIntersectBatch");
+        RR_REQUIRE(q.query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT ||
+                       q.query_output == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
+                       q.query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
+                   "rrxCmdIntersectBatch");
+    }
+    return Guarded("rrxCmdIntersectBatch", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        CommandStream* cs = AsStream(command_stream);
+        DeviceGuard g(c->device);
+        // Split: geometry requests go into the batch table, scene requests
+        // run one launch each after it.
+        struct Single {
+            void* accel;
+            rr::QueryKind q;
+            rr::OutputKind o;
+            const void* rays;
+            uint32_t n;
+            void* hits;
+        };
+        std::vector<rr::BatchRequest> table;
+        std::vector<Single> singles;
+        uint64_t chunks = 0;
+        for (uint32_t i = 0; i < request_count; ++i) {
+            const RRxIntersectRequest& q = requests[i];
+            if (q.ray_count == 0) continue;
+            void* accel = Addr(q.scene_buffer);
+            rr::AccelKind kind;
+            const RRError e = KindOf(c, accel, kind);
+            if (e != RR_SUCCESS) return e;
+            const auto qk = q.query == RR_INTERSECT_QUERY_ANY ? rr::kQueryAny : rr::kQueryClosest;
+            const auto ok = static_cast<rr::OutputKind>(q.query_output);
+            if (kind == rr::kAccelTlas) {
+                singles.push_back({accel, qk, ok, Addr(q.rays), q.ray_count, Addr(q.hits)});
+                continue;
+            }
+            rr::BatchRequest b{};
+            b.accel = reinterpret_cast<uint64_t>(accel);
+            b.rays = reinterpret_cast<uint64_t>(Addr(q.rays));
+            b.hits = reinterpret_cast<uint64_t>(Addr(q.hits));
+            b.ray_count = q.ray_count;
+            b.flags = (qk == rr::kQueryAny ? 1u : 0u) | (uint32_t(ok) << 1);
+            b.first_chunk = uint32_t(chunks);
+            chunks += (q.ray_count + rr::kBatchChunk - 1) / rr::kBatchChunk;
+            RR_REQUIRE(chunks < (uint64_t(1) << 32), "rrxCmdIntersectBatch");
+            table.push_back(b);
+        }
+        const uint32_t n_req = uint32_t(table.size());
+        const uint32_t n_chunks = uint32_t(chunks);
+        void* sc = Addr(scratch);
+        void* host = nullptr;
+        const size_t table_bytes = sizeof(rr::BatchRequest) * n_req;
+        if (n_req) {
+            std::lock_guard<std::mutex> lk(c->mu);
+            host = c->AcquirePinned(table_bytes);
+        }
+        if (host) std::memcpy(host, table.data(), table_bytes);
+        auto cmd = [=](hipStream_t s) {
+            if (n_req) {
+                auto* dev_table =
+                    reinterpret_cast<rr::BatchRequest*>(static_cast<uint8_t*>(sc) + rr::kBatchTableOffset);
+                Check(hipMemcpyAsync(dev_table, host, table_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+                rr::TraceBatch(dev_table, n_req, n_chunks, sc, s);
+            }
+            for (const Single& x : singles)
+                rr::Trace(x.accel, rr::kAccelTlas, x.q, x.o, x.rays, x.n, nullptr, x.hits, sc, s);
+        };
+        Record(cs, cmd);
+        if (host) {
+            if (cs->external) {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->FreeAfter(cs->stream, host);
+            } else {
+                cs->staging.push_back(host);
+            }
+        }
+        return RR_SUCCESS;
+    });
+}
+
 }  // extern "C"
+
+int rr::ContextDevice(RRContext context) { return AsCtx(context)->device; }

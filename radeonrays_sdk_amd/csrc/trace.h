@@ -37,6 +37,29 @@ enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };
 void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output, const void* rays, uint32_t ray_count,
            const uint32_t* indirect, void* hits, void* scratch, hipStream_t stream);
 
+// One request of a batched (coalesced) query: a single-level BVH, its rays
+// and its hit buffer.  flags: bit 0 any-hit, bits 1-2 OutputKind.
+struct BatchRequest {
+    uint64_t accel;
+    uint64_t rays;
+    uint64_t hits;
+    uint32_t ray_count;
+    uint32_t flags;
+    uint32_t first_chunk;  // chunks of kBatchChunk rays before this request
+    uint32_t pad[3];
+};
+static_assert(sizeof(BatchRequest) == 48, "BatchRequest must be 48 bytes");
+constexpr uint32_t kBatchChunk = 64;
+
+// Scratch for a batch of `requests` requests and `total_rays` rays; the
+// request table is uploaded to byte kBatchTableOffset of it.
+size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays);
+constexpr size_t kBatchTableOffset = 1024;
+
+// Trace every request of the table (already in device memory) in one
+// persistent launch; requests may name different BVHs, queries and outputs.
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, void* scratch, hipStream_t stream);
+
 // Same traversal with per-ray visit counters written to `stats`
 // (uint32 x4 per ray: internal nodes visited, leaves visited, box tests,
 // triangle tests), for the roofline's algorithmic byte count.

@@ -20,6 +20,7 @@
 #include "trace.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 
@@ -413,6 +414,163 @@ void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched traversal: one persistent launch over many requests.  Waves take
+// 64-ray chunks from one counter; a chunk belongs to one request (found by
+// binary search over first_chunk), and each lane carries its own request's
+// BVH, query and output, so lanes of a wave may serve different requests.
+__global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* __restrict__ table, uint32_t requests,
+                                                           uint32_t total_chunks, uint32_t* __restrict__ scratch,
+                                                           uint32_t min_busy) {
+    __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
+    uint32_t* counter = scratch;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = LaneMaskLt();
+    const size_t spill_base = (kBatchTableOffset + (((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255)) / 4;
+
+    // Wave-uniform queue: rays [q_next, q_end) of request q_req.
+    uint32_t q_req = 0, q_next = 0, q_end = 0;
+    bool drained = false;
+
+    // Lane state.
+    uint32_t rid = kInvalid, lreq = 0, lflags = 0, n_int = 0, total = 0;
+    const BvhNode* nodes = nullptr;
+    RayState r;
+    float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
+    uint32_t best_prim = kInvalid;
+    uint32_t node = kInvalid, leaf = kInvalid;
+    LaneStack st{s_stack, scratch + spill_base + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries, 0, 0};
+
+    for (;;) {
+        const uint64_t idle = __ballot(rid == kInvalid);
+        if (idle && !drained) {
+            if (q_next == q_end) {
+                uint32_t c = 0;
+                if (lane == 0) c = atomicAdd(counter, 1u);
+                c = __shfl(c, 0, 64);
+                if (c >= total_chunks) {
+                    drained = true;
+                } else {
+                    uint32_t lo = 0, hi = requests;  // last request with first_chunk <= c
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) / 2;
+                        if (table[mid].first_chunk <= c) lo = mid;
+                        else hi = mid;
+                    }
+                    q_req = lo;
+                    q_next = (c - table[lo].first_chunk) * kBatchChunk;
+                    q_end = min(q_next + kBatchChunk, table[lo].ray_count);
+                }
+            }
+            const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (rid == kInvalid && rank < take) {
+                rid = q_next + rank;
+                lreq = q_req;
+                const BatchRequest& rq = table[q_req];
+                lflags = rq.flags;
+                const uint8_t* accel = reinterpret_cast<const uint8_t*>(rq.accel);
+                const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
+                total = hdr->node_count;
+                n_int = hdr->prim_count ? hdr->prim_count - 1 : 0;
+                nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
+                const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
+                SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
+                tmax = ray.b.w;
+                best_t = tmax;
+                best_prim = kInvalid;
+                bV = 0.0f; bW = 0.0f; bdet = 1.0f;
+                limit = WidenLimit(tmax);
+                st.sp = 0;
+                st.gbase = 0;
+                node = total ? 0u : kInvalid;
+                leaf = kInvalid;
+            }
+            q_next += take;
+        }
+        if (__ballot(rid != kInvalid) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (rid == kInvalid) continue;
+
+        const bool any = (lflags & 1u) != 0;
+        for (;;) {
+            while (node < n_int) {
+                float4 q0, q1, q2, q3;
+                LoadNode(nodes, node, q0, q1, q2, q3);
+                const uint32_t c0 = __float_as_uint(q0.w);
+                const uint32_t c1 = __float_as_uint(q1.w);
+                float ta, tb;
+                const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
+                const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
+                if (h0 || h1) {
+                    const bool c1first = h1 && (!h0 || ta > tb);
+                    if (h0 && h1) st.push(c1first ? c0 : c1);
+                    node = c1first ? c1 : c0;
+                } else {
+                    node = st.empty() ? kInvalid : st.pop();
+                }
+                if (node >= total && node != kInvalid) node = kInvalid;
+                if (node != kInvalid && node >= n_int && leaf == kInvalid) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                }
+                if (!__any(leaf == kInvalid) || (uint32_t)__popcll(__ballot(1)) < min_busy) break;
+            }
+            if (leaf == kInvalid && node != kInvalid && node >= n_int) {
+                leaf = node;
+                node = st.empty() ? kInvalid : st.pop();
+            }
+            bool finished = false;
+            while (leaf != kInvalid) {
+                float4 q0, q1, q2, q3;
+                LoadNode(nodes, leaf, q0, q1, q2, q3);
+                const uint32_t prim = __float_as_uint(q1.w);
+                TriHit h;
+                if (WatertightTest(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) && h.t >= r.tmin &&
+                    h.t <= best_t) {
+                    if (any) {
+                        best_t = h.t; best_prim = prim;
+                        bV = h.V; bW = h.W; bdet = h.det;
+                        finished = true;
+                        break;
+                    }
+                    if (CloserHit(h.t, 0u, prim, best_t, 0u, best_prim)) {
+                        best_t = h.t; best_prim = prim;
+                        bV = h.V; bW = h.W; bdet = h.det;
+                        limit = WidenLimit(best_t);
+                    }
+                }
+                leaf = kInvalid;
+                if (node != kInvalid && node >= n_int && node < total) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                }
+            }
+            if (finished || (node == kInvalid && leaf == kInvalid)) {
+                const bool hit = best_prim != kInvalid;
+                const float u = hit ? bV / bdet : 0.0f, v = hit ? bW / bdet : 0.0f;
+                void* out = reinterpret_cast<void*>(table[lreq].hits);
+                switch ((lflags >> 1) & 3u) {
+                    case kOutFullHit: WriteHit<kOutFullHit, false>(out, rid, hit, u, v, 0u, best_prim, best_t, tmax); break;
+                    case kOutInstanceId: WriteHit<kOutInstanceId, false>(out, rid, hit, u, v, 0u, best_prim, best_t, tmax); break;
+                    default: WriteHit<kOutFullHitT, false>(out, rid, hit, u, v, 0u, best_prim, best_t, tmax); break;
+                }
+                rid = kInvalid;
+                node = kInvalid;
+                leaf = kInvalid;
+                break;
+            }
+            if ((uint32_t)__popcll(__ballot(1)) < min_busy) break;
+        }
+    }
+}
+
+__global__ void BatchInitKernel(uint32_t* counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
 }  // namespace
 
 size_t TraceScratchBytes(uint32_t ray_count) {
@@ -443,19 +601,29 @@ void SetTraceMode(int mode) { g_trace_mode = mode; }
 namespace {
 
 // Resident waves of the persistent kernel (queried once per device).
-uint32_t PersistentBlocks() {
-    static uint32_t blocks[64] = {};
+// Resident blocks of `kernel` on the current device (cached per device;
+// which = 0 for the single-request kernel, 1 for the batch kernel).
+uint32_t ResidentBlocks(const void* kernel, int which) {
+    static std::atomic<uint32_t> blocks[2][64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) dev = 0;
-    if (!blocks[dev]) {
+    uint32_t b = blocks[which][dev].load(std::memory_order_relaxed);
+    if (!b) {
         int per_cu = 0, cus = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, TracePersistentKernel<false, kOutFullHit>, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        blocks[dev] = (uint32_t)std::max(1, per_cu * std::max(1, cus));
+        b = (uint32_t)std::max(1, per_cu * std::max(1, cus));
+        blocks[which][dev].store(b, std::memory_order_relaxed);
     }
-    return blocks[dev];
+    return b;
 }
+
+uint32_t PersistentBlocks() {
+    return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit>), 0);
+}
+
+uint32_t BatchBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatchKernel), 1); }
 
 template <bool ANY>
 void LaunchPersistent(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
@@ -515,6 +683,23 @@ void TraceWithStats(const void* accel, AccelKind kind, QueryKind query, const vo
         if (two) TraceKernel<false, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats);
         else TraceKernel<false, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats);
     }
+}
+
+size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays) {
+    // The launch has at most one block per 4 chunks, and the chunks number
+    // fewer than total_rays / kBatchChunk + requests.
+    const uint64_t lanes = std::min<uint64_t>((uint64_t)BatchBlocks() * kBlock,
+                                              AlignUp(total_rays + (uint64_t)requests * kBatchChunk, kBlock));
+    return kBatchTableOffset + AlignUp((size_t)requests * sizeof(BatchRequest), 256) +
+           AlignUp((size_t)lanes * kSpillEntries * 4, 256);
+}
+
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, void* scratch, hipStream_t s) {
+    if (requests == 0 || total_chunks == 0) return;
+    uint32_t* sc = static_cast<uint32_t*>(scratch);
+    const uint32_t blocks = std::min<uint64_t>(BatchBlocks(), ((uint64_t)total_chunks * kBatchChunk + kBlock - 1) / kBlock);
+    BatchInitKernel<<<1, 64, 0, s>>>(sc);
+    TraceBatchKernel<<<blocks, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes());
 }
 
 }  // namespace rr

@@ -9,7 +9,7 @@ the helpers in radeonrays_sdk_amd.api take them from torch tensors.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, Structure, c_char_p, c_float, c_int, c_size_t, c_uint32, c_void_p
+from ctypes import POINTER, Structure, c_char_p, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libradeonrays.so"
@@ -49,6 +49,9 @@ RR_INTERSECT_QUERY_ANY = 1
 RR_INTERSECT_QUERY_OUTPUT_FULL_HIT = 0
 RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID = 1
 RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T = 2
+
+RRX_SERVE_BATCHED = 0
+RRX_SERVE_PLAIN = 1
 
 RR_API_VERSION = 0 * 1000000 + 1 * 1000 + 1
 RR_INVALID = 0xFFFFFFFF
@@ -97,6 +100,21 @@ class RRMemoryRequirements(Structure):
                 ("result_buffer_size", c_size_t)]
 
 
+class RRxIntersectRequest(Structure):
+    _fields_ = [("scene_buffer", c_void_p), ("rays", c_void_p), ("hits", c_void_p), ("ray_count", c_uint32),
+                ("query", c_int), ("query_output", c_int), ("padding", c_uint32)]
+
+
+class RRxServerOptions(Structure):
+    _fields_ = [("mode", c_int), ("max_inflight", c_uint32), ("max_batch_requests", c_uint32),
+                ("max_batch_rays", c_uint32), ("memory_budget", c_uint64)]
+
+
+class RRxServedTiming(Structure):
+    _fields_ = [("arrival_ns", c_uint64), ("dispatch_ns", c_uint64), ("complete_ns", c_uint64),
+                ("batch_id", c_uint32), ("batch_size", c_uint32)]
+
+
 # Every exported entry point with its argument types; tests check that the
 # library exports exactly what the headers declare.
 _SIGNATURES = {
@@ -133,8 +151,17 @@ _SIGNATURES = {
     "rrxGetSortMemoryRequirements": [c_void_p, c_uint32, POINTER(c_size_t)],
     "rrxCmdSortPairs": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p],
     "rrxCmdIntersectWithStats": [c_void_p, c_void_p, c_int, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p],
+    "rrxGetIntersectBatchMemoryRequirements": [c_void_p, c_uint32, c_uint64, c_uint32, POINTER(c_size_t)],
+    "rrxCmdIntersectBatch": [c_void_p, POINTER(RRxIntersectRequest), c_uint32, c_void_p, c_void_p],
+    "rrxCreateServer": [c_void_p, POINTER(RRxServerOptions), POINTER(c_void_p)],
+    "rrxDestroyServer": [c_void_p],
+    "rrxServerNow": [c_void_p],
+    "rrxServerSubmit": [c_void_p, POINTER(RRxIntersectRequest), POINTER(c_uint64), c_uint32, POINTER(c_uint64)],
+    "rrxServerDrain": [c_void_p],
+    "rrxServerGetTimings": [c_void_p, c_uint64, c_uint32, POINTER(RRxServedTiming)],
+    "rrxServerGetPeakAdmitted": [c_void_p, POINTER(c_uint64)],
 }
-_RESTYPES = {"rrGetHipAddress": c_void_p, "rrGetHipStream": c_void_p}
+_RESTYPES = {"rrGetHipAddress": c_void_p, "rrGetHipStream": c_void_p, "rrxServerNow": c_uint64}
 
 _lib = None
 

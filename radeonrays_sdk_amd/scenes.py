@@ -168,3 +168,83 @@ def random_transforms(rng, n, scale=(0.5, 2.0), trans=50.0):
     s = rng.uniform(scale[0], scale[1], size=(n, 1, 1))
     t = rng.uniform(-trans, trans, size=(n, 3, 1))
     return np.concatenate([R * s, t], axis=2).astype(np.float32)
+
+
+# --- serving stream ------------------------------------------------------------
+@dataclass
+class ServingScene:
+    kind: str  # "soup" (triangle soup in [0,1]^3) or "terrain" (fBm heightfield)
+    vertices: np.ndarray
+    indices: np.ndarray
+
+
+@dataclass
+class ServingRequest:
+    scene: int
+    pool: int    # ray pool index (SERVING_POOLS)
+    offset: int  # first ray of the pool
+    count: int
+    query: str   # "closest" or "any"
+
+
+# Ray pools the requests draw contiguous slices from: primary-camera and
+# uniform-sphere rays for each scene kind.
+SERVING_POOLS = ("soup_camera", "soup_sphere", "terrain_camera", "terrain_sphere")
+
+
+def serving_scenes(n_scenes: int = 32, lo: int = 10_000, hi: int = 2_000_000) -> list:
+    """Resident scenes of the serving stream: triangle counts log-uniform in
+    [lo, hi]; even-numbered scenes are soups (as the 65,536-triangle soup),
+    odd-numbered ones fBm heightfields with about that many triangles."""
+    rng = rng_for(4)
+    sizes = log_uniform_ints(rng, n_scenes, lo, hi)
+    out = []
+    for k, n in enumerate(sizes):
+        if k % 2 == 0:
+            v, i = triangle_soup(int(n), rng_for(4, 1 + k))
+            out.append(ServingScene("soup", v, i))
+        else:
+            res = max(2, int(round(math.sqrt(n / 2.0))) + 1)
+            v, i = heightfield_mesh(res, seed=SEED_BASE + 4 + 1000 * k)
+            out.append(ServingScene("terrain", v, i))
+    return out
+
+
+def serving_ray_pools(pool_rays: int = 1 << 20) -> list:
+    """The four ray pools, pool_rays rays each (a square camera image when
+    pool_rays is a square)."""
+    side = int(math.isqrt(pool_rays))
+    rng = rng_for(4, 100)
+    soup_cam = camera_rays(side, pool_rays // side, (0.5, 0.5, -1.5))
+    o = rng.uniform(0.0, 1.0, size=(pool_rays, 3)).astype(np.float32)
+    soup_sph = make_rays(o, unit_sphere(pool_rays, rng))
+    terr_cam = camera_rays(side, pool_rays // side, (0.5, 0.35, -0.2))
+    o = rng.uniform([0.0, 0.25, 0.0], [1.0, 0.5, 1.0], size=(pool_rays, 3)).astype(np.float32)
+    terr_sph = make_rays(o, unit_sphere(pool_rays, rng))
+    return [soup_cam, soup_sph, terr_cam, terr_sph]
+
+
+def serving_requests(n_requests: int, scenes: list, pool_rays: int, lo: int = 1_000, hi: int = 1_000_000,
+                     p_closest: float = 0.7, stream: int = 0) -> list:
+    """Requests: scene uniform, ray batch log-uniform in [lo, hi] rays drawn
+    as a contiguous slice of a camera or sphere pool of the scene's kind
+    (even odds), closest-hit with probability p_closest, else any-hit."""
+    rng = rng_for(4, 200 + stream)
+    counts = np.minimum(log_uniform_ints(rng, n_requests, lo, hi), pool_rays)
+    out = []
+    for k in range(n_requests):
+        s = int(rng.integers(0, len(scenes)))
+        pool = (0 if scenes[s].kind == "soup" else 2) + int(rng.integers(0, 2))
+        c = int(counts[k])
+        off = int(rng.integers(0, pool_rays - c + 1))
+        q = "closest" if rng.uniform() < p_closest else "any"
+        out.append(ServingRequest(s, pool, off, c, q))
+    return out
+
+
+def poisson_arrivals(n: int, qps: float, stream: int = 0) -> np.ndarray:
+    """Arrival offsets in ns of a Poisson process of rate qps (first at 0)."""
+    rng = rng_for(4, 300 + stream)
+    gaps = rng.exponential(1e9 / qps, size=n)
+    gaps[0] = 0.0
+    return np.cumsum(gaps).astype(np.int64)

@@ -51,6 +51,10 @@ def test_struct_layouts_match_the_abi():
     assert ctypes.sizeof(rr.RRInstance) == 56
     assert ctypes.sizeof(rr.RRSceneBuildInput) == 16
     assert ctypes.sizeof(rr.RRMemoryRequirements) == 24
+    # include/radeonrays_ext.h
+    assert ctypes.sizeof(rr.RRxIntersectRequest) == 40
+    assert ctypes.sizeof(rr.RRxServerOptions) == 24
+    assert ctypes.sizeof(rr.RRxServedTiming) == 32
 
 
 def _mesh_input(tris, ptype=rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, count=1):
@@ -85,6 +89,26 @@ def test_null_arguments_are_invalid_parameter():
     assert lib.rrReleaseExternalCommandStream(FAKE_CTX, None) == E
     assert lib.rrGetDevicePtrFromHipPtr(FAKE_CTX, None, 0, None) == E
     assert lib.rrSetLogFile(None) == E
+    # Extensions.
+    assert lib.rrxCmdIntersectBatch(FAKE_CTX, None, 1, c_void_p(1), c_void_p(1)) == E
+    assert lib.rrxCmdIntersectBatch(FAKE_CTX, None, 0, None, c_void_p(1)) == E
+    bad = (rr.RRxIntersectRequest * 1)(rr.RRxIntersectRequest(None, None, None, 5, 0, 0, 0))
+    assert lib.rrxCmdIntersectBatch(FAKE_CTX, bad, 1, c_void_p(1), c_void_p(1)) == E
+    bad[0] = rr.RRxIntersectRequest(c_void_p(1), c_void_p(1), c_void_p(1), 5, 7, 0, 0)  # unknown query
+    assert lib.rrxCmdIntersectBatch(FAKE_CTX, bad, 1, c_void_p(1), c_void_p(1)) == E
+    assert lib.rrxGetIntersectBatchMemoryRequirements(FAKE_CTX, 1, 1, 1, None) == E
+    opts = rr.RRxServerOptions(rr.RRX_SERVE_BATCHED, 0, 16, 1 << 20, 1 << 30)  # no streams
+    h = c_void_p()
+    assert lib.rrxCreateServer(FAKE_CTX, byref(opts), byref(h)) == E
+    opts = rr.RRxServerOptions(5, 1, 16, 1 << 20, 1 << 30)  # unknown mode
+    assert lib.rrxCreateServer(FAKE_CTX, byref(opts), byref(h)) == E
+    assert lib.rrxCreateServer(None, byref(opts), byref(h)) == E
+    assert lib.rrxDestroyServer(None) == E
+    assert lib.rrxServerDrain(None) == E
+    assert lib.rrxServerSubmit(None, None, None, 0, None) == E
+    assert lib.rrxServerGetTimings(None, 0, 0, None) == E
+    assert lib.rrxServerGetPeakAdmitted(None, None) == E
+    assert lib.rrxServerNow(None) == 0
 
 
 def test_aabb_list_is_not_implemented():
