@@ -35,14 +35,15 @@ struct SortScratch {
     uint32_t* hist;     // [kSortMaxPasses][256] global digit counts
     uint32_t* tickets;  // [kSortMaxPasses] tile tickets
     uint32_t* status;   // [kSortMaxPasses][tiles][256] look-back words
-    size_t status_words;
+    size_t status_words;  // words of pass 0's look-back region, which must be zero before a sort
 };
 SortScratch CarveSortScratch(void* base, uint32_t n);
 
 // Sort keys/vals in place (ping-ponging through keys_tmp/vals_tmp) by the
 // low key_bits bits; stable.  When hist_ready is true the caller has already
 // cleared hist/tickets, accumulated the digit histograms into hist and
-// cleared the status words (the build fuses that into its Morton kernel).
+// cleared the first status_words look-back words (the build fuses that into
+// its Morton kernel).
 // Returns true if the result ended in keys/vals, false if in the tmp arrays.
 bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
                     const SortScratch& sc, bool hist_ready, hipStream_t stream);

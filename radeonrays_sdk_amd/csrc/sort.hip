@@ -17,6 +17,8 @@
 #include "rr_device.h"
 #include "sort.h"
 
+#include <cstdlib>
+
 namespace rr {
 
 namespace {
@@ -26,6 +28,7 @@ constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagInclusive = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kLookback = 8;
 
 __device__ __forceinline__ uint32_t AtomicLoadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -58,40 +61,59 @@ __global__ __launch_bounds__(kBlock) void SortClearKernel(uint32_t* hist, uint32
     if (threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0;
 }
 
+// Digit counts of every pass in one read of the keys: each thread takes 16
+// consecutive keys (four 16-byte loads) per step of a grid-stride loop.
+constexpr int kHistKeysPerThread = 16;
+
 __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __restrict__ keys, uint32_t n, int passes,
                                                               uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
                                                               size_t status_words) {
     __shared__ uint32_t s_h[kSortMaxPasses][kSortRadix];
     for (uint32_t i = threadIdx.x; i < kSortMaxPasses * kSortRadix; i += kBlock) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t stride = gridDim.x * kBlock;
-    // Every lane of a wave runs the same number of iterations so that the
-    // ballots in AccumulateDigits see the whole wave.
-    const uint32_t wave_base = blockIdx.x * kBlock + (threadIdx.x & ~63u);
-    for (uint32_t b = wave_base; b < n; b += stride) {
+    constexpr uint32_t kStep = kBlock * kHistKeysPerThread;
+    // Steps whose keys are all in range (none when keys is not 16-byte aligned).
+    const uint32_t full = (reinterpret_cast<uintptr_t>(keys) & 15u) ? 0u : n / kStep;
+    for (uint32_t step = blockIdx.x; step < full; step += gridDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(keys + (size_t)step * kStep);
+        uint4 q[kHistKeysPerThread / 4];
+#pragma unroll
+        for (int j = 0; j < kHistKeysPerThread / 4; ++j) q[j] = src[j * kBlock + threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < kHistKeysPerThread / 4; ++j) {
+            AccumulateDigits(q[j].x, true, s_h, passes);
+            AccumulateDigits(q[j].y, true, s_h, passes);
+            AccumulateDigits(q[j].z, true, s_h, passes);
+            AccumulateDigits(q[j].w, true, s_h, passes);
+        }
+    }
+    // The rest, one key per lane (whole waves iterate together).
+    for (uint32_t b = full * kStep + blockIdx.x * kBlock + (threadIdx.x & ~63u); b < n; b += gridDim.x * kBlock) {
         const uint32_t i = b + __lane_id();
         const bool valid = i < n;
-        const uint32_t k = valid ? keys[i] : 0u;
-        AccumulateDigits(k, valid, s_h, passes);
+        AccumulateDigits(valid ? keys[i] : 0u, valid, s_h, passes);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kSortRadix; i += kBlock) {
         const uint32_t c = (&s_h[0][0])[i];
         if (c) atomicAdd(&hist[i], c);
     }
-    for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += stride) status[w] = 0;
+    for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += (size_t)gridDim.x * kBlock)
+        status[w] = 0;
 }
 
+// Four waves per SIMD: the LDS of a 4096-key tile allows four blocks per CU.
 template <int ITEMS>
-__global__ __launch_bounds__(kBlock) void SortPassKernel(const uint32_t* __restrict__ keys_in,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(const uint32_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
                                                          uint32_t* __restrict__ keys_out,
                                                          uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
                                                          const uint32_t* __restrict__ hist, uint32_t* tickets,
-                                                         uint32_t* status) {
+                                                         uint32_t* status, uint32_t* next_status) {
     constexpr int kWaves = kBlock / 64;
     constexpr int kTile = kBlock * ITEMS;
     __shared__ uint32_t s_buf[kTile];
+    __shared__ uint32_t s_vbuf[kTile];
     __shared__ uint32_t s_wave_hist[kWaves][kSortRadix];
     __shared__ uint32_t s_digit_start[kSortRadix];
     __shared__ uint32_t s_global_off[kSortRadix];
@@ -105,32 +127,52 @@ __global__ __launch_bounds__(kBlock) void SortPassKernel(const uint32_t* __restr
     const uint32_t hist_d = hist[tid];
     const uint32_t hist_scan = BlockExclusiveScan256(hist_d, s_wave_tot);  // contains __syncthreads
     const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * kTile + (size_t)wave * (64 * ITEMS);
+    const uint32_t base = tile * kTile + wave * (64 * ITEMS) + lane;  // n <= 2^30: 32-bit indices
 
     uint32_t k[ITEMS], v[ITEMS], rank[ITEMS];
+    const bool full = (tile + 1) * kTile <= n;  // no bounds checks needed
+    if (full) {
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const size_t idx = base + (size_t)j * 64 + lane;
-        const bool valid = idx < n;
-        k[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
-        v[j] = valid ? vals_in[idx] : 0u;
+        for (int j = 0; j < ITEMS; ++j) k[j] = keys_in[base + (uint32_t)j * 64];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t idx = base + (uint32_t)j * 64;
+            k[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
+        }
     }
-    const uint64_t lt = LaneMaskLt();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
-        uint64_t match = ~0ull;
+        const uint32_t pre = s_wave_hist[wave][d];
+        // Lanes holding the same digit: AND over the digit's bits of the
+        // bit's ballot or its complement (0 - bit selects which), one
+        // three-input bit operation per half.
+        uint32_t m_lo = ~0u, m_hi = ~0u;
 #pragma unroll
         for (int b = 0; b < kSortRadixBits; ++b) {
-            const uint64_t bal = __ballot((d >> b) & 1u);
-            match &= ((d >> b) & 1u) ? bal : ~bal;
+            const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)k[j], shift + b, 1);  // 0 or ~0
+            const uint64_t bal = __ballot(neg != 0u);
+            m_lo &= ~((uint32_t)bal ^ neg);
+            m_hi &= ~((uint32_t)(bal >> 32) ^ neg);
         }
-        const uint32_t before = (uint32_t)__popcll(match & lt);
-        const uint32_t pre = s_wave_hist[wave][d];
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
         rank[j] = pre + before;
         __builtin_amdgcn_wave_barrier();
-        if (before == 0) s_wave_hist[wave][d] = pre + (uint32_t)__popcll(match);
+        if (before == 0) s_wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
         __builtin_amdgcn_wave_barrier();
+    }
+    // Values are loaded now, so that the loads are in flight during the
+    // look-back rather than holding registers through the ranking.
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) v[j] = vals_in[base + (uint32_t)j * 64];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t idx = base + (uint32_t)j * 64;
+            v[j] = idx < n ? vals_in[idx] : 0u;
+        }
     }
     __syncthreads();
     // Per digit: exclusive offsets of each wave and the tile total.
@@ -152,57 +194,78 @@ __global__ __launch_bounds__(kBlock) void SortPassKernel(const uint32_t* __restr
             AtomicStoreAgent(&status[d], kFlagInclusive | count);
         } else {
             AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagAggregate | count);
+            // Look back kLookback predecessors at a time: their words are
+            // loaded together, so a chain of aggregates costs one memory
+            // round trip per kLookback tiles rather than per tile.
             int64_t p = (int64_t)tile - 1;
             uint32_t spins = 0;
-            while (p >= 0) {
-                const uint32_t s = AtomicLoadAgent(&status[(size_t)p * kSortRadix + d]);
-                if ((s & ~kCountMask) == 0) {
+            bool done = false;
+            while (!done && p >= 0) {
+                uint32_t w[kLookback];
+#pragma unroll
+                for (int k = 0; k < kLookback; ++k)
+                    w[k] = (p - k >= 0) ? AtomicLoadAgent(&status[(size_t)(p - k) * kSortRadix + d]) : kFlagInclusive;
+                int used = 0;
+                bool blocked = false;
+#pragma unroll
+                for (int k = 0; k < kLookback; ++k) {
+                    if (done || blocked) continue;
+                    const uint32_t f = w[k] & ~kCountMask;
+                    if (f == 0) {
+                        blocked = true;  // not published yet: poll again from here
+                        continue;
+                    }
+                    excl += w[k] & kCountMask;
+                    used = k + 1;
+                    done = f == kFlagInclusive;
+                }
+                p -= used;
+                if (blocked) {
                     if (++spins > kSpinLimit) break;
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                excl += s & kCountMask;
-                if ((s & ~kCountMask) == kFlagInclusive) break;
-                --p;
             }
             AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagInclusive | (excl + count));
         }
         s_global_off[d] = hist_scan + excl;
+        // The next pass finds this tile's look-back words cleared.
+        if (next_status) next_status[(size_t)tile * kSortRadix + d] = 0;
     }
     const uint32_t dstart = BlockExclusiveScan256(count, s_wave_tot);  // contains __syncthreads
     s_digit_start[tid] = dstart;
     __syncthreads();
 
-    // Keys: rank into LDS, then write each digit run contiguously.
-    uint32_t pos[ITEMS];
+    // Keys and values: place by rank in LDS, then write each digit's run
+    // contiguously.
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
-        pos[j] = s_digit_start[d] + s_wave_hist[wave][d] + rank[j];
-        s_buf[pos[j]] = k[j];
+        const uint32_t pos = s_digit_start[d] + s_wave_hist[wave][d] + rank[j];
+        s_buf[pos] = k[j];
+        s_vbuf[pos] = v[j];
     }
     __syncthreads();
-    uint32_t gpos[ITEMS];  // n <= 2^30, so positions fit in 32 bits
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t i = (uint32_t)j * kBlock + tid;
         const uint32_t key = s_buf[i];
         const uint32_t d = (key >> shift) & (kSortRadix - 1);
-        gpos[j] = s_global_off[d] + (i - s_digit_start[d]);
-        if (gpos[j] < n) keys_out[gpos[j]] = key;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) s_buf[pos[j]] = v[j];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t i = (uint32_t)j * kBlock + tid;
-        if (gpos[j] < n) vals_out[gpos[j]] = s_buf[i];
+        const uint32_t gpos = s_global_off[d] + (i - s_digit_start[d]);  // n <= 2^30: fits in 32 bits
+        if (gpos < n) {
+            keys_out[gpos] = key;
+            vals_out[gpos] = s_vbuf[i];
+        }
     }
 }
 
 SortShape ChooseSortShape(uint32_t n) {
+    // RR_SORT_ITEMS=4|8|16 fixes the keys per thread (tuning runs).
+    static const int forced = [] {
+        const char* e = std::getenv("RR_SORT_ITEMS");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16) ? v : 0;
+    }();
+    if (forced) return {kBlock, forced};
     if (n >= (1u << 22)) return {kBlock, 16};
     if (n >= (1u << 19)) return {kBlock, 8};
     return {kBlock, 4};
@@ -224,7 +287,7 @@ SortScratch CarveSortScratch(void* base, uint32_t n) {
     s.hist = p;
     s.tickets = p + kSortMaxPasses * kSortRadix;
     s.status = s.tickets + 64;
-    s.status_words = (size_t)kSortMaxPasses * SortTiles(n) * kSortRadix;
+    s.status_words = SortTiles(n) * kSortRadix;  // pass 0's; each pass clears the next pass's
     return s;
 }
 
@@ -238,7 +301,8 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
     const int passes = SortPasses(key_bits);
     if (!hist_ready) {
         LaunchSortClear(sc, stream);
-        const uint32_t blocks = (uint32_t)std::min<size_t>(2048, (n + kBlock * 8 - 1) / (kBlock * 8));
+        const uint32_t blocks =
+            (uint32_t)std::max<size_t>(1, std::min<size_t>(1024, (size_t)n / (kBlock * kHistKeysPerThread)));
         SortHistogramKernel<<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, sc.status_words);
     }
     const SortShape shape = ChooseSortShape(n);
@@ -246,17 +310,18 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
     for (int p = 0; p < passes; ++p) {
         uint32_t* st = sc.status + (size_t)p * tiles * kSortRadix;
+        uint32_t* nst = p + 1 < passes ? st + tiles * kSortRadix : nullptr;
         const uint32_t* h = sc.hist + p * kSortRadix;
         const uint32_t shift = (uint32_t)(p * kSortRadixBits);
         switch (shape.items) {
             case 16:
-                SortPassKernel<16><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st);
+                SortPassKernel<16><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
             case 8:
-                SortPassKernel<8><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st);
+                SortPassKernel<8><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
             default:
-                SortPassKernel<4><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st);
+                SortPassKernel<4><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
         }
         std::swap(ki, ko);
