@@ -52,12 +52,19 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
 void LaunchSortClear(const SortScratch& sc, hipStream_t stream);
 
 // Accumulate the 8-bit digits of `key` for `passes` passes into LDS
-// histograms.  A wave whose valid keys all share a digit adds once.
+// histograms.  The top two digits of spatially ordered keys (Morton codes of
+// a mesh in triangle order) are mostly equal across a wave, and same-address
+// LDS atomics serialise, so for those a wave whose valid keys all share the
+// digit adds once; lower digits are near-random and go straight to atomics.
 __device__ __forceinline__ void AccumulateDigits(uint32_t key, bool valid, uint32_t (*s_h)[kSortRadix], int passes) {
     const uint64_t act = __ballot(valid);
     const uint32_t lane = __lane_id();
     for (int p = 0; p < passes; ++p) {
         const uint32_t d = (key >> (p * kSortRadixBits)) & (kSortRadix - 1);
+        if (p + 2 < passes) {
+            if (valid) atomicAdd(&s_h[p][d], 1u);
+            continue;
+        }
         const uint32_t d0 = __shfl(d, 0, 64);
         const bool same = __all(!valid || d == d0);
         if (same) {

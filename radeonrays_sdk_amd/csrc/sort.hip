@@ -28,7 +28,7 @@ constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagInclusive = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr int kLookback = 8;
+constexpr int kLookback = 4;
 
 __device__ __forceinline__ uint32_t AtomicLoadAgent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -37,8 +37,10 @@ __device__ __forceinline__ void AtomicStoreAgent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ uint32_t BlockExclusiveScan256(uint32_t v, uint32_t* s_wave_tot) {
+// Exclusive scan of one value per thread over a BLOCK-thread block.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* s_wave_tot) {
+    constexpr int kWaves = BLOCK / 64;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -49,7 +51,8 @@ __device__ __forceinline__ uint32_t BlockExclusiveScan256(uint32_t v, uint32_t* 
     if (lane == 63) s_wave_tot[wave] = x;
     __syncthreads();
     uint32_t add = 0;
-    for (uint32_t w = 0; w < wave; ++w) add += s_wave_tot[w];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) add += (w < (int)wave) ? s_wave_tot[w] : 0u;
     __syncthreads();
     return x - v + add;
 }
@@ -102,16 +105,18 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
         status[w] = 0;
 }
 
-// Four waves per SIMD: the LDS of a 4096-key tile allows four blocks per CU.
-template <int ITEMS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(const uint32_t* __restrict__ keys_in,
+// Four waves per SIMD: with 8192-key tiles (512 threads x 16 keys) the LDS
+// holds two tiles per CU, and at most 128 VGPRs let both be resident.
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(const uint32_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
                                                          uint32_t* __restrict__ keys_out,
                                                          uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
                                                          const uint32_t* __restrict__ hist, uint32_t* tickets,
                                                          uint32_t* status, uint32_t* next_status) {
-    constexpr int kWaves = kBlock / 64;
-    constexpr int kTile = kBlock * ITEMS;
+    static_assert(BLOCK >= kSortRadix && BLOCK % 64 == 0, "one thread per digit");
+    constexpr int kWaves = BLOCK / 64;
+    constexpr int kTile = BLOCK * ITEMS;
     __shared__ uint32_t s_buf[kTile];
     __shared__ uint32_t s_vbuf[kTile];
     __shared__ uint32_t s_wave_hist[kWaves][kSortRadix];
@@ -121,11 +126,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const bool digit_thread = tid < (uint32_t)kSortRadix;  // thread d handles digit d
     if (tid == 0) s_tile = atomicAdd(tickets, 1u);
-    for (uint32_t i = tid; i < kWaves * kSortRadix; i += kBlock) (&s_wave_hist[0][0])[i] = 0;
+    for (uint32_t i = tid; i < kWaves * kSortRadix; i += BLOCK) (&s_wave_hist[0][0])[i] = 0;
     // Global exclusive digit offsets from the upfront histogram.
-    const uint32_t hist_d = hist[tid];
-    const uint32_t hist_scan = BlockExclusiveScan256(hist_d, s_wave_tot);  // contains __syncthreads
+    const uint32_t hist_d = digit_thread ? hist[tid] : 0u;
+    const uint32_t hist_scan = BlockExclusiveScan<BLOCK>(hist_d, s_wave_tot);  // contains __syncthreads
     const uint32_t tile = s_tile;
     const uint32_t base = tile * kTile + wave * (64 * ITEMS) + lane;  // n <= 2^30: 32-bit indices
 
@@ -177,7 +183,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __syncthreads();
     // Per digit: exclusive offsets of each wave and the tile total.
     uint32_t count = 0;
-    {
+    if (digit_thread) {
         const uint32_t d = tid;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
     }
     // Decoupled look-back for digit `tid`.
-    {
+    if (digit_thread) {
         const uint32_t d = tid;
         uint32_t excl = 0;
         if (tile == 0) {
@@ -231,8 +237,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // The next pass finds this tile's look-back words cleared.
         if (next_status) next_status[(size_t)tile * kSortRadix + d] = 0;
     }
-    const uint32_t dstart = BlockExclusiveScan256(count, s_wave_tot);  // contains __syncthreads
-    s_digit_start[tid] = dstart;
+    const uint32_t dstart = BlockExclusiveScan<BLOCK>(count, s_wave_tot);  // contains __syncthreads
+    if (digit_thread) s_digit_start[tid] = dstart;
     __syncthreads();
 
     // Keys and values: place by rank in LDS, then write each digit's run
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t i = (uint32_t)j * kBlock + tid;
+        const uint32_t i = (uint32_t)j * BLOCK + tid;
         const uint32_t key = s_buf[i];
         const uint32_t d = (key >> shift) & (kSortRadix - 1);
         const uint32_t gpos = s_global_off[d] + (i - s_digit_start[d]);  // n <= 2^30: fits in 32 bits
@@ -265,8 +271,10 @@ SortShape ChooseSortShape(uint32_t n) {
         const int v = e ? std::atoi(e) : 0;
         return (v == 4 || v == 8 || v == 16) ? v : 0;
     }();
-    if (forced) return {kBlock, forced};
-    if (n >= (1u << 22)) return {kBlock, 16};
+    if (forced) return {forced == 16 ? 512 : kBlock, forced};
+    // Large inputs: 8192-key tiles (fewer tiles, so fewer look-back rounds,
+    // and 32-key digit runs per tile on average, i.e. whole cache lines).
+    if (n >= (1u << 22)) return {512, 16};
     if (n >= (1u << 19)) return {kBlock, 8};
     return {kBlock, 4};
 }
@@ -315,13 +323,13 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         const uint32_t shift = (uint32_t)(p * kSortRadixBits);
         switch (shape.items) {
             case 16:
-                SortPassKernel<16><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
+                SortPassKernel<512, 16><<<(uint32_t)tiles, 512, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
             case 8:
-                SortPassKernel<8><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
+                SortPassKernel<kBlock, 8><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
             default:
-                SortPassKernel<4><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
+                SortPassKernel<kBlock, 4><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
         }
         std::swap(ki, ko);
