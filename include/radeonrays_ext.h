@@ -46,6 +46,22 @@ RR_API RRError rrxCmdIntersectWithStats(RRContext        context,
                                         RRDevicePtr      stats,
                                         RRCommandStream  command_stream);
 
+/* Write a geometry (BLAS) in the dump format of the reference's bvh_analyzer:
+ * the VkBvhNode array to bvh_path, width x height RRRay records from `rays`
+ * (may be NULL: no rays file is written) to rays_path, and the analyzer's
+ * 7-line config (bvh_analyzer/config.h:46-68) to config_path.  Waits for the
+ * device first.  The reference writes the same files from its tests
+ * (test/test_vk/basic_test.h:519-555); scenes (TLAS) are not readable by the
+ * analyzer and give RR_ERROR_NOT_IMPLEMENTED. */
+RR_API RRError rrxWriteAnalyzerDump(RRContext   context,
+                                    RRDevicePtr geometry,
+                                    RRDevicePtr rays,
+                                    uint32_t    width,
+                                    uint32_t    height,
+                                    const char* bvh_path,
+                                    const char* rays_path,
+                                    const char* config_path);
+
 /* One query of a batch: the arguments of one rrCmdIntersect call
  * (radeonrays.h:391-400) without the indirect ray count. */
 typedef struct RRxIntersectRequest

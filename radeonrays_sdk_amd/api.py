@@ -195,6 +195,15 @@ class Context:
                      "rrxCmdIntersectWithStats")
         return hits, stats
 
+    def write_analyzer_dump(self, accel: Accel, bvh_path, config_path, rays=None, width: int = 0, height: int = 0,
+                            rays_path=None):
+        """rrxWriteAnalyzerDump: the BLAS as a bvh_analyzer dump + 7-line config (rays optional)."""
+        rays_path = rays_path or (str(bvh_path) + ".rays")
+        with self.ptrs(accel.address, rays.data_ptr() if rays is not None else None) as (a, r):
+            rr.check(self.lib.rrxWriteAnalyzerDump(self.ctx, a, r, width, height, str(bvh_path).encode(),
+                                                    str(rays_path).encode(), str(config_path).encode()),
+                     "rrxWriteAnalyzerDump")
+
     def batch_scratch(self, request_count: int, total_rays: int, max_rays: int):
         size = c_size_t()
         rr.check(self.lib.rrxGetIntersectBatchMemoryRequirements(self.ctx, request_count, total_rays, max_rays,

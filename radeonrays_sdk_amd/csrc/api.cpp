@@ -893,6 +893,46 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
     });
 }
 
+RRError rrxWriteAnalyzerDump(RRContext context, RRDevicePtr geometry, RRDevicePtr rays, uint32_t width, uint32_t height,
+                             const char* bvh_path, const char* rays_path, const char* config_path) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrxWriteAnalyzerDump");
+    RR_REQUIRE(context && geometry && bvh_path && rays_path && config_path, "rrxWriteAnalyzerDump");
+    RR_REQUIRE(!rays || (uint64_t)width * height > 0, "rrxWriteAnalyzerDump");
+    return Guarded("rrxWriteAnalyzerDump", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        // Everything enqueued so far (the build) must have finished.
+        Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        const uint8_t* accel = static_cast<const uint8_t*>(Addr(geometry));
+        rr::AccelHeader h;
+        Check(hipMemcpy(&h, accel, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (h.magic != rr::kAccelMagic) return RR_ERROR_INVALID_PARAMETER;
+        // The analyzer reads single-level BVHs only (bvh_analyzer/main.cpp:77).
+        if (h.kind != rr::kAccelBlas) return RR_ERROR_NOT_IMPLEMENTED;
+        std::vector<uint8_t> nodes(size_t(h.node_count) * sizeof(rr::BvhNode));
+        if (!nodes.empty())
+            Check(hipMemcpy(nodes.data(), accel + h.nodes_offset, nodes.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+        auto write = [](const char* path, const void* data, size_t bytes) {
+            FILE* f = std::fopen(path, "wb");
+            if (!f) return false;
+            const bool ok = std::fwrite(data, 1, bytes, f) == bytes;
+            return std::fclose(f) == 0 && ok;
+        };
+        if (!write(bvh_path, nodes.data(), nodes.size())) return RR_ERROR_INTERNAL;
+        if (rays) {
+            std::vector<uint8_t> r(size_t(width) * height * 32);
+            Check(hipMemcpy(r.data(), Addr(rays), r.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+            if (!write(rays_path, r.data(), r.size())) return RR_ERROR_INTERNAL;
+        }
+        // bvh_analyzer/config.h:46-68: bvh path, type, internal node count,
+        // triangle count, rays path, ray width, ray height.
+        FILE* f = std::fopen(config_path, "w");
+        if (!f) return RR_ERROR_INTERNAL;
+        std::fprintf(f, "%s\nvkbvh2\n%u\n%u\n%s\n%u\n%u\n", bvh_path, h.prim_count ? h.prim_count - 1 : 0, h.prim_count,
+                     rays_path, width, height);
+        return std::fclose(f) == 0 ? RR_SUCCESS : RR_ERROR_INTERNAL;
+    });
+}
+
 }  // extern "C"
 
 int rr::ContextDevice(RRContext context) { return AsCtx(context)->device; }

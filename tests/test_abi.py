@@ -109,6 +109,8 @@ def test_null_arguments_are_invalid_parameter():
     assert lib.rrxServerGetTimings(None, 0, 0, None) == E
     assert lib.rrxServerGetPeakAdmitted(None, None) == E
     assert lib.rrxServerNow(None) == 0
+    assert lib.rrxWriteAnalyzerDump(FAKE_CTX, None, None, 0, 0, b"a", b"b", b"c") == E
+    assert lib.rrxWriteAnalyzerDump(FAKE_CTX, c_void_p(1), None, 0, 0, None, b"b", b"c") == E
 
 
 def test_aabb_list_is_not_implemented():
