@@ -105,21 +105,25 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
         status[w] = 0;
 }
 
-// Four waves per SIMD: with 8192-key tiles (512 threads x 16 keys) the LDS
-// holds two tiles per CU, and at most 128 VGPRs let both be resident.
+// One pass over one tile, in this order: load the keys; count the tile's
+// digits (LDS atomics) and publish them at once, then resolve the tile's
+// global offsets by look-back; only then rank the keys (ballots) and
+// scatter.  Publishing before ranking lets successors' look-backs finish
+// while this tile is still ranking.  Four waves per SIMD: with 16384-key
+// tiles (1024 threads x 16 keys) one tile fills a CU's LDS and at most 128
+// VGPRs keep all 16 waves resident.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(const uint32_t* __restrict__ keys_in,
-                                                         const uint32_t* __restrict__ vals_in,
-                                                         uint32_t* __restrict__ keys_out,
-                                                         uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
-                                                         const uint32_t* __restrict__ hist, uint32_t* tickets,
-                                                         uint32_t* status, uint32_t* next_status) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, const uint32_t* __restrict__ hist, uint32_t* tickets,
+    uint32_t* status, uint32_t* next_status) {
     static_assert(BLOCK >= kSortRadix && BLOCK % 64 == 0, "one thread per digit");
     constexpr int kWaves = BLOCK / 64;
     constexpr int kTile = BLOCK * ITEMS;
     __shared__ uint32_t s_buf[kTile];
     __shared__ uint32_t s_vbuf[kTile];
     __shared__ uint32_t s_wave_hist[kWaves][kSortRadix];
+    __shared__ uint32_t s_count[kSortRadix];
     __shared__ uint32_t s_digit_start[kSortRadix];
     __shared__ uint32_t s_global_off[kSortRadix];
     __shared__ uint32_t s_wave_tot[kWaves];
@@ -129,6 +133,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const bool digit_thread = tid < (uint32_t)kSortRadix;  // thread d handles digit d
     if (tid == 0) s_tile = atomicAdd(tickets, 1u);
     for (uint32_t i = tid; i < kWaves * kSortRadix; i += BLOCK) (&s_wave_hist[0][0])[i] = 0;
+    if (digit_thread) s_count[tid] = 0;
     // Global exclusive digit offsets from the upfront histogram.
     const uint32_t hist_d = digit_thread ? hist[tid] : 0u;
     const uint32_t hist_scan = BlockExclusiveScan<BLOCK>(hist_d, s_wave_tot);  // contains __syncthreads
@@ -147,13 +152,65 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             k[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
         }
     }
+    // The tile's digit counts.  Padding keys (all ones) are counted in digit
+    // 255 and are ranked last, so they never displace a real key.
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) atomicAdd(&s_count[(k[j] >> shift) & (kSortRadix - 1)], 1u);
+    __syncthreads();
+
+    // Decoupled look-back for digit `tid`, kLookback predecessors at a time:
+    // their words are loaded together, so a chain of aggregates costs one
+    // memory round trip per kLookback tiles rather than per tile.
+    if (digit_thread) {
+        const uint32_t d = tid;
+        const uint32_t count = s_count[d];
+        uint32_t excl = 0;
+        if (tile == 0) {
+            AtomicStoreAgent(&status[d], kFlagInclusive | count);
+        } else {
+            AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagAggregate | count);
+            int64_t p = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            bool done = false;
+            while (!done && p >= 0) {
+                uint32_t w[kLookback];
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q)
+                    w[q] = (p - q >= 0) ? AtomicLoadAgent(&status[(size_t)(p - q) * kSortRadix + d]) : kFlagInclusive;
+                int used = 0;
+                bool blocked = false;
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q) {
+                    if (done || blocked) continue;
+                    const uint32_t f = w[q] & ~kCountMask;
+                    if (f == 0) {
+                        blocked = true;  // not published yet: poll again from here
+                        continue;
+                    }
+                    excl += w[q] & kCountMask;
+                    used = q + 1;
+                    done = f == kFlagInclusive;
+                }
+                p -= used;
+                if (blocked) {
+                    if (++spins > kSpinLimit) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagInclusive | (excl + count));
+        }
+        s_global_off[d] = hist_scan + excl;
+        // The next pass finds this tile's look-back words cleared.
+        if (next_status) next_status[(size_t)tile * kSortRadix + d] = 0;
+    }
+
+    // Rank within each wave: lanes holding the same digit are the AND over
+    // the digit's bits of the bit's ballot or its complement (0 - bit
+    // selects which), one three-input bit operation per half.
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
         const uint32_t pre = s_wave_hist[wave][d];
-        // Lanes holding the same digit: AND over the digit's bits of the
-        // bit's ballot or its complement (0 - bit selects which), one
-        // three-input bit operation per half.
         uint32_t m_lo = ~0u, m_hi = ~0u;
 #pragma unroll
         for (int b = 0; b < kSortRadixBits; ++b) {
@@ -168,8 +225,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         if (before == 0) s_wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
         __builtin_amdgcn_wave_barrier();
     }
-    // Values are loaded now, so that the loads are in flight during the
-    // look-back rather than holding registers through the ranking.
+    // Values are loaded only now, so they do not hold registers through the ranking.
     if (full) {
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) v[j] = vals_in[base + (uint32_t)j * 64];
@@ -181,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         }
     }
     __syncthreads();
-    // Per digit: exclusive offsets of each wave and the tile total.
+    // Per digit: exclusive offsets of each wave within the tile.
     uint32_t count = 0;
     if (digit_thread) {
         const uint32_t d = tid;
@@ -191,51 +247,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             s_wave_hist[w][d] = count;
             count += c;
         }
-    }
-    // Decoupled look-back for digit `tid`.
-    if (digit_thread) {
-        const uint32_t d = tid;
-        uint32_t excl = 0;
-        if (tile == 0) {
-            AtomicStoreAgent(&status[d], kFlagInclusive | count);
-        } else {
-            AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagAggregate | count);
-            // Look back kLookback predecessors at a time: their words are
-            // loaded together, so a chain of aggregates costs one memory
-            // round trip per kLookback tiles rather than per tile.
-            int64_t p = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            bool done = false;
-            while (!done && p >= 0) {
-                uint32_t w[kLookback];
-#pragma unroll
-                for (int k = 0; k < kLookback; ++k)
-                    w[k] = (p - k >= 0) ? AtomicLoadAgent(&status[(size_t)(p - k) * kSortRadix + d]) : kFlagInclusive;
-                int used = 0;
-                bool blocked = false;
-#pragma unroll
-                for (int k = 0; k < kLookback; ++k) {
-                    if (done || blocked) continue;
-                    const uint32_t f = w[k] & ~kCountMask;
-                    if (f == 0) {
-                        blocked = true;  // not published yet: poll again from here
-                        continue;
-                    }
-                    excl += w[k] & kCountMask;
-                    used = k + 1;
-                    done = f == kFlagInclusive;
-                }
-                p -= used;
-                if (blocked) {
-                    if (++spins > kSpinLimit) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagInclusive | (excl + count));
-        }
-        s_global_off[d] = hist_scan + excl;
-        // The next pass finds this tile's look-back words cleared.
-        if (next_status) next_status[(size_t)tile * kSortRadix + d] = 0;
     }
     const uint32_t dstart = BlockExclusiveScan<BLOCK>(count, s_wave_tot);  // contains __syncthreads
     if (digit_thread) s_digit_start[tid] = dstart;
@@ -271,10 +282,10 @@ SortShape ChooseSortShape(uint32_t n) {
         const int v = e ? std::atoi(e) : 0;
         return (v == 4 || v == 8 || v == 16) ? v : 0;
     }();
-    if (forced) return {forced == 16 ? 512 : kBlock, forced};
-    // Large inputs: 8192-key tiles (fewer tiles, so fewer look-back rounds,
-    // and 32-key digit runs per tile on average, i.e. whole cache lines).
-    if (n >= (1u << 22)) return {512, 16};
+    if (forced) return {forced == 16 ? 1024 : kBlock, forced};
+    // Large inputs: 16384-key tiles (fewer tiles, so fewer look-back rounds,
+    // and 64-key digit runs per tile on average, i.e. whole cache lines).
+    if (n >= (1u << 22)) return {1024, 16};
     if (n >= (1u << 19)) return {kBlock, 8};
     return {kBlock, 4};
 }
@@ -323,7 +334,7 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         const uint32_t shift = (uint32_t)(p * kSortRadixBits);
         switch (shape.items) {
             case 16:
-                SortPassKernel<512, 16><<<(uint32_t)tiles, 512, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
+                SortPassKernel<1024, 16><<<(uint32_t)tiles, 1024, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
                 break;
             case 8:
                 SortPassKernel<kBlock, 8><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
