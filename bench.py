@@ -164,6 +164,46 @@ def serving_bench(ctx, dev, n_requests, log):
     }
 
 
+def instanced_bench(ctx, dev, log, reps=5):
+    """Two-level scene: 64 soups (log-uniform 10k-500k triangles) under 4,096
+    instances; 2048x2048 primary rays from (0, 0, -150), then 8,388,608
+    cosine-weighted secondary rays from the primary hits (no-op rays where
+    the primary missed); closest hit, FULL_HIT (instance + primitive id)."""
+    import torch
+
+    from radeonrays_sdk_amd import api, scenes
+
+    log("instanced: generating 64 meshes x 4096 instances")
+    w = scenes.instanced()
+    blases = [ctx.build_geometry(torch.from_numpy(v).to(dev), torch.from_numpy(i).to(dev), restructure=True)
+              for v, i in w.meshes]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tlas = ctx.build_scene([blases[m] for m in w.mesh_of_instance], w.transforms)
+    torch.cuda.synchronize()
+    tlas_ms = (time.perf_counter() - t0) * 1e3
+    primary = scenes.camera_rays(w.width, w.height, w.eye)
+    dprim = torch.from_numpy(primary).to(dev)
+    hits = ctx.intersect(tlas, dprim, "closest", "full_t")
+    torch.cuda.synchronize()
+    h = api.decode_hits(hits, "full_t")
+    sec = scenes.secondary_rays(w, primary, h["prim_id"], h["inst_id"], h["t"])
+    dsec = torch.from_numpy(sec).to(dev)
+    n = sec.shape[0]
+    hits2 = ctx.empty(n * HIT_BYTES)
+    scratch = ctx.trace_scratch(n)
+    prim_hits = ctx.empty(primary.shape[0] * HIT_BYTES)
+    ms_sec = timed(lambda: ctx.intersect(tlas, dsec, "closest", "full", hits=hits2, scratch=scratch), reps)
+    ms_prim = timed(lambda: ctx.intersect(tlas, dprim, "closest", "full", hits=prim_hits, scratch=scratch), reps)
+    return {"workload": f"64 soups ({sum(i.shape[0] for _, i in w.meshes)} tris) x 4096 instances, "
+                        f"{n} secondary rays (2 per pixel of a 2048x2048 primary pass), closest, FULL_HIT",
+            "secondary_mrays_s": round(n / ms_sec / 1e3, 2), "secondary_ms": round(ms_sec, 4),
+            "noop_rays": int((sec[:, 7] < 0).sum()),
+            "primary_mrays_s": round(primary.shape[0] / ms_prim / 1e3, 2), "primary_ms": round(ms_prim, 4),
+            "primary_hit_fraction": round(float((h["inst_id"] != 0xFFFFFFFF).mean()), 4),
+            "tlas_build_ms_wall": round(tlas_ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +212,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip build/refit/sort timings")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream comparison")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene traversal")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
@@ -290,6 +331,8 @@ def main():
         out["sort16m_gbs"] = round(sort_gbs, 1)
         out["sort16m_hbm_frac"] = round(sort_gbs / HBM_PEAK_GBS, 4)
 
+    if not args.no_instanced:
+        out["instanced"] = instanced_bench(ctx, dev, log)
     if not args.no_serving:
         out["serving"] = serving_bench(ctx, dev, args.serving_requests, log)
 

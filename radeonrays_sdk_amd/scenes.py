@@ -248,3 +248,87 @@ def poisson_arrivals(n: int, qps: float, stream: int = 0) -> np.ndarray:
     gaps = rng.exponential(1e9 / qps, size=n)
     gaps[0] = 0.0
     return np.cumsum(gaps).astype(np.int64)
+
+
+# --- instanced scene ---------------------------------------------------------------
+@dataclass
+class InstancedWorkload:
+    meshes: list            # [(vertices, indices)] per distinct mesh
+    mesh_of_instance: np.ndarray  # [I] int
+    transforms: np.ndarray  # [I, 3, 4] object -> world, row-major
+    eye: tuple
+    width: int
+    height: int
+
+
+def instanced(n_meshes: int = 64, n_inst: int = 4096, lo: int = 10_000, hi: int = 500_000, width: int = 2048,
+              height: int = 2048, trans: float = 50.0) -> InstancedWorkload:
+    """Two-level scene: n_meshes soups (as the 65,536-triangle soup) with
+    triangle counts log-uniform in [lo, hi]; n_inst instances, each of a
+    uniformly chosen mesh, with a uniform rotation, scale U[0.5, 2] and
+    translation U[-trans, trans]^3 (50 by default).  The primary camera sits
+    at (0, 0, -3 trans) looking along +z with a 45 degree vertical field of
+    view, so the translation cube fills the view."""
+    rng = rng_for(2)
+    counts = log_uniform_ints(rng, n_meshes, lo, hi)
+    meshes = [triangle_soup(int(c), rng_for(2, 1 + k)) for k, c in enumerate(counts)]
+    pick = rng.integers(0, n_meshes, size=n_inst)
+    xf = random_transforms(rng, n_inst, trans=trans)
+    return InstancedWorkload(meshes, pick, xf, (0.0, 0.0, -3.0 * trans), width, height)
+
+
+def _onb(n):
+    """Orthonormal tangents of unit normals n [R, 3] (Duff et al., JCGT 6(1) 2017)."""
+    s = np.where(n[:, 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (s + n[:, 2])
+    b = n[:, 0] * n[:, 1] * a
+    t = np.stack([1 + s * n[:, 0] ** 2 * a, s * b, -s * n[:, 0]], axis=1)
+    u = np.stack([b, s + n[:, 1] ** 2 * a, -n[:, 1]], axis=1)
+    return t, u
+
+
+def secondary_rays(w: InstancedWorkload, primary: np.ndarray, prim_id: np.ndarray, inst_id: np.ndarray,
+                   t_hit: np.ndarray, per_pixel: int = 2, seed: int = 7) -> np.ndarray:
+    """per_pixel cosine-weighted (Malley) rays from every primary hit point,
+    about the hit triangle's world normal turned towards the camera, with the
+    origin pushed off the surface by 1e-4 (1 + max|p|); pixels whose primary
+    ray missed emit no-op rays (max_t = -1).  Returns [per_pixel * R, 8],
+    the rays of one pixel adjacent."""
+    hit = inst_id != 0xFFFFFFFF
+    R = primary.shape[0]
+    o = primary[:, 0:3].astype(np.float64)
+    d = primary[:, 4:7].astype(np.float64)
+    n = np.zeros((R, 3))
+    idx = np.nonzero(hit)[0]
+    if idx.size:
+        inst = inst_id[idx].astype(np.int64)
+        mesh = w.mesh_of_instance[inst]
+        prim = prim_id[idx].astype(np.int64)
+        v0 = np.empty((idx.size, 3)); v1 = np.empty((idx.size, 3)); v2 = np.empty((idx.size, 3))
+        for m in np.unique(mesh):
+            sel = mesh == m
+            V, I = w.meshes[m]
+            tri = I[prim[sel]]
+            v0[sel], v1[sel], v2[sel] = V[tri[:, 0]], V[tri[:, 1]], V[tri[:, 2]]
+        n_obj = np.cross(v1 - v0, v2 - v0)
+        A = w.transforms[:, :, :3].astype(np.float64)
+        inv_t = np.transpose(np.linalg.inv(A), (0, 2, 1))  # normals transform by the inverse transpose
+        nw = np.einsum("kij,kj->ki", inv_t[inst], n_obj)
+        nw /= np.maximum(np.linalg.norm(nw, axis=1, keepdims=True), 1e-300)
+        flip = np.einsum("ki,ki->k", nw, d[idx]) > 0
+        nw[flip] = -nw[flip]
+        n[idx] = nw
+    p = o + t_hit.astype(np.float64)[:, None] * d
+    rng = rng_for(2, 100 + seed)
+    out = np.zeros((R, per_pixel, 8), np.float32)
+    tb, ub = _onb(np.where(hit[:, None], n, np.array([0.0, 0.0, 1.0])))
+    off = (1e-4 * (1.0 + np.abs(p).max(axis=1)))[:, None] * n
+    for k in range(per_pixel):
+        u1, u2 = rng.uniform(size=R), rng.uniform(size=R)
+        r, phi = np.sqrt(u1), 2 * np.pi * u2
+        dl = (r * np.cos(phi))[:, None] * tb + (r * np.sin(phi))[:, None] * ub + np.sqrt(1 - u1)[:, None] * n
+        out[:, k, 0:3] = np.where(hit[:, None], p + off, 0.0)
+        out[:, k, 3] = 0.0
+        out[:, k, 4:7] = np.where(hit[:, None], dl, np.array([0.0, 0.0, 1.0]))
+        out[:, k, 7] = np.where(hit, 1e30, -1.0)
+    return out.reshape(-1, 8)
