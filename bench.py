@@ -204,6 +204,89 @@ def instanced_bench(ctx, dev, log, reps=5):
             "tlas_build_ms_wall": round(tlas_ms, 3)}
 
 
+def batched_bench(ctx, dev, log, frames=16):
+    """Batched build / refit: 8,192 soups (log-uniform 64-16,384 triangles)
+    built by one rrxCmdBuildGeometryBatch (against one rrCmdBuildGeometry per
+    mesh), then refit + query frames replayed from a HIP graph: each frame
+    jitters every vertex by N(0, 1e-3^2), refits all BLASes in one launch and
+    traces 256 any-hit rays per mesh in one batched launch."""
+    import torch
+
+    from radeonrays_sdk_amd import scenes
+
+    log("batched: generating 8192 meshes")
+    w = scenes.batched()
+    V = torch.from_numpy(w.vertices).to(dev)
+    M = len(w.tri_counts)
+    I = torch.arange(3 * int(w.tri_counts.max()), dtype=torch.int32, device=dev).reshape(-1, 3)
+    spans = [(int(o), int(n)) for o, n in zip(w.vertex_offsets, w.tri_counts)]
+    meshes = [(V[o:o + 3 * n], I[:n]) for o, n in spans]
+    rays = torch.from_numpy(w.rays).to(dev)
+    R = rays.shape[0]
+    results = [ctx.empty(ctx.geometry_requirements(n, 3 * n).result_buffer_size) for _, n in spans]
+    scratch = ctx.empty(ctx.batch_build_requirements(meshes))
+    torch.cuda.synchronize()
+    log("batched: build")
+    accels = ctx.build_geometry_batch(meshes, scratch=scratch, results=results)
+    build_ms = timed(lambda: ctx.build_geometry_batch(meshes, scratch=scratch, results=results), 3, warm=1)
+    single_scratch = ctx.empty(ctx.geometry_requirements(int(w.tri_counts.max())).temporary_build_buffer_size)
+    singles = [ctx.empty(r.numel()) for r in results]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (v, i), r in zip(meshes, singles):
+        ctx.build_geometry(v, i, False, scratch=single_scratch, result=r)
+    torch.cuda.synchronize()
+    per_call_ms = (time.perf_counter() - t0) * 1e3
+    same = all(torch.equal(a.buffer[128:], b[128:]) for a, b in zip(accels[:64], singles[:64]))
+
+    log("batched: refit + query frames (HIP graph)")
+    cur = V.clone()
+    cur_meshes = [(cur[o:o + 3 * n], I[:n]) for o, n in spans]
+    hits = ctx.empty(R * HIT_BYTES)
+    rpm = w.rays_per_mesh
+    requests = [(accels[k], rays[k * rpm:(k + 1) * rpm], "any", "full", hits[k * rpm * HIT_BYTES:(k + 1) * rpm * HIT_BYTES])
+                for k in range(M)]
+    qscratch = ctx.batch_scratch(M, R, rpm)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x52520003)
+
+    def refit():
+        ctx.update_geometry_batch(accels, cur_meshes, scratch=scratch)
+
+    def query():
+        ctx.intersect_batch(requests, scratch=qscratch)
+
+    refit_ms = timed(refit, 3, warm=1)
+    query_ms = timed(query, 3, warm=1)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
+            with ctx.on_stream(s):
+                refit()
+                query()
+    torch.cuda.synchronize()
+    frame_ms = []
+    for f in range(frames):
+        torch.add(V, torch.randn(V.shape, generator=gen, device=dev), alpha=1e-3, out=cur)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        frame_ms.append(a.elapsed_time(b))
+    fm = float(np.median(frame_ms))
+    tris = int(w.tri_counts.sum())
+    return {"workload": f"{M} soups, {tris} tris (log-uniform 64-16384), {rpm} any-hit rays per mesh per frame, "
+                        f"{frames} refit+query frames replayed from a HIP graph",
+            "batch_build_ms": round(build_ms, 3), "batch_build_mtris_s": round(tris / build_ms / 1e3, 1),
+            "per_call_build_ms_wall": round(per_call_ms, 1), "batch_equals_per_call": bool(same),
+            "refit_ms": round(refit_ms, 3), "refit_gbs": round(tris * 304 / (refit_ms * 1e-3) / 1e9, 1),
+            "query_ms": round(query_ms, 3), "query_mrays_s": round(R / query_ms / 1e3, 1),
+            "graph_frame_ms": round(fm, 3), "graph_frame_mrays_s": round(R / fm / 1e3, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +296,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip build/refit/sort timings")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream comparison")
     ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene traversal")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched build / graph refit workload")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
@@ -333,6 +417,8 @@ def main():
 
     if not args.no_instanced:
         out["instanced"] = instanced_bench(ctx, dev, log)
+    if not args.no_batched:
+        out["batched"] = batched_bench(ctx, dev, log)
     if not args.no_serving:
         out["serving"] = serving_bench(ctx, dev, args.serving_requests, log)
 

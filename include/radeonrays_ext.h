@@ -46,6 +46,32 @@ RR_API RRError rrxCmdIntersectWithStats(RRContext        context,
                                         RRDevicePtr      stats,
                                         RRCommandStream  command_stream);
 
+/* Scratch bytes for rrxCmdBuildGeometryBatch over `count` build inputs
+ * (each one triangle mesh, as for rrGetGeometryBuildMemoryRequirements).
+ * Each geometry's result buffer is sized by rrGetGeometryBuildMemoryRequirements
+ * for that input alone. */
+RR_API RRError rrxGetGeometryBatchBuildMemoryRequirements(RRContext                   context,
+                                                          RRGeometryBuildInput const* build_inputs,
+                                                          uint32_t                    count,
+                                                          RRBuildOptions const*       build_options,
+                                                          size_t*                     scratch_size);
+
+/* Build (or, with RR_BUILD_OPERATION_UPDATE, refit) `count` geometries as one
+ * command: meshes of up to 65,536 triangles are built by one persistent
+ * launch in which each workgroup builds whole meshes taken from a queue;
+ * larger meshes follow one at a time.  geometry_buffers[i] receives exactly
+ * what rrCmdBuildGeometry (radeonrays.h:322-327) writes for build_inputs[i]
+ * with the same options.  No counterpart in the reference, which builds
+ * one geometry per call (vlk/intersector.cpp:124-182). */
+RR_API RRError rrxCmdBuildGeometryBatch(RRContext                   context,
+                                        RRBuildOperation            build_operation,
+                                        RRGeometryBuildInput const* build_inputs,
+                                        uint32_t                    count,
+                                        RRBuildOptions const*       build_options,
+                                        RRDevicePtr                 temporary_buffer,
+                                        RRDevicePtr const*          geometry_buffers,
+                                        RRCommandStream             command_stream);
+
 /* Write a geometry (BLAS) in the dump format of the reference's bvh_analyzer:
  * the VkBvhNode array to bvh_path, width x height RRRay records from `rays`
  * (may be NULL: no rays file is written) to rays_path, and the analyzer's

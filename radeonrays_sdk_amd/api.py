@@ -76,6 +76,20 @@ class Context:
                 if p is not None:
                     self.lib.rrReleaseDevicePtr(self.ctx, p)
 
+    @contextmanager
+    def on_stream(self, stream):
+        """Enqueue on `stream` (a torch stream) instead of the context's stream
+        inside the block, e.g. while that stream is captured into a graph."""
+        cs = c_void_p()
+        rr.check(self.lib.rrGetCommandStreamFromHipStream(self.ctx, c_void_p(stream.cuda_stream), byref(cs)),
+                 "rrGetCommandStreamFromHipStream")
+        saved, self.cs = self.cs, cs
+        try:
+            yield self
+        finally:
+            self.cs = saved
+            self.lib.rrReleaseExternalCommandStream(self.ctx, cs)
+
     def empty(self, nbytes: int):
         torch = _torch()
         return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=self.torch_device)
@@ -137,6 +151,51 @@ class Context:
             prim, inp = self._mesh(vertices, indices, v, i)
             rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_UPDATE, byref(inp), None, None, r,
                                                  self.cs), "rrCmdBuildGeometry(update)")
+
+    def _batch_inputs(self, meshes, handles):
+        """ctypes build inputs for [(vertices, indices)] with their RRDevicePtr handles."""
+        n = len(meshes)
+        prims = (rr.RRTriangleMeshPrimitive * max(n, 1))()
+        inputs = (rr.RRGeometryBuildInput * max(n, 1))()
+        for k, (v, i) in enumerate(meshes):
+            vs = v.stride(0) * v.element_size()
+            prims[k] = rr.RRTriangleMeshPrimitive(handles[2 * k], int(v.shape[0]), int(vs), handles[2 * k + 1],
+                                                  int(i.shape[0]),
+                                                  rr.RR_INDEX_TYPE_UINT16 if i.element_size() == 2 else rr.RR_INDEX_TYPE_UINT32)
+            inputs[k] = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, 1,
+                                                c_void_p(ctypes.addressof(prims) + k * ctypes.sizeof(prims[0])))
+        return prims, inputs
+
+    def batch_build_requirements(self, meshes, restructure: bool = False) -> int:
+        _, inputs = self._batch_inputs(meshes, [None] * (2 * len(meshes)))
+        size = c_size_t()
+        opts = self._options(restructure)
+        rr.check(self.lib.rrxGetGeometryBatchBuildMemoryRequirements(self.ctx, inputs, len(meshes), byref(opts),
+                                                                    byref(size)),
+                 "rrxGetGeometryBatchBuildMemoryRequirements")
+        return size.value
+
+    def build_geometry_batch(self, meshes, restructure: bool = False, scratch=None, results=None, update=False):
+        """rrxCmdBuildGeometryBatch over [(vertices, indices)] (cuda tensors); returns one Accel per mesh."""
+        meshes = [(v, i.contiguous()) for v, i in meshes]
+        if results is None:
+            results = [self.empty(self.geometry_requirements(int(i.shape[0]), max(int(v.shape[0]), 1),
+                                                             restructure).result_buffer_size) for v, i in meshes]
+        scratch = scratch if scratch is not None else self.empty(self.batch_build_requirements(meshes, restructure))
+        addrs = [a for v, i in meshes for a in (v.data_ptr(), i.data_ptr())]
+        with self.ptrs(*addrs, *[r.data_ptr() for r in results], scratch.data_ptr()) as hs:
+            n = len(meshes)
+            prims, inputs = self._batch_inputs(meshes, hs[:2 * n])
+            outs = (c_void_p * max(n, 1))(*[h.value if h is not None else None for h in hs[2 * n:3 * n]])
+            opts = self._options(restructure)
+            op = rr.RR_BUILD_OPERATION_UPDATE if update else rr.RR_BUILD_OPERATION_BUILD
+            rr.check(self.lib.rrxCmdBuildGeometryBatch(self.ctx, op, inputs, n, byref(opts), hs[-1], outs, self.cs),
+                     "rrxCmdBuildGeometryBatch")
+        return [Accel(r, "blas", int(i.shape[0])) for r, (v, i) in zip(results, meshes)]
+
+    def update_geometry_batch(self, accels, meshes, scratch=None):
+        """Refit every accel to its mesh's moved vertices (rrxCmdBuildGeometryBatch, UPDATE)."""
+        return self.build_geometry_batch(meshes, scratch=scratch, results=[a.buffer for a in accels], update=True)
 
     def scene_requirements(self, count: int):
         inp = rr.RRSceneBuildInput(None, count)

@@ -31,6 +31,7 @@
 #include "../../include/radeonrays_ext.h"
 #include "../../include/radeonrays_hip.h"
 #include "api_internal.h"
+#include "batch.h"
 #include "build.h"
 #include "rr_layout.h"
 #include "sort.h"
@@ -125,6 +126,9 @@ struct Context {
     std::unordered_map<void*, size_t> pinned_size;
     std::vector<void*> pinned_free;
     size_t pinned_free_bytes = 0;
+    // Staging read by commands captured into a graph: every replay reads it
+    // again, so it lives until the context is destroyed.
+    std::vector<void*> graph_staging;
 
     // Pinned host block of at least `bytes` (caller holds mu).
     void* AcquirePinned(size_t bytes) {
@@ -158,6 +162,8 @@ struct Context {
         for (void* p : pinned_free) (void)hipHostFree(p);
         pinned_free.clear();
         pinned_free_bytes = 0;
+        for (void* p : graph_staging) (void)hipHostFree(p);
+        graph_staging.clear();
     }
 
     void Reap(bool wait_all) {
@@ -207,6 +213,26 @@ void Record(CommandStream* cs, Command cmd) {
     } else {
         cs->commands.push_back(std::move(cmd));
     }
+}
+
+bool IsCapturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// Hand pinned staging read by a just-recorded command to its owner: freed
+// once the work has run (external stream), or with the command stream's
+// submission (internal stream); kept for the context's life when the stream
+// is being captured into a graph.
+void KeepStaging(Context* c, CommandStream* cs, void* host) {
+    if (!host) return;
+    if (!cs->external) {
+        cs->staging.push_back(host);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (IsCapturing(cs->stream)) c->graph_staging.push_back(host);
+    else c->FreeAfter(cs->stream, host);
 }
 
 template <typename F>
@@ -471,16 +497,8 @@ RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input,
                       "hipMemcpyAsync");
             rr::BuildTlas(dev_inst, count, scratch, result, s);
         };
-        if (cs->external) {
-            Record(cs, cmd);
-            if (host) {
-                std::lock_guard<std::mutex> lk(c->mu);
-                c->FreeAfter(cs->stream, host);
-            }
-        } else {
-            Record(cs, cmd);
-            if (host) cs->staging.push_back(host);
-        }
+        Record(cs, cmd);
+        KeepStaging(c, cs, host);
         return RR_SUCCESS;
     });
 }
@@ -547,7 +565,11 @@ RRError rrAllocateCommandStream(RRContext context, RRCommandStream* command_stre
 static void ReleaseStaging(Context* c, CommandStream* cs) {
     if (cs->staging.empty()) return;
     std::lock_guard<std::mutex> lk(c->mu);
-    for (void* p : cs->staging) c->FreeAfter(c->stream, p);
+    const bool capturing = IsCapturing(c->stream);
+    for (void* p : cs->staging) {
+        if (capturing) c->graph_staging.push_back(p);
+        else c->FreeAfter(c->stream, p);
+    }
     cs->staging.clear();
 }
 
@@ -881,14 +903,127 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
                 rr::Trace(x.accel, rr::kAccelTlas, x.q, x.o, x.rays, x.n, nullptr, x.hits, sc, s);
         };
         Record(cs, cmd);
-        if (host) {
-            if (cs->external) {
-                std::lock_guard<std::mutex> lk(c->mu);
-                c->FreeAfter(cs->stream, host);
-            } else {
-                cs->staging.push_back(host);
-            }
+        KeepStaging(c, cs, host);
+        return RR_SUCCESS;
+    });
+}
+
+// Batched geometry builds.  Scratch: queue + mesh table, then a work area
+// per batched mesh; meshes above kBatchMaxTris are built one at a time from
+// the start of the scratch after the batch.
+struct BatchPlan {
+    std::vector<rr::MeshDesc> meshes;
+    std::vector<uint32_t> small, large;  // indices into meshes; small sorted by size, largest first
+    std::vector<size_t> work_offset;     // per small mesh
+    size_t head = 0, total = 0;
+};
+
+static RRError PlanBatch(const RRGeometryBuildInput* inputs, uint32_t count, bool restructure, bool resolve,
+                         BatchPlan& plan, const char* name) {
+    plan.meshes.resize(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const RRError e = MeshFromInput(&inputs[i], plan.meshes[i], name, resolve);
+        if (e != RR_SUCCESS) return e;
+        (plan.meshes[i].tri_count <= rr::kBatchMaxTris ? plan.small : plan.large).push_back(i);
+    }
+    std::stable_sort(plan.small.begin(), plan.small.end(),
+                     [&](uint32_t a, uint32_t b) { return plan.meshes[a].tri_count > plan.meshes[b].tri_count; });
+    plan.head = rr::BatchHeadBytes((uint32_t)plan.small.size());
+    size_t o = plan.head;
+    plan.work_offset.resize(plan.small.size());
+    for (size_t k = 0; k < plan.small.size(); ++k) {
+        plan.work_offset[k] = o;
+        o += rr::BatchWorkBytes(plan.meshes[plan.small[k]].tri_count);
+    }
+    plan.total = o;
+    for (uint32_t i : plan.large) plan.total = std::max(plan.total, rr::BlasBuildScratchBytes(plan.meshes[i].tri_count, restructure));
+    return RR_SUCCESS;
+}
+
+RRError rrxGetGeometryBatchBuildMemoryRequirements(RRContext context, const RRGeometryBuildInput* build_inputs,
+                                                   uint32_t count, const RRBuildOptions* build_options,
+                                                   size_t* scratch_size) {
+    RR_REQUIRE(context && scratch_size && (count == 0 || build_inputs), "rrxGetGeometryBatchBuildMemoryRequirements");
+    BatchPlan plan;
+    const RRError e = PlanBatch(build_inputs, count, WantsRestructure(build_options), false, plan,
+                                "rrxGetGeometryBatchBuildMemoryRequirements");
+    if (e != RR_SUCCESS) return e;
+    *scratch_size = std::max<size_t>(plan.total, 256);
+    return RR_SUCCESS;
+}
+
+RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_operation,
+                                 const RRGeometryBuildInput* build_inputs, uint32_t count,
+                                 const RRBuildOptions* build_options, RRDevicePtr temporary_buffer,
+                                 const RRDevicePtr* geometry_buffers, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrxCmdBuildGeometryBatch");
+    RR_REQUIRE(context && command_stream && temporary_buffer, "rrxCmdBuildGeometryBatch");
+    RR_REQUIRE(count == 0 || (build_inputs && geometry_buffers), "rrxCmdBuildGeometryBatch");
+    RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
+               "rrxCmdBuildGeometryBatch");
+    for (uint32_t i = 0; i < count; ++i) RR_REQUIRE(geometry_buffers[i], "rrxCmdBuildGeometryBatch");
+    return Guarded("rrxCmdBuildGeometryBatch", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        CommandStream* cs = AsStream(command_stream);
+        DeviceGuard g(c->device);
+        const bool restructure = WantsRestructure(build_options);
+        BatchPlan plan;
+        const RRError e = PlanBatch(build_inputs, count, restructure, true, plan, "rrxCmdBuildGeometryBatch");
+        if (e != RR_SUCCESS) return e;
+        void* scratch = Addr(temporary_buffer);
+        std::vector<void*> results(count);
+        for (uint32_t i = 0; i < count; ++i) results[i] = Addr(geometry_buffers[i]);
+        const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
+        if (!update) {
+            std::lock_guard<std::mutex> lk(c->mu);
+            for (void* r : results) c->kinds[reinterpret_cast<uintptr_t>(r)] = rr::kAccelBlas;
         }
+        // The mesh table, packed now into pinned memory that the recorded
+        // upload reads at execution time.
+        const uint32_t n_small = (uint32_t)plan.small.size();
+        rr::BatchMesh* host = nullptr;
+        if (n_small) {
+            std::lock_guard<std::mutex> lk(c->mu);
+            host = static_cast<rr::BatchMesh*>(c->AcquirePinned(sizeof(rr::BatchMesh) * n_small));
+        }
+        for (uint32_t k = 0; k < n_small; ++k) {
+            const rr::MeshDesc& m = plan.meshes[plan.small[k]];
+            rr::BatchMesh b{};
+            b.vertices = reinterpret_cast<uint64_t>(m.vertices);
+            b.indices = reinterpret_cast<uint64_t>(m.indices);
+            b.result = reinterpret_cast<uint64_t>(results[plan.small[k]]);
+            b.work = reinterpret_cast<uint64_t>(static_cast<uint8_t*>(scratch) + plan.work_offset[k]);
+            b.vertex_count = m.vertex_count;
+            b.vertex_stride = m.vertex_stride;
+            b.index16 = m.index16;
+            b.tri_count = m.tri_count;
+            b.pad[0] = (restructure && m.tri_count >= 64) ? rr::kAccelFlagRestructured : 0u;
+            host[k] = b;
+        }
+        auto cmd = [=](hipStream_t s) {
+            if (n_small) {
+                auto* table = reinterpret_cast<rr::BatchMesh*>(static_cast<uint8_t*>(scratch) + rr::kMeshTableOffset);
+                Check(hipMemcpyAsync(table, host, sizeof(rr::BatchMesh) * n_small, hipMemcpyHostToDevice, s),
+                      "hipMemcpyAsync");
+                if (update) {
+                    rr::BatchRefit(table, n_small, scratch, s);
+                } else {
+                    rr::BatchBuild(table, n_small, scratch, s);
+                    // Treelet restructuring, mesh by mesh, in each mesh's own work area.
+                    if (restructure)
+                        for (uint32_t k = 0; k < n_small; ++k)
+                            rr::RestructureBlas(plan.meshes[plan.small[k]].tri_count,
+                                                static_cast<uint8_t*>(scratch) + plan.work_offset[k],
+                                                results[plan.small[k]], s);
+                }
+            }
+            for (uint32_t i : plan.large) {
+                if (update) rr::UpdateBlas(plan.meshes[i], results[i], s);
+                else rr::BuildBlas(plan.meshes[i], restructure, scratch, results[i], s);
+            }
+        };
+        Record(cs, cmd);
+        KeepStaging(c, cs, host);
         return RR_SUCCESS;
     });
 }

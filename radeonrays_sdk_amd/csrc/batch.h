@@ -1,0 +1,51 @@
+// batch.h — host interface of the batched BLAS build and refit: many small
+// meshes built (or refitted) by one persistent launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "build.h"
+
+namespace rr {
+
+// Meshes up to this many triangles go through the batched kernels; larger
+// ones are built one at a time by BuildBlas.
+constexpr uint32_t kBatchMaxTris = 1u << 16;
+
+// One mesh of a batch as uploaded to the device (64 bytes).
+struct BatchMesh {
+    uint64_t vertices;
+    uint64_t indices;
+    uint64_t result;   // BLAS result buffer
+    uint64_t work;     // this mesh's key/value work area (BatchWorkBytes)
+    uint32_t vertex_count;
+    uint32_t vertex_stride;
+    uint32_t index16;
+    uint32_t tri_count;
+    uint32_t pad[4];
+};
+static_assert(sizeof(BatchMesh) == 64, "BatchMesh must be 64 bytes");
+
+// Bytes of a mesh's work area: keys, values and their ping-pong copies, or
+// the treelet-restructuring scratch if that is larger.
+size_t BatchWorkBytes(uint32_t tris);
+
+// Scratch head of a batch: queue counter, then the mesh table.
+constexpr size_t kMeshTableOffset = 256;
+inline size_t BatchHeadBytes(uint32_t meshes) {
+    return kMeshTableOffset + AlignUp(size_t(meshes) * sizeof(BatchMesh), 256);
+}
+
+// LBVH build of every mesh in the table (already in device memory at
+// `table`, inside `scratch`): each workgroup takes the next mesh from a
+// queue and runs bounds, Morton codes, a stable four-pass radix sort,
+// Karras emission and refit for it alone.  The results equal BuildBlas with
+// restructure off, mesh by mesh.
+void BatchBuild(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t stream);
+
+// Refit of every mesh in the table (the topology of each BLAS is kept), as
+// UpdateBlas mesh by mesh.
+void BatchRefit(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t stream);
+
+}  // namespace rr

@@ -1,0 +1,287 @@
+// batch.hip — batched BLAS build and refit for gfx950.
+//
+// Many small meshes (up to kBatchMaxTris triangles each) are built by one
+// persistent launch: each workgroup takes the next mesh from a queue (the
+// host orders the table largest first) and runs the whole LBVH pipeline of
+// the single-mesh builder for it with block-level synchronisation only —
+// bounds (wave64 reductions), Morton codes, a stable LSD radix sort of the
+// (code, index) pairs in four 8-bit passes, Karras emission and the
+// bottom-up refit.  Every step computes exactly what the single-mesh kernels
+// compute (min/max are exact, the sort is stable, emission and refit are the
+// shared device functions), so each result equals BuildBlas of that mesh.
+// Behavioural spec of each step: as build.hip (vlk/hlbvh_builder.cpp:159-351).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+
+#include "batch.h"
+#include "build_device.h"
+#include "rr_device.h"
+#include "sort.h"
+
+namespace rr {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kChunkItems = 4;  // keys per thread per sort chunk
+constexpr int kChunk = kBlock * kChunkItems;
+
+__device__ __forceinline__ uint32_t BlockScan(uint32_t v, uint32_t* s_wave_tot) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) s_wave_tot[wave] = x;
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) add += (w < (int)wave) ? s_wave_tot[w] : 0u;
+    __syncthreads();
+    return x - v + add;
+}
+
+struct BatchShared {
+    uint32_t mesh;
+    uint32_t bounds[kWaves][6];
+    uint32_t wave_tot[kWaves];
+    uint32_t offs[kSortRadix];                 // running output base per digit
+    uint32_t wave_hist[kWaves][kSortRadix];
+};
+
+__device__ __forceinline__ MeshView ViewOfBatch(const BatchMesh& b) {
+    return MeshView{reinterpret_cast<const uint8_t*>(b.vertices), reinterpret_cast<const void*>(b.indices),
+                    b.vertex_stride, b.index16, b.vertex_count, b.tri_count};
+}
+
+// Stable LSD sort of n (key, value) pairs by the 8-bit digit at `shift`,
+// from (ki, vi) to (ko, vo), by one workgroup.
+__device__ void BlockSortPass(const uint32_t* __restrict__ ki, const uint32_t* __restrict__ vi, uint32_t* __restrict__ ko,
+                              uint32_t* __restrict__ vo, uint32_t n, uint32_t shift, BatchShared& sh) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // Digit counts, then their exclusive scan: each digit's output base.
+    for (uint32_t i = tid; i < kWaves * kSortRadix; i += kBlock) (&sh.wave_hist[0][0])[i] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += kBlock) atomicAdd(&sh.wave_hist[0][(ki[t] >> shift) & (kSortRadix - 1)], 1u);
+    __syncthreads();
+    const uint32_t total = sh.wave_hist[0][tid];
+    const uint32_t base = BlockScan(total, sh.wave_tot);
+    sh.offs[tid] = base;
+    sh.wave_hist[0][tid] = 0;
+    __syncthreads();
+    // Chunks in order; waves in order within a chunk; lanes in order within
+    // an item: keys keep their relative order within each digit.
+    for (uint32_t c0 = 0; c0 < n; c0 += kChunk) {
+        uint32_t k[kChunkItems], v[kChunkItems], rank[kChunkItems];
+        bool ok[kChunkItems];
+#pragma unroll
+        for (int j = 0; j < kChunkItems; ++j) {
+            const uint32_t idx = c0 + wave * (64 * kChunkItems) + j * 64 + lane;
+            ok[j] = idx < n;
+            k[j] = ok[j] ? ki[idx] : 0u;
+            v[j] = ok[j] ? vi[idx] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kChunkItems; ++j) {
+            const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
+            const uint64_t valid = __ballot(ok[j]);
+            uint32_t m_lo = (uint32_t)valid, m_hi = (uint32_t)(valid >> 32);
+#pragma unroll
+            for (int b = 0; b < kSortRadixBits; ++b) {
+                const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)k[j], shift + b, 1);
+                const uint64_t bal = __ballot(neg != 0u);
+                m_lo &= ~((uint32_t)bal ^ neg);
+                m_hi &= ~((uint32_t)(bal >> 32) ^ neg);
+            }
+            const uint32_t before = __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
+            const uint32_t pre = sh.wave_hist[wave][d];
+            rank[j] = pre + before;
+            __builtin_amdgcn_wave_barrier();
+            if (ok[j] && before == 0) sh.wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        // Per digit: exclusive offsets of the waves, and the chunk's count.
+        uint32_t count = 0;
+        {
+            const uint32_t d = tid;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = sh.wave_hist[w][d];
+                sh.wave_hist[w][d] = count;
+                count += c;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kChunkItems; ++j) {
+            if (!ok[j]) continue;
+            const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
+            const uint32_t pos = sh.offs[d] + sh.wave_hist[wave][d] + rank[j];
+            ko[pos] = k[j];
+            vo[pos] = v[j];
+        }
+        __syncthreads();
+        sh.offs[tid] += count;
+        for (int w = 0; w < kWaves; ++w) sh.wave_hist[w][tid] = 0;
+        __syncthreads();
+    }
+}
+
+__global__ void BatchQueueInitKernel(uint32_t* queue) {
+    if (threadIdx.x == 0) *queue = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                                           uint32_t* queue) {
+    __shared__ BatchShared sh;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (;;) {
+        if (tid == 0) sh.mesh = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t mi = sh.mesh;
+        __syncthreads();
+        if (mi >= meshes) break;
+        const BatchMesh bm = table[mi];
+        const MeshView m = ViewOfBatch(bm);
+        const uint32_t n = bm.tri_count;
+        uint8_t* result = reinterpret_cast<uint8_t*>(bm.result);
+        AccelHeader* hdr = reinterpret_cast<AccelHeader*>(result);
+        BvhNode* nodes = reinterpret_cast<BvhNode*>(result + sizeof(AccelHeader));
+        if (tid == 0) {
+            // As InitBuildKernel.
+            const uint32_t serial = hdr->magic == kAccelMagic ? hdr->build_serial + 1 : 1;
+            AccelHeader h = {};
+            h.magic = kAccelMagic;
+            h.kind = kAccelBlas;
+            h.prim_count = n;
+            h.node_count = NodeCount(n);
+            h.nodes_offset = sizeof(AccelHeader);
+            h.flags = bm.pad[0];
+            h.build_serial = serial;
+            *hdr = h;
+        }
+        if (n == 0) continue;
+
+        // Bounds of all vertices (ordered-uint min/max: exact).
+        uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+        for (uint32_t t = tid; t < n; t += kBlock) {
+            float v[9];
+            LoadTriangle(m, t, v);
+            for (int c = 0; c < 3; ++c)
+                for (int a = 0; a < 3; ++a) {
+                    const uint32_t o = FloatToOrdered(v[3 * c + a]);
+                    mn[a] = min(mn[a], o);
+                    mx[a] = max(mx[a], o);
+                }
+        }
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = WaveMin(mn[a]);
+            mx[a] = WaveMax(mx[a]);
+        }
+        if (lane == 0)
+            for (int a = 0; a < 3; ++a) {
+                sh.bounds[wave][a] = mn[a];
+                sh.bounds[wave][3 + a] = mx[a];
+            }
+        __syncthreads();
+        float lo[3], ext[3];
+        for (int a = 0; a < 3; ++a) {
+            uint32_t l = sh.bounds[0][a], h = sh.bounds[0][3 + a];
+            for (int w = 1; w < kWaves; ++w) {
+                l = min(l, sh.bounds[w][a]);
+                h = max(h, sh.bounds[w][3 + a]);
+            }
+            lo[a] = OrderedToFloat(l);
+            ext[a] = OrderedToFloat(h) - lo[a];
+        }
+
+        // Morton codes of the triangle-box centroids, values = triangle index.
+        uint32_t* keys = reinterpret_cast<uint32_t*>(bm.work);
+        uint32_t* vals = keys + AlignUp(n, 64);
+        uint32_t* ktmp = vals + AlignUp(n, 64);
+        uint32_t* vtmp = ktmp + AlignUp(n, 64);
+        for (uint32_t t = tid; t < n; t += kBlock) {
+            float c[3];
+            TriangleCentroid(m, t, c);
+            keys[t] = Morton30(UnitCoord(c[0], lo[0], ext[0]), UnitCoord(c[1], lo[1], ext[1]),
+                               UnitCoord(c[2], lo[2], ext[2]));
+            vals[t] = t;
+        }
+        __syncthreads();
+        // Four passes: keys -> tmp -> keys -> tmp -> keys.
+        for (int p = 0; p < SortPasses(30); ++p) {
+            const bool even = (p & 1) == 0;
+            BlockSortPass(even ? keys : ktmp, even ? vals : vtmp, even ? ktmp : keys, even ? vtmp : vals, n,
+                          (uint32_t)(p * kSortRadixBits), sh);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kBlock) EmitNode(keys, vals, n, i, nodes);
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kBlock) RefitLeaf(m, nodes, i);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                                           uint32_t* queue) {
+    __shared__ uint32_t s_mesh;
+    for (;;) {
+        if (threadIdx.x == 0) s_mesh = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t mi = s_mesh;
+        __syncthreads();
+        if (mi >= meshes) break;
+        const BatchMesh bm = table[mi];
+        const MeshView m = ViewOfBatch(bm);
+        BvhNode* nodes = reinterpret_cast<BvhNode*>(reinterpret_cast<uint8_t*>(bm.result) + sizeof(AccelHeader));
+        for (uint32_t i = threadIdx.x; i < bm.tri_count; i += kBlock) RefitLeaf(m, nodes, i);
+    }
+}
+
+// Resident blocks of `kernel` on the current device, cached per device
+// (which: 0 build, 1 refit).
+uint32_t ResidentBlocks(const void* kernel, int which) {
+    static std::atomic<uint32_t> cache[2][64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    uint32_t b = cache[which][dev].load(std::memory_order_relaxed);
+    if (!b) {
+        int per_cu = 0, cus = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        b = (uint32_t)std::max(1, per_cu * std::max(1, cus));
+        cache[which][dev].store(b, std::memory_order_relaxed);
+    }
+    return b;
+}
+
+}  // namespace
+
+size_t BatchWorkBytes(uint32_t tris) {
+    return std::max(AlignUp(size_t(AlignUp(tris, 64)) * 16, 256), RestructureScratchBytes(tris));
+}
+
+void BatchBuild(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t s) {
+    if (meshes == 0) return;
+    uint32_t* queue = static_cast<uint32_t*>(scratch);
+    BatchQueueInitKernel<<<1, 64, 0, s>>>(queue);
+    const uint32_t blocks = std::min(meshes, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildKernel), 0));
+    BatchBuildKernel<<<blocks, kBlock, 0, s>>>(table, meshes, queue);
+}
+
+void BatchRefit(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t s) {
+    if (meshes == 0) return;
+    uint32_t* queue = static_cast<uint32_t*>(scratch);
+    BatchQueueInitKernel<<<1, 64, 0, s>>>(queue);
+    const uint32_t blocks = std::min(meshes, ResidentBlocks(reinterpret_cast<const void*>(&BatchRefitKernel), 1));
+    BatchRefitKernel<<<blocks, kBlock, 0, s>>>(table, meshes, queue);
+}
+
+}  // namespace rr

@@ -28,6 +28,8 @@ struct InstanceUpload {
 static_assert(sizeof(InstanceUpload) == 64, "InstanceUpload must be 64 bytes");
 
 size_t BlasBuildScratchBytes(uint32_t tris, bool restructure);
+// Scratch RestructureBlas needs for a BLAS of `tris` triangles.
+size_t RestructureScratchBytes(uint32_t tris);
 size_t TlasBuildScratchBytes(uint32_t instances);
 
 // LBVH build of one mesh into `result` (BlasResultSize bytes), optionally

@@ -1,0 +1,178 @@
+// build_device.h — device helpers shared by the single-mesh builder
+// (build.hip) and the batched builder (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "build.h"
+#include "rr_device.h"
+
+namespace rr {
+
+struct MeshView {
+    const uint8_t* vertices;
+    const void*    indices;
+    uint32_t       stride;
+    uint32_t       index16;
+    uint32_t       vertex_count;
+    uint32_t       tri_count;
+};
+
+inline MeshView ViewOf(const MeshDesc& m) {
+    return MeshView{static_cast<const uint8_t*>(m.vertices), m.indices, m.vertex_stride, m.index16, m.vertex_count,
+                    m.tri_count};
+}
+
+// Fetch triangle t.  Indices past the vertex array are clamped to its last
+// vertex so that a bad index buffer cannot read out of bounds.
+__device__ __forceinline__ void LoadTriangle(const MeshView& m, uint32_t t, float v[9]) {
+    uint32_t i0, i1, i2;
+    if (m.index16) {
+        const uint16_t* ix = static_cast<const uint16_t*>(m.indices) + 3 * (size_t)t;
+        i0 = ix[0]; i1 = ix[1]; i2 = ix[2];
+    } else {
+        const uint32_t* ix = static_cast<const uint32_t*>(m.indices) + 3 * (size_t)t;
+        i0 = ix[0]; i1 = ix[1]; i2 = ix[2];
+    }
+    const uint32_t last = m.vertex_count - 1;
+    i0 = min(i0, last); i1 = min(i1, last); i2 = min(i2, last);
+    const float* p0 = reinterpret_cast<const float*>(m.vertices + (size_t)i0 * m.stride);
+    const float* p1 = reinterpret_cast<const float*>(m.vertices + (size_t)i1 * m.stride);
+    const float* p2 = reinterpret_cast<const float*>(m.vertices + (size_t)i2 * m.stride);
+    v[0] = p0[0]; v[1] = p0[1]; v[2] = p0[2];
+    v[3] = p1[0]; v[4] = p1[1]; v[5] = p1[2];
+    v[6] = p2[0]; v[7] = p2[1]; v[8] = p2[2];
+}
+
+// Centroid of triangle t's box (lbvh_calc_morton_codes_mesh.comp:106-112).
+__device__ __forceinline__ void TriangleCentroid(const MeshView& m, uint32_t t, float c[3]) {
+    float v[9];
+    LoadTriangle(m, t, v);
+    for (int a = 0; a < 3; ++a) {
+        const float lo = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+        const float hi = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+        c[a] = (lo + hi) * 0.5f;
+    }
+}
+
+// Karras 2012 binary radix tree over sorted codes
+// (lbvh_emit_hierarchy_mesh.comp:85-217).  delta(i, j) is the common prefix
+// length of codes i and j, with 32 + clz(i ^ j) for equal codes and -1 for
+// j outside [0, n).
+__device__ __forceinline__ int Delta(const uint32_t* __restrict__ codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = codes[i], b = codes[j];
+    return a != b ? __clz(a ^ b) : 32 + __clz((uint32_t)(i ^ j));
+}
+
+// Node i's topology (and leaf i's links) from the sorted codes: Karras 2012.
+__device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ prims,
+                                         uint32_t n, uint32_t i, BvhNode* __restrict__ nodes) {
+    const uint32_t leaf = n - 1 + i;
+    nodes[leaf].child0 = kInvalid;
+    nodes[leaf].child1 = prims[i];
+    nodes[leaf].update = 0;
+    if (n == 1) {
+        nodes[0].parent = kInvalid;
+        return;
+    }
+    if (i >= n - 1) return;
+    const int N = (int)n, I = (int)i;
+    // Direction and far end of node i's key range.
+    const int d = (Delta(codes, N, I, I + 1) - Delta(codes, N, I, I - 1)) > 0 ? 1 : -1;
+    const int dmin = Delta(codes, N, I, I - d);
+    int lmax = 2;
+    while (Delta(codes, N, I, I + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (Delta(codes, N, I, I + (l + t) * d) > dmin) l += t;
+    const int j = I + l * d;
+    const int first = min(I, j), last = max(I, j);
+    // Split: last position that shares more than the range's common prefix with `first`.
+    const int common = Delta(codes, N, first, last);
+    int lo = first, hi = last;
+    while (hi > lo + 1) {
+        const int mid = (lo + hi) / 2;
+        if (Delta(codes, N, first, mid) > common) lo = mid;
+        else hi = mid;
+    }
+    const int split = lo;
+    const uint32_t left = (split == first) ? (n - 1 + (uint32_t)split) : (uint32_t)split;
+    const uint32_t right = (split + 1 == last) ? (n - 1 + (uint32_t)split + 1) : (uint32_t)split + 1;
+    nodes[i].child0 = left;
+    nodes[i].child1 = right;
+    nodes[i].update = 0;
+    nodes[left].parent = i;
+    nodes[right].parent = i;
+    if (i == 0) nodes[0].parent = kInvalid;
+}
+
+// Bottom-up refit shared by BLAS build/update and TLAS build
+// (lbvh_fit_aabb_mesh.comp:122-205, lbvh_fit_aabb_scene.comp:98-167).  Each
+// finished node stores its box into its parent's child slot, then bumps the
+// parent's arrival counter; an odd previous value means the sibling already
+// stored its box, so this thread merges both and climbs on.
+//
+// Cross-workgroup hand-off without an L2 write-back per step: the slot is
+// written with agent-scope (write-through) stores, the wave waits for them to
+// complete (vmcnt(0)) before the counter's atomic add, and the second
+// arriver reads the sibling's slot with agent-scope loads, which do not hit
+// stale cache lines.  Both threads' accesses to a slot are agent-scope
+// atomics, so the counter orders them.
+__device__ __forceinline__ void StoreAgent(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float LoadAgent(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void WaitStores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+__device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t child, uint32_t p,
+                                           float lo[3], float hi[3]) {
+    // Guards: a parent index outside the node array or more levels than any
+    // tree of `total` nodes can have stops the climb instead of faulting.
+    for (uint32_t level = 0; p != kInvalid && p < total && level < total; ++level) {
+        BvhNode* pn = nodes + p;
+        const uint32_t c0 = pn->child0;
+        float* slot_lo = c0 == child ? pn->a0_min_or_v0 : pn->a1_min_or_v2;
+        float* slot_hi = c0 == child ? pn->a0_max_or_v1 : pn->a1_max_or_v3;
+        const float* other_lo = c0 == child ? pn->a1_min_or_v2 : pn->a0_min_or_v0;
+        const float* other_hi = c0 == child ? pn->a1_max_or_v3 : pn->a0_max_or_v1;
+        for (int a = 0; a < 3; ++a) {
+            StoreAgent(slot_lo + a, lo[a]);
+            StoreAgent(slot_hi + a, hi[a]);
+        }
+        WaitStores();
+        const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 1u) == 0) return;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], LoadAgent(other_lo + a));
+            hi[a] = fmaxf(hi[a], LoadAgent(other_hi + a));
+        }
+        child = p;
+        p = pn->parent;
+    }
+}
+
+// Leaf i: write its triangle (and links) into the leaf node, then climb.
+__device__ __forceinline__ void RefitLeaf(const MeshView& m, BvhNode* __restrict__ nodes, uint32_t i) {
+    const uint32_t n = m.tri_count;
+    const uint32_t leaf = n - 1 + i;
+    const uint32_t prim = nodes[leaf].child1;
+    const uint32_t parent = nodes[leaf].parent;
+    float v[9];
+    LoadTriangle(m, prim, v);
+    float4* q = reinterpret_cast<float4*>(nodes + leaf);
+    q[0] = make_float4(v[0], v[1], v[2], __uint_as_float(kInvalid));
+    q[1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
+    q[2] = make_float4(v[6], v[7], v[8], __uint_as_float(parent));
+    q[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+        hi[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+    }
+    ClimbRefit(nodes, 2 * n - 1, leaf, parent, lo, hi);
+}
+
+}  // namespace rr

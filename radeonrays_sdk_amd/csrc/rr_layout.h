@@ -63,7 +63,7 @@ struct Transform {
 };
 static_assert(sizeof(Transform) == 48, "Transform must be 48 bytes");
 
-inline size_t AlignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+__host__ __device__ inline size_t AlignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr uint32_t NodeCount(uint32_t prims) { return prims ? 2u * prims - 1u : 0u; }
 

@@ -29,7 +29,7 @@ SortShape ChooseSortShape(uint32_t n);
 size_t SortScratchBytes(uint32_t n);
 
 // Number of 8-bit passes for keys whose bits above key_bits are zero.
-inline int SortPasses(int key_bits) { return (key_bits + kSortRadixBits - 1) / kSortRadixBits; }
+__host__ __device__ inline int SortPasses(int key_bits) { return (key_bits + kSortRadixBits - 1) / kSortRadixBits; }
 
 struct SortScratch {
     uint32_t* hist;     // [kSortMaxPasses][256] global digit counts

@@ -151,6 +151,10 @@ _SIGNATURES = {
     "rrxGetSortMemoryRequirements": [c_void_p, c_uint32, POINTER(c_size_t)],
     "rrxCmdSortPairs": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p],
     "rrxCmdIntersectWithStats": [c_void_p, c_void_p, c_int, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p],
+    "rrxGetGeometryBatchBuildMemoryRequirements": [c_void_p, c_void_p, c_uint32, POINTER(RRBuildOptions),
+                                                   POINTER(c_size_t)],
+    "rrxCmdBuildGeometryBatch": [c_void_p, c_int, c_void_p, c_uint32, POINTER(RRBuildOptions), c_void_p, c_void_p,
+                                 c_void_p],
     "rrxWriteAnalyzerDump": [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_char_p, c_char_p, c_char_p],
     "rrxGetIntersectBatchMemoryRequirements": [c_void_p, c_uint32, c_uint64, c_uint32, POINTER(c_size_t)],
     "rrxCmdIntersectBatch": [c_void_p, POINTER(RRxIntersectRequest), c_uint32, c_void_p, c_void_p],

@@ -332,3 +332,31 @@ def secondary_rays(w: InstancedWorkload, primary: np.ndarray, prim_id: np.ndarra
         out[:, k, 4:7] = np.where(hit[:, None], dl, np.array([0.0, 0.0, 1.0]))
         out[:, k, 7] = np.where(hit, 1e30, -1.0)
     return out.reshape(-1, 8)
+
+
+# --- batched build / refit -------------------------------------------------------
+@dataclass
+class BatchedWorkload:
+    vertices: np.ndarray    # all meshes' vertices, concatenated [sum 3 n_k, 3]
+    tri_counts: np.ndarray  # [M]
+    vertex_offsets: np.ndarray  # [M] first vertex of each mesh
+    rays: np.ndarray        # [M * rays_per_mesh, 8], mesh k's rays at k * rays_per_mesh
+    rays_per_mesh: int
+
+
+def batched(n_meshes: int = 8192, lo: int = 64, hi: int = 16384, rays_per_mesh: int = 256) -> BatchedWorkload:
+    """Independent soups (as the 65,536-triangle soup) with triangle counts
+    log-uniform in [lo, hi]; per mesh, rays_per_mesh rays with origins
+    uniform in the mesh's box and directions uniform on the sphere."""
+    rng = rng_for(3)
+    counts = log_uniform_ints(rng, n_meshes, lo, hi)
+    total = int(counts.sum())
+    c = rng.uniform(0.0, 1.0, size=(total, 1, 3))
+    v = (c + rng.normal(0.0, 0.01, size=(total, 3, 3))).astype(np.float32).reshape(-1, 3)
+    offs = np.concatenate([[0], np.cumsum(3 * counts)[:-1]]).astype(np.int64)
+    lo_box = np.minimum.reduceat(v, offs, axis=0).astype(np.float64)
+    hi_box = np.maximum.reduceat(v, offs, axis=0).astype(np.float64)
+    u = rng.uniform(size=(n_meshes, rays_per_mesh, 3))
+    o = (lo_box[:, None, :] + u * (hi_box - lo_box)[:, None, :]).astype(np.float32).reshape(-1, 3)
+    rays = make_rays(o, unit_sphere(n_meshes * rays_per_mesh, rng))
+    return BatchedWorkload(v, counts.astype(np.int64), offs, rays, rays_per_mesh)
