@@ -127,8 +127,20 @@ struct Context {
     std::vector<void*> pinned_free;
     size_t pinned_free_bytes = 0;
     // Staging read by commands captured into a graph: every replay reads it
-    // again, so it lives until the context is destroyed.
+    // again, so it lives until the context is destroyed.  Captured commands
+    // take it from an arena reserved by the first uncaptured staging
+    // request, so that recording under capture makes no HIP call that could
+    // invalidate the capture; only when the arena is full is a block
+    // allocated (and kept in graph_staging).
+    static constexpr size_t kGraphArenaBytes = size_t(16) << 20;
     std::vector<void*> graph_staging;
+    uint8_t* graph_arena = nullptr;
+    size_t graph_arena_used = 0;
+
+    bool InArena(const void* p) const {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        return graph_arena && b >= graph_arena && b < graph_arena + kGraphArenaBytes;
+    }
 
     // Pinned host block of at least `bytes` (caller holds mu).
     void* AcquirePinned(size_t bytes) {
@@ -164,6 +176,8 @@ struct Context {
         pinned_free_bytes = 0;
         for (void* p : graph_staging) (void)hipHostFree(p);
         graph_staging.clear();
+        if (graph_arena) (void)hipHostFree(graph_arena);
+        graph_arena = nullptr;
     }
 
     void Reap(bool wait_all) {
@@ -220,6 +234,26 @@ bool IsCapturing(hipStream_t s) {
     return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
 }
 
+// Pinned staging for a command about to be recorded on `cs`.
+void* AcquireStaging(Context* c, CommandStream* cs, size_t bytes) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (cs->external && IsCapturing(cs->stream)) {
+        const size_t need = rr::AlignUp(bytes, 256);
+        if (c->graph_arena && c->graph_arena_used + need <= Context::kGraphArenaBytes) {
+            void* p = c->graph_arena + c->graph_arena_used;
+            c->graph_arena_used += need;
+            return p;
+        }
+        void* p = nullptr;
+        Check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+        c->graph_staging.push_back(p);
+        return p;
+    }
+    if (!c->graph_arena) Check(hipHostMalloc(reinterpret_cast<void**>(&c->graph_arena), Context::kGraphArenaBytes,
+                                             hipHostMallocDefault), "hipHostMalloc");
+    return c->AcquirePinned(bytes);
+}
+
 // Hand pinned staging read by a just-recorded command to its owner: freed
 // once the work has run (external stream), or with the command stream's
 // submission (internal stream); kept for the context's life when the stream
@@ -231,8 +265,9 @@ void KeepStaging(Context* c, CommandStream* cs, void* host) {
         return;
     }
     std::lock_guard<std::mutex> lk(c->mu);
-    if (IsCapturing(cs->stream)) c->graph_staging.push_back(host);
-    else c->FreeAfter(cs->stream, host);
+    // Captured staging is already owned by the arena or graph_staging.
+    if (c->InArena(host) || IsCapturing(cs->stream)) return;
+    c->FreeAfter(cs->stream, host);
 }
 
 template <typename F>
@@ -476,10 +511,7 @@ RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input,
         // recorded upload reads at execution time.
         rr::InstanceUpload* host = nullptr;
         if (count) {
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                host = static_cast<rr::InstanceUpload*>(c->AcquirePinned(sizeof(rr::InstanceUpload) * count));
-            }
+            host = static_cast<rr::InstanceUpload*>(AcquireStaging(c, cs, sizeof(rr::InstanceUpload) * count));
             for (uint32_t i = 0; i < count; ++i) {
                 std::memcpy(host[i].xform.m, build_input->instances[i].transform, sizeof(rr::Transform));
                 host[i].blas = reinterpret_cast<uint64_t>(Addr(build_input->instances[i].geometry));
@@ -887,10 +919,7 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
         void* sc = Addr(scratch);
         void* host = nullptr;
         const size_t table_bytes = sizeof(rr::BatchRequest) * n_req;
-        if (n_req) {
-            std::lock_guard<std::mutex> lk(c->mu);
-            host = c->AcquirePinned(table_bytes);
-        }
+        if (n_req) host = AcquireStaging(c, cs, table_bytes);
         if (host) std::memcpy(host, table.data(), table_bytes);
         auto cmd = [=](hipStream_t s) {
             if (n_req) {
@@ -982,10 +1011,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
         // upload reads at execution time.
         const uint32_t n_small = (uint32_t)plan.small.size();
         rr::BatchMesh* host = nullptr;
-        if (n_small) {
-            std::lock_guard<std::mutex> lk(c->mu);
-            host = static_cast<rr::BatchMesh*>(c->AcquirePinned(sizeof(rr::BatchMesh) * n_small));
-        }
+        if (n_small) host = static_cast<rr::BatchMesh*>(AcquireStaging(c, cs, sizeof(rr::BatchMesh) * n_small));
         for (uint32_t k = 0; k < n_small; ++k) {
             const rr::MeshDesc& m = plan.meshes[plan.small[k]];
             rr::BatchMesh b{};
