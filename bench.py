@@ -256,17 +256,27 @@ def batched_bench(ctx, dev, log, frames=16):
     def query():
         ctx.intersect_batch(requests, scratch=qscratch)
 
-    refit_ms = timed(refit, 3, warm=1)
-    query_ms = timed(query, 3, warm=1)
-    g = torch.cuda.CUDAGraph()
-    s = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    with torch.cuda.stream(s):
-        with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
-            with ctx.on_stream(s):
-                refit()
-                query()
-    torch.cuda.synchronize()
+    def graph_of(*fns):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
+                with ctx.on_stream(s):
+                    for fn in fns:
+                        fn()
+        torch.cuda.synchronize()
+        return g
+
+    def build():
+        ctx.build_geometry_batch(meshes, scratch=scratch, results=results)
+
+    # GPU times from graph replays (the eager calls' host-side setup of
+    # 8192-entry tables is not GPU work); the graphs replay the same calls.
+    build_gpu_ms = timed(graph_of(build).replay, 3, warm=1)
+    refit_ms = timed(graph_of(refit).replay, 3, warm=1)
+    query_ms = timed(graph_of(query).replay, 3, warm=1)
+    g = graph_of(refit, query)
     frame_ms = []
     for f in range(frames):
         torch.add(V, torch.randn(V.shape, generator=gen, device=dev), alpha=1e-3, out=cur)
@@ -280,7 +290,8 @@ def batched_bench(ctx, dev, log, frames=16):
     tris = int(w.tri_counts.sum())
     return {"workload": f"{M} soups, {tris} tris (log-uniform 64-16384), {rpm} any-hit rays per mesh per frame, "
                         f"{frames} refit+query frames replayed from a HIP graph",
-            "batch_build_ms": round(build_ms, 3), "batch_build_mtris_s": round(tris / build_ms / 1e3, 1),
+            "batch_build_ms": round(build_ms, 3), "batch_build_gpu_ms": round(build_gpu_ms, 3),
+            "batch_build_mtris_s": round(tris / build_gpu_ms / 1e3, 1),
             "per_call_build_ms_wall": round(per_call_ms, 1), "batch_equals_per_call": bool(same),
             "refit_ms": round(refit_ms, 3), "refit_gbs": round(tris * 304 / (refit_ms * 1e-3) / 1e9, 1),
             "query_ms": round(query_ms, 3), "query_mrays_s": round(R / query_ms / 1e3, 1),
@@ -297,6 +308,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream comparison")
     ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene traversal")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched build / graph refit workload")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the traversal autotuner sweep")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
@@ -388,6 +400,16 @@ def main():
                      "algorithmic_bytes": int(alg_bytes),
                      "visits_per_ray": round(visits / n_rays, 3)},
     }
+
+    if not args.no_autotune:
+        from radeonrays_sdk_amd import autotune
+
+        log("autotune sweep (persistent traversal knobs)")
+        start = autotune.current(ctx.lib)
+        t = autotune.tune_traversal(ctx, accel, rays, reps=3)
+        autotune.apply(ctx.lib, start)  # the other sections run with the defaults
+        out["autotune"] = {"default": t["default"], "best": t["best"], "configs": len(t["table"]),
+                           "note": "hits identical under every configuration"}
 
     if not args.no_extras:
         log("build / refit / sort timings")

@@ -46,6 +46,19 @@ RR_API RRError rrxCmdIntersectWithStats(RRContext        context,
                                         RRDevicePtr      stats,
                                         RRCommandStream  command_stream);
 
+/* Process-wide tuning of the persistent traversal kernel (the autotuner in
+ * radeonrays_sdk_amd/autotune.py sweeps these).  Results do not depend on
+ * them, only speed does. */
+typedef enum RRxTuning
+{
+    RRX_TUNING_TRACE_MIN_BUSY = 0,        /* refill a wave once fewer lanes traverse (1..64; default 24) */
+    RRX_TUNING_TRACE_RAYS_PER_FETCH = 1,  /* ray indices per queue atomic (32, 64, 128; default 128) */
+    RRX_TUNING_TRACE_LDS_STACK = 2        /* LDS stack entries per lane (8, 16, 32; default 8) */
+} RRxTuning;
+
+RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);
+RR_API RRError rrxGetTuning(RRxTuning key, uint32_t* value);
+
 /* Scratch bytes for rrxCmdBuildGeometryBatch over `count` build inputs
  * (each one triangle mesh, as for rrGetGeometryBuildMemoryRequirements).
  * Each geometry's result buffer is sized by rrGetGeometryBuildMemoryRequirements

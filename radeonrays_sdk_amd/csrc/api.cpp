@@ -1054,6 +1054,30 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
     });
 }
 
+RRError rrxSetTuning(RRxTuning key, uint32_t value) {
+    switch (key) {
+        case RRX_TUNING_TRACE_MIN_BUSY:
+            RR_REQUIRE(value >= 1 && value <= 64, "rrxSetTuning");
+            rr::SetMinBusyLanes(value);
+            return RR_SUCCESS;
+        case RRX_TUNING_TRACE_RAYS_PER_FETCH:
+            return rr::SetRaysPerFetch(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_TRACE_LDS_STACK:
+            return rr::SetLdsStackDepth(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+    }
+    return RR_ERROR_INVALID_PARAMETER;
+}
+
+RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
+    RR_REQUIRE(value, "rrxGetTuning");
+    switch (key) {
+        case RRX_TUNING_TRACE_MIN_BUSY: *value = rr::MinBusyLanes(); return RR_SUCCESS;
+        case RRX_TUNING_TRACE_RAYS_PER_FETCH: *value = rr::RaysPerFetch(); return RR_SUCCESS;
+        case RRX_TUNING_TRACE_LDS_STACK: *value = rr::LdsStackDepth(); return RR_SUCCESS;
+    }
+    return RR_ERROR_INVALID_PARAMETER;
+}
+
 RRError rrxWriteAnalyzerDump(RRContext context, RRDevicePtr geometry, RRDevicePtr rays, uint32_t width, uint32_t height,
                              const char* bvh_path, const char* rays_path, const char* config_path) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrxWriteAnalyzerDump");

@@ -8,11 +8,16 @@
 
 namespace rr {
 
-// Per-lane traversal stack: kLdsStackEntries in LDS, kSpillEntries more in
-// the trace scratch buffer (the reference keeps 16 in LDS and 64 per ray in
-// scratch, vlk/kernels/isect.comp:26-27, geometry_trace.cpp:169).
+// Per-lane traversal stack of kStackCapacity entries: a ring of
+// kLdsStackEntries (or the persistent kernel's tuned depth) in LDS, the rest
+// spilled to the lane's kSpillEntries slots of trace scratch (the reference
+// keeps 16 in LDS and 64 per ray in scratch, vlk/kernels/isect.comp:26-27,
+// geometry_trace.cpp:169).  The capacity is the same for every LDS depth, so
+// tuning never changes which pushes a too-deep tree drops.
+constexpr int kStackCapacity = 80;
 constexpr int kLdsStackEntries = 16;
-constexpr int kSpillEntries = 64;
+constexpr int kMinLdsStackEntries = 8;
+constexpr int kSpillEntries = kStackCapacity - kMinLdsStackEntries;
 
 size_t TraceScratchBytes(uint32_t ray_count);
 
@@ -25,9 +30,20 @@ void SetTraceMode(int mode);
 
 // Persistent kernel: a wave goes back to fetch rays for its idle lanes once
 // fewer than this many lanes are still traversing (RR_TRACE_MIN_BUSY).
-constexpr uint32_t kDefaultMinBusy = 32;
+constexpr uint32_t kDefaultMinBusy = 24;
 uint32_t MinBusyLanes();
 void SetMinBusyLanes(uint32_t lanes);
+
+// Tuning of the persistent kernel (process-wide): ray indices a wave takes
+// per queue atomic (32, 64 or 128) and LDS ring entries per lane (8, 16 or
+// 32).  Setters reject other values.  Defaults: the sweep's best on the
+// 1M-triangle heightfield with 4M incoherent rays.
+constexpr uint32_t kDefaultFetch = 128;
+constexpr uint32_t kDefaultLdsStack = 8;
+uint32_t RaysPerFetch();
+bool SetRaysPerFetch(uint32_t rays);
+uint32_t LdsStackDepth();
+bool SetLdsStackDepth(uint32_t entries);
 
 enum QueryKind : int { kQueryClosest = 0, kQueryAny = 1 };
 enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };

@@ -35,7 +35,6 @@ constexpr int kBlock = 256;
 constexpr uint32_t kQueues = 8;
 constexpr uint32_t kQueueStride = 32;
 constexpr uint32_t kQueueHeadWords = kQueues * kQueueStride;
-constexpr uint32_t kFetch = 64;
 
 struct RayIn {
     float4 a;  // origin.xyz, min_t
@@ -49,8 +48,11 @@ __device__ __forceinline__ RayIn LoadRay(const float4* __restrict__ rays, uint32
     return r;
 }
 
-// LDS ring + scratch spill stack for one lane.
+// LDS ring of DEPTH entries + scratch spill stack for one lane.
+template <int DEPTH = kLdsStackEntries>
 struct LaneStack {
+    static_assert((DEPTH & (DEPTH - 1)) == 0, "ring depth must be a power of two");
+    static_assert(DEPTH >= kMinLdsStackEntries && kStackCapacity - DEPTH <= kSpillEntries, "spill slots per lane");
     uint32_t (*lds)[kBlock];
     uint32_t* spill;
     int sp;     // entries on the stack
@@ -58,19 +60,19 @@ struct LaneStack {
 
     __device__ __forceinline__ bool empty() const { return sp == 0; }
     __device__ __forceinline__ void push(uint32_t v) {
-        if (sp == kLdsStackEntries + kSpillEntries) return;  // full: drop (tree deeper than the stack)
-        if (sp - gbase == kLdsStackEntries) {
-            spill[gbase] = lds[gbase & (kLdsStackEntries - 1)][threadIdx.x];
+        if (sp == kStackCapacity) return;  // full: drop (tree deeper than the stack)
+        if (sp - gbase == DEPTH) {
+            spill[gbase] = lds[gbase & (DEPTH - 1)][threadIdx.x];
             ++gbase;
         }
-        lds[sp & (kLdsStackEntries - 1)][threadIdx.x] = v;
+        lds[sp & (DEPTH - 1)][threadIdx.x] = v;
         ++sp;
     }
     __device__ __forceinline__ uint32_t pop() {
         --sp;
         // Read the LDS slot unconditionally (a plain ds_read) and replace it
         // from scratch only when the entry was spilled.
-        uint32_t v = lds[sp & (kLdsStackEntries - 1)][threadIdx.x];
+        uint32_t v = lds[sp & (DEPTH - 1)][threadIdx.x];
         if (sp < gbase) {
             gbase = sp;
             v = spill[sp];
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     float limit = WidenLimit(tmax);
     uint32_t n_internal_visits = 0, n_leaf_visits = 0, n_box_tests = 0, n_tri_tests = 0;
 
-    LaneStack st{s_stack, scratch + kQueueHeadWords + (size_t)i * kSpillEntries, 0, 0};
+    LaneStack<> st{s_stack, scratch + kQueueHeadWords + (size_t)i * kSpillEntries, 0, 0};
     const BvhNode* nodes = top_nodes;
     uint32_t node_count = top_count;  // of the level being traversed (bounds every node index)
     uint32_t inst = kInvalid;  // kInvalid while in the top level
@@ -249,13 +251,13 @@ __global__ void QueueInitKernel(uint32_t* counters, uint32_t ray_count, const ui
     if (k < kQueues) counters[k * kQueueStride] = min(k * part, n);
 }
 
-template <bool ANY, int OUT>
+template <bool ANY, int OUT, int DEPTH>
 __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* __restrict__ accel,
                                                                 const float4* __restrict__ rays, uint32_t ray_count,
                                                                 const uint32_t* __restrict__ indirect,
                                                                 void* __restrict__ hits, uint32_t* __restrict__ scratch,
-                                                                uint32_t min_busy) {
-    __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
+                                                                uint32_t min_busy, uint32_t fetch) {
+    __shared__ uint32_t s_stack[DEPTH][kBlock];
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
@@ -277,7 +279,8 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack st{s_stack, scratch + kQueueHeadWords + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries, 0, 0};
+    LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries,
+                        0, 0};
 
     for (;;) {
         // Refill idle lanes.
@@ -286,12 +289,12 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
             if (q_next == q_end) {
                 for (;;) {
                     uint32_t b = 0;
-                    if (lane == 0) b = atomicAdd(&counters[q * kQueueStride], kFetch);
+                    if (lane == 0) b = atomicAdd(&counters[q * kQueueStride], fetch);
                     b = __shfl(b, 0, 64);
                     const uint32_t end = min((q + 1) * part, n);
                     if (b < end) {
                         q_next = b;
-                        q_end = min(b + kFetch, end);
+                        q_end = min(b + fetch, end);
                         break;
                     }
                     if (++tries == kQueues) {
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
     float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack st{s_stack, scratch + spill_base + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries, 0, 0};
+    LaneStack<> st{s_stack, scratch + spill_base + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries, 0, 0};
 
     for (;;) {
         const uint64_t idle = __ballot(rid == kInvalid);
@@ -573,8 +576,12 @@ __global__ void BatchInitKernel(uint32_t* counter) {
 
 }  // namespace
 
+// Spill slots for every lane the launch can have: the persistent kernel runs
+// ceil(ray_count / kBlock) blocks at most and hands rays to any of their
+// lanes, so the slots are indexed by lane (up to ray_count rounded up to
+// kBlock), not by ray.
 size_t TraceScratchBytes(uint32_t ray_count) {
-    return AlignUp((size_t)kQueueHeadWords * 4 + (size_t)ray_count * kSpillEntries * 4, 256);
+    return AlignUp((size_t)kQueueHeadWords * 4 + AlignUp((size_t)ray_count, kBlock) * kSpillEntries * 4, 256);
 }
 
 static uint32_t g_min_busy = 0;
@@ -587,6 +594,21 @@ uint32_t MinBusyLanes() {
     return g_min_busy;
 }
 void SetMinBusyLanes(uint32_t v) { g_min_busy = (v >= 1 && v <= 64) ? v : kDefaultMinBusy; }
+
+static std::atomic<uint32_t> g_fetch{kDefaultFetch};
+static std::atomic<uint32_t> g_depth{kDefaultLdsStack};
+uint32_t RaysPerFetch() { return g_fetch.load(std::memory_order_relaxed); }
+bool SetRaysPerFetch(uint32_t v) {
+    if (v != 32 && v != 64 && v != 128) return false;
+    g_fetch.store(v, std::memory_order_relaxed);
+    return true;
+}
+uint32_t LdsStackDepth() { return g_depth.load(std::memory_order_relaxed); }
+bool SetLdsStackDepth(uint32_t v) {
+    if (v != 8 && v != 16 && v != 32) return false;
+    g_depth.store(v, std::memory_order_relaxed);
+    return true;
+}
 
 static int g_trace_mode = -1;
 int TraceMode() {
@@ -604,7 +626,7 @@ namespace {
 // Resident blocks of `kernel` on the current device (cached per device;
 // which = 0 for the single-request kernel, 1 for the batch kernel).
 uint32_t ResidentBlocks(const void* kernel, int which) {
-    static std::atomic<uint32_t> blocks[2][64];
+    static std::atomic<uint32_t> blocks[4][64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) dev = 0;
@@ -619,26 +641,43 @@ uint32_t ResidentBlocks(const void* kernel, int which) {
     return b;
 }
 
+// Resident blocks of the persistent kernel with the current LDS stack depth
+// (the occupancy depends on its LDS).
 uint32_t PersistentBlocks() {
-    return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit>), 0);
+    switch (LdsStackDepth()) {
+        case 8: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit, 8>), 2);
+        case 32: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit, 32>), 3);
+        default: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit, 16>), 0);
+    }
 }
 
 uint32_t BatchBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatchKernel), 1); }
+
+template <bool ANY, int DEPTH>
+void LaunchPersistentDepth(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r,
+                           uint32_t n, const uint32_t* ind, void* hits, uint32_t* sc) {
+    const uint32_t mb = MinBusyLanes(), f = RaysPerFetch();
+    switch (out) {
+        case kOutFullHit:
+            TracePersistentKernel<ANY, kOutFullHit, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
+            break;
+        case kOutInstanceId:
+            TracePersistentKernel<ANY, kOutInstanceId, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
+            break;
+        default:
+            TracePersistentKernel<ANY, kOutFullHitT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
+            break;
+    }
+}
 
 template <bool ANY>
 void LaunchPersistent(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
                       const uint32_t* ind, void* hits, uint32_t* sc) {
     QueueInitKernel<<<1, 64, 0, s>>>(sc, n, ind);
-    switch (out) {
-        case kOutFullHit:
-            TracePersistentKernel<ANY, kOutFullHit><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, MinBusyLanes());
-            break;
-        case kOutInstanceId:
-            TracePersistentKernel<ANY, kOutInstanceId><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, MinBusyLanes());
-            break;
-        default:
-            TracePersistentKernel<ANY, kOutFullHitT><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, MinBusyLanes());
-            break;
+    switch (LdsStackDepth()) {
+        case 8: LaunchPersistentDepth<ANY, 8>(out, blocks, s, a, r, n, ind, hits, sc); break;
+        case 32: LaunchPersistentDepth<ANY, 32>(out, blocks, s, a, r, n, ind, hits, sc); break;
+        default: LaunchPersistentDepth<ANY, 16>(out, blocks, s, a, r, n, ind, hits, sc); break;
     }
 }
 
@@ -652,7 +691,8 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
     uint32_t* sc = static_cast<uint32_t*>(scratch);
     const bool two = kind == kAccelTlas;
     if (!two && TraceMode() != kTraceModePlain) {
-        // Never more lanes than rays, so per-lane spill stacks fit the scratch.
+        // At most ceil(ray_count / kBlock) blocks: TraceScratchBytes holds
+        // spill slots for that many lanes.
         const uint32_t blocks = std::min(PersistentBlocks(), (ray_count + kBlock - 1) / kBlock);
         if (query == kQueryAny) LaunchPersistent<true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
         else LaunchPersistent<false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);

@@ -50,6 +50,10 @@ RR_INTERSECT_QUERY_OUTPUT_FULL_HIT = 0
 RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID = 1
 RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T = 2
 
+RRX_TUNING_TRACE_MIN_BUSY = 0
+RRX_TUNING_TRACE_RAYS_PER_FETCH = 1
+RRX_TUNING_TRACE_LDS_STACK = 2
+
 RRX_SERVE_BATCHED = 0
 RRX_SERVE_PLAIN = 1
 
@@ -151,6 +155,8 @@ _SIGNATURES = {
     "rrxGetSortMemoryRequirements": [c_void_p, c_uint32, POINTER(c_size_t)],
     "rrxCmdSortPairs": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p],
     "rrxCmdIntersectWithStats": [c_void_p, c_void_p, c_int, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p],
+    "rrxSetTuning": [c_int, c_uint32],
+    "rrxGetTuning": [c_int, POINTER(c_uint32)],
     "rrxGetGeometryBatchBuildMemoryRequirements": [c_void_p, c_void_p, c_uint32, POINTER(RRBuildOptions),
                                                    POINTER(c_size_t)],
     "rrxCmdBuildGeometryBatch": [c_void_p, c_int, c_void_p, c_uint32, POINTER(RRBuildOptions), c_void_p, c_void_p,

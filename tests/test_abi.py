@@ -113,6 +113,21 @@ def test_null_arguments_are_invalid_parameter():
     assert lib.rrxWriteAnalyzerDump(FAKE_CTX, c_void_p(1), None, 0, 0, None, b"b", b"c") == E
 
 
+def test_tuning_validation():
+    lib = rr.load()
+    from ctypes import c_uint32
+    v = c_uint32()
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, byref(v)) == rr.RR_SUCCESS and v.value == 128
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, byref(v)) == rr.RR_SUCCESS and v.value == 8
+    for key, bad in ((rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, 48), (rr.RRX_TUNING_TRACE_LDS_STACK, 12),
+                     (rr.RRX_TUNING_TRACE_MIN_BUSY, 0), (rr.RRX_TUNING_TRACE_MIN_BUSY, 65), (7, 1)):
+        assert lib.rrxSetTuning(key, bad) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, None) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 32) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, byref(v)) == rr.RR_SUCCESS and v.value == 32
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 8) == rr.RR_SUCCESS
+
+
 def test_aabb_list_is_not_implemented():
     # radeonrays.cpp:286-289.
     lib = rr.load()
