@@ -85,6 +85,19 @@ RR_API RRError rrxCmdBuildGeometryBatch(RRContext                   context,
                                         RRDevicePtr const*          geometry_buffers,
                                         RRCommandStream             command_stream);
 
+/* Refit a scene built by rrCmdBuildScene (radeonrays.h:357-362) to new
+ * instance transforms: build_input must name the same instance count and
+ * geometries as the build; their transforms may change.  The scene keeps its
+ * topology and gets the new transforms and refitted boxes, so hits are those
+ * of the moved instances.  temporary_buffer needs
+ * temporary_update_buffer_size bytes from rrGetSceneBuildMemoryRequirements.
+ * No counterpart in the reference, which rebuilds scenes only. */
+RR_API RRError rrxCmdUpdateScene(RRContext                context,
+                                 RRSceneBuildInput const* build_input,
+                                 RRDevicePtr              temporary_buffer,
+                                 RRDevicePtr              scene_buffer,
+                                 RRCommandStream          command_stream);
+
 /* Write a geometry (BLAS) in the dump format of the reference's bvh_analyzer:
  * the VkBvhNode array to bvh_path, width x height RRRay records from `rays`
  * (may be NULL: no rays file is written) to rays_path, and the analyzer's

@@ -128,7 +128,8 @@ class Context:
                  "rrGetGeometryBuildMemoryRequirements")
         return req
 
-    def build_geometry(self, vertices, indices, restructure: bool = False, scratch=None, result=None) -> Accel:
+    def build_geometry(self, vertices, indices, restructure: bool = False, scratch=None, result=None,
+                       extra_flags: int = 0) -> Accel:
         """LBVH (+ treelet restructuring) of a mesh: vertices float32 [V, >=3], indices int32/uint16 [T, 3]."""
         torch = _torch()
         assert vertices.is_cuda and indices.is_cuda and vertices.dtype == torch.float32
@@ -140,6 +141,7 @@ class Context:
         with self.ptrs(vertices.data_ptr(), indices.data_ptr(), scratch.data_ptr(), result.data_ptr()) as (v, i, s, r):
             prim, inp = self._mesh(vertices, indices, v, i)
             opts = self._options(restructure)
+            opts.build_flags |= extra_flags
             rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_BUILD, byref(inp), byref(opts), s, r,
                                                  self.cs), "rrCmdBuildGeometry")
         return Accel(result, "blas", tris)
@@ -151,6 +153,28 @@ class Context:
             prim, inp = self._mesh(vertices, indices, v, i)
             rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_UPDATE, byref(inp), None, None, r,
                                                  self.cs), "rrCmdBuildGeometry(update)")
+
+    def build_geometry_multi(self, meshes, restructure: bool = False, update: Accel = None) -> Accel:
+        """One BLAS over several meshes [(vertices, indices)] (one build input with
+        primitive_count = len(meshes)); with update=<accel>, refit it instead."""
+        meshes = [(v, i.contiguous()) for v, i in meshes]
+        n = len(meshes)
+        prims0, _ = self._batch_inputs(meshes, [None] * (2 * n))
+        inp0 = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, n, c_void_p(ctypes.addressof(prims0)))
+        req = rr.RRMemoryRequirements()
+        opts = self._options(restructure)
+        rr.check(self.lib.rrGetGeometryBuildMemoryRequirements(self.ctx, byref(inp0), byref(opts), byref(req)),
+                 "rrGetGeometryBuildMemoryRequirements")
+        result = update.buffer if update is not None else self.empty(req.result_buffer_size)
+        scratch = self.empty(req.temporary_update_buffer_size if update is not None else req.temporary_build_buffer_size)
+        addrs = [a for v, i in meshes for a in (v.data_ptr(), i.data_ptr())]
+        with self.ptrs(*addrs, scratch.data_ptr(), result.data_ptr()) as hs:
+            prims, _ = self._batch_inputs(meshes, hs[:2 * n])
+            inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, n, c_void_p(ctypes.addressof(prims)))
+            op = rr.RR_BUILD_OPERATION_UPDATE if update is not None else rr.RR_BUILD_OPERATION_BUILD
+            rr.check(self.lib.rrCmdBuildGeometry(self.ctx, op, byref(inp), byref(opts), hs[-2], hs[-1], self.cs),
+                     "rrCmdBuildGeometry(multi)")
+        return Accel(result, "blas", sum(int(i.shape[0]) for _, i in meshes))
 
     def _batch_inputs(self, meshes, handles):
         """ctypes build inputs for [(vertices, indices)] with their RRDevicePtr handles."""
@@ -204,13 +228,14 @@ class Context:
                  "rrGetSceneBuildMemoryRequirements")
         return req
 
-    def build_scene(self, blases, transforms) -> Accel:
-        """TLAS over instances: blases[i] (Accel) placed by transforms[i] (3x4 row-major)."""
+    def build_scene(self, blases, transforms, update: Accel = None) -> Accel:
+        """TLAS over instances: blases[i] (Accel) placed by transforms[i] (3x4 row-major).
+        With update=<scene>, refit that scene to the new transforms instead (rrxCmdUpdateScene)."""
         transforms = np.asarray(transforms, dtype=np.float32).reshape(len(blases), 3, 4)
         n = len(blases)
         req = self.scene_requirements(n)
-        result = self.empty(req.result_buffer_size)
-        scratch = self.empty(req.temporary_build_buffer_size)
+        result = update.buffer if update is not None else self.empty(req.result_buffer_size)
+        scratch = self.empty(req.temporary_update_buffer_size if update is not None else req.temporary_build_buffer_size)
         with self.ptrs(*[b.address for b in blases], scratch.data_ptr(), result.data_ptr()) as hs:
             geo, s, r = hs[:n], hs[n], hs[n + 1]
             arr = (rr.RRInstance * max(n, 1))()
@@ -220,7 +245,10 @@ class Context:
                     for b in range(4):
                         arr[k].transform[a][b] = float(transforms[k, a, b])
             inp = rr.RRSceneBuildInput(arr, n)
-            rr.check(self.lib.rrCmdBuildScene(self.ctx, byref(inp), None, s, r, self.cs), "rrCmdBuildScene")
+            if update is not None:
+                rr.check(self.lib.rrxCmdUpdateScene(self.ctx, byref(inp), s, r, self.cs), "rrxCmdUpdateScene")
+            else:
+                rr.check(self.lib.rrCmdBuildScene(self.ctx, byref(inp), None, s, r, self.cs), "rrCmdBuildScene")
         return Accel(result, "tlas", n, tuple(blases))
 
     # -- queries --------------------------------------------------------------

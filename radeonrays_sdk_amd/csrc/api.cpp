@@ -333,6 +333,40 @@ long WaitSpinMicros() {
     return us;
 }
 
+// Several triangle meshes in one build input (primitive_count > 1; the
+// reference asserts a single mesh, intersector.cpp:110,146).
+bool IsMultiMesh(const RRGeometryBuildInput* in) {
+    return in->primitive_type == RR_PRIMITIVE_TYPE_TRIANGLE_MESH && in->primitive_count > 1;
+}
+
+RRError SubMeshesFromInput(const RRGeometryBuildInput* in, std::vector<rr::SubMesh>& out, uint32_t& tris,
+                           const char* name, bool resolve) {
+    RR_REQUIRE(in->triangle_mesh_primitives, name);
+    out.clear();
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < in->primitive_count; ++k) {
+        RRGeometryBuildInput one = *in;
+        one.primitive_count = 1;
+        one.triangle_mesh_primitives = in->triangle_mesh_primitives + k;
+        rr::MeshDesc m;
+        const RRError e = MeshFromInput(&one, m, name, resolve);
+        if (e != RR_SUCCESS) return e;
+        rr::SubMesh sm{};
+        sm.vertices = reinterpret_cast<uint64_t>(m.vertices);
+        sm.indices = reinterpret_cast<uint64_t>(m.indices);
+        sm.vertex_count = m.vertex_count;
+        sm.vertex_stride = m.vertex_stride;
+        sm.index16 = m.index16;
+        sm.tri_count = m.tri_count;
+        sm.first_tri = (uint32_t)total;
+        total += m.tri_count;
+        RR_REQUIRE(total < (1u << 30), name);
+        out.push_back(sm);
+    }
+    tris = (uint32_t)total;
+    return RR_SUCCESS;
+}
+
 bool WantsRestructure(const RRBuildOptions* o) {
     // intersector.cpp:116,170: restructure only with options that do not ask for a fast build.
     return o && (o->build_flags & RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD) == 0;
@@ -429,6 +463,18 @@ RRError rrGetGeometryBuildMemoryRequirements(RRContext context, const RRGeometry
                                              RRMemoryRequirements* memory_requirements) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrGetGeometryBuildMemoryRequirements");
     RR_REQUIRE(context && build_input && memory_requirements, "rrGetGeometryBuildMemoryRequirements");
+    if (IsMultiMesh(build_input)) {
+        std::vector<rr::SubMesh> subs;
+        uint32_t tris = 0;
+        const RRError e = SubMeshesFromInput(build_input, subs, tris, "rrGetGeometryBuildMemoryRequirements", false);
+        if (e != RR_SUCCESS) return e;
+        memory_requirements->result_buffer_size = rr::BlasResultSize(tris);
+        memory_requirements->temporary_build_buffer_size =
+            rr::MultiMeshScratchBytes((uint32_t)subs.size(), tris, true, WantsRestructure(build_options));
+        // Refitting several meshes gathers them again, into scratch.
+        memory_requirements->temporary_update_buffer_size = rr::MultiMeshScratchBytes((uint32_t)subs.size(), tris, false, false);
+        return RR_SUCCESS;
+    }
     rr::MeshDesc m;
     const RRError e = MeshFromInput(build_input, m, "rrGetGeometryBuildMemoryRequirements", false);
     if (e != RR_SUCCESS) return e;
@@ -467,6 +513,38 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
     RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildGeometry");
     RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
                "rrCmdBuildGeometry");
+    if (IsMultiMesh(build_input)) {
+        std::vector<rr::SubMesh> subs;
+        uint32_t tris = 0;
+        const RRError e = SubMeshesFromInput(build_input, subs, tris, "rrCmdBuildGeometry", true);
+        if (e != RR_SUCCESS) return e;
+        RR_REQUIRE(geometry_buffer && temporary_buffer, "rrCmdBuildGeometry");
+        return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
+            Context* c = AsCtx(context);
+            CommandStream* cs = AsStream(command_stream);
+            DeviceGuard g(c->device);
+            void* result = Addr(geometry_buffer);
+            void* scratch = Addr(temporary_buffer);
+            const bool restructure = WantsRestructure(build_options);
+            const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelBlas;
+            }
+            const uint32_t count = (uint32_t)subs.size();
+            const size_t bytes = sizeof(rr::SubMesh) * count;
+            void* host = AcquireStaging(c, cs, bytes);
+            std::memcpy(host, subs.data(), bytes);
+            Record(cs, [=](hipStream_t s) {
+                auto* table = reinterpret_cast<rr::SubMesh*>(static_cast<uint8_t*>(scratch) + rr::MultiMeshTableOffset(tris));
+                Check(hipMemcpyAsync(table, host, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+                if (update) rr::UpdateMultiMesh(table, count, tris, scratch, result, s);
+                else rr::BuildMultiMesh(table, count, tris, restructure, scratch, result, s);
+            });
+            KeepStaging(c, cs, host);
+            return RR_SUCCESS;
+        });
+    }
     rr::MeshDesc m;
     const RRError e = MeshFromInput(build_input, m, "rrCmdBuildGeometry", true);
     if (e != RR_SUCCESS) return e;
@@ -490,9 +568,9 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
     });
 }
 
-RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input, const RRBuildOptions*,
-                        RRDevicePtr temporary_buffer, RRDevicePtr scene_buffer, RRCommandStream command_stream) {
-    RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdBuildScene");
+// Build (or refit) a scene over the instances, as rrCmdBuildScene.
+static RRError SceneCommand(bool update, RRContext context, const RRSceneBuildInput* build_input,
+                            RRDevicePtr temporary_buffer, RRDevicePtr scene_buffer, RRCommandStream command_stream) {
     RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildScene");
     RR_REQUIRE(scene_buffer && temporary_buffer, "rrCmdBuildScene");
     RR_REQUIRE(build_input->instance_count == 0 || build_input->instances, "rrCmdBuildScene");
@@ -527,12 +605,25 @@ RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input,
             if (count)
                 Check(hipMemcpyAsync(dev_inst, host, sizeof(rr::InstanceUpload) * count, hipMemcpyHostToDevice, s),
                       "hipMemcpyAsync");
-            rr::BuildTlas(dev_inst, count, scratch, result, s);
+            if (update) rr::UpdateTlas(dev_inst, count, scratch, result, s);
+            else rr::BuildTlas(dev_inst, count, scratch, result, s);
         };
         Record(cs, cmd);
         KeepStaging(c, cs, host);
         return RR_SUCCESS;
     });
+}
+
+RRError rrCmdBuildScene(RRContext context, const RRSceneBuildInput* build_input, const RRBuildOptions*,
+                        RRDevicePtr temporary_buffer, RRDevicePtr scene_buffer, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdBuildScene");
+    return SceneCommand(false, context, build_input, temporary_buffer, scene_buffer, command_stream);
+}
+
+RRError rrxCmdUpdateScene(RRContext context, const RRSceneBuildInput* build_input, RRDevicePtr temporary_buffer,
+                          RRDevicePtr scene_buffer, RRCommandStream command_stream) {
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrxCmdUpdateScene");
+    return SceneCommand(true, context, build_input, temporary_buffer, scene_buffer, command_stream);
 }
 
 static RRError KindOf(Context* c, void* accel, rr::AccelKind& kind) {

@@ -39,9 +39,39 @@ void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* resu
 // Refit of a BLAS built from a mesh with the same triangles and indices.
 void UpdateBlas(const MeshDesc& mesh, void* result, hipStream_t stream);
 
+// Several meshes built as one BLAS (RRGeometryBuildInput with
+// primitive_count > 1): their triangles are numbered consecutively, mesh 0
+// first, and gathered into an unindexed triangle list in scratch, which is
+// then built (or refitted) as one mesh.
+struct SubMesh {
+    uint64_t vertices;
+    uint64_t indices;
+    uint32_t vertex_count;
+    uint32_t vertex_stride;
+    uint32_t index16;
+    uint32_t tri_count;
+    uint32_t first_tri;
+    uint32_t pad[3];
+};
+static_assert(sizeof(SubMesh) == 48, "SubMesh must be 48 bytes");
+// Scratch: gathered vertices (36 bytes per triangle), the sub-mesh table,
+// then the single-mesh build scratch (none for an update).
+size_t MultiMeshScratchBytes(uint32_t meshes, uint32_t tris, bool build, bool restructure);
+size_t MultiMeshTableOffset(uint32_t tris);
+void BuildMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, bool restructure, void* scratch,
+                    void* result, hipStream_t stream);
+void UpdateMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, void* scratch, void* result,
+                     hipStream_t stream);
+
 // TLAS build over `count` instances whose descriptions are already in
 // device memory at `instances` (typically the head of `scratch`).
 void BuildTlas(const InstanceUpload* instances, uint32_t count, void* scratch, void* result, hipStream_t stream);
+
+// Refit of a TLAS to new instance transforms (same instance count and
+// BLASes): instance boxes, transforms and the BLAS table are rewritten and
+// the node boxes refitted; the topology is kept.
+size_t TlasUpdateScratchBytes(uint32_t instances);
+void UpdateTlas(const InstanceUpload* instances, uint32_t count, void* scratch, void* result, hipStream_t stream);
 
 // Byte offset inside the TLAS scratch where the instance upload goes.
 inline size_t TlasUploadOffset() { return 0; }

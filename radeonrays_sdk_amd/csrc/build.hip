@@ -589,6 +589,27 @@ __global__ __launch_bounds__(kBlock) void RefitSceneKernel(const float* __restri
 
 }  // namespace
 
+// Gather the triangles of several meshes into one unindexed list: thread t
+// finds its mesh by binary search over first_tri.
+__global__ __launch_bounds__(kBlock) void GatherSubMeshesKernel(const SubMesh* __restrict__ table, uint32_t meshes,
+                                                                uint32_t tris, float* __restrict__ out) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= tris) return;
+    uint32_t lo = 0, hi = meshes;  // last mesh with first_tri <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (table[mid].first_tri <= t) lo = mid;
+        else hi = mid;
+    }
+    const SubMesh& sm = table[lo];
+    const MeshView m{reinterpret_cast<const uint8_t*>(sm.vertices), reinterpret_cast<const void*>(sm.indices),
+                     sm.vertex_stride, sm.index16, sm.vertex_count, sm.tri_count};
+    float v[9];
+    LoadTriangle(m, t - sm.first_tri, v);
+    float* o = out + 9 * (size_t)t;
+    for (int k = 0; k < 9; ++k) o[k] = v[k];
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 
@@ -626,6 +647,37 @@ void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* resu
     if (do_restructure) RestructureBlas(n, scratch, result, s);
 }
 
+size_t MultiMeshTableOffset(uint32_t tris) { return AlignUp(size_t(tris) * 36, 256); }
+
+size_t MultiMeshScratchBytes(uint32_t meshes, uint32_t tris, bool build, bool restructure) {
+    const size_t head = MultiMeshTableOffset(tris) + AlignUp(size_t(meshes) * sizeof(SubMesh), 256);
+    return head + (build ? BlasBuildScratchBytes(tris, restructure) : 0);
+}
+
+static MeshDesc GatheredMesh(void* scratch, uint32_t tris) {
+    MeshDesc m;
+    m.vertices = scratch;
+    m.vertex_count = 3 * tris;
+    m.vertex_stride = 12;
+    m.indices = nullptr;
+    m.tri_count = tris;
+    return m;
+}
+
+void BuildMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, bool restructure, void* scratch,
+                    void* result, hipStream_t s) {
+    if (tris) GatherSubMeshesKernel<<<Blocks(tris), kBlock, 0, s>>>(table, meshes, tris, static_cast<float*>(scratch));
+    void* rest = static_cast<uint8_t*>(scratch) + MultiMeshScratchBytes(meshes, tris, false, false);
+    BuildBlas(GatheredMesh(scratch, tris), restructure, rest, result, s);
+}
+
+void UpdateMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, void* scratch, void* result,
+                     hipStream_t s) {
+    if (!tris) return;
+    GatherSubMeshesKernel<<<Blocks(tris), kBlock, 0, s>>>(table, meshes, tris, static_cast<float*>(scratch));
+    UpdateBlas(GatheredMesh(scratch, tris), result, s);
+}
+
 void UpdateBlas(const MeshDesc& mesh, void* result, hipStream_t s) {
     const uint32_t n = mesh.tri_count;
     if (n == 0) return;
@@ -649,6 +701,21 @@ void RestructureBlas(uint32_t n, void* scratch, void* result, hipStream_t s) {
 }
 
 size_t TlasBuildScratchBytes(uint32_t instances) { return LayoutTlas(instances).total; }
+
+size_t TlasUpdateScratchBytes(uint32_t instances) { return LayoutTlas(instances).keys; }
+
+void UpdateTlas(const InstanceUpload* instances, uint32_t n, void* scratch, void* result, hipStream_t s) {
+    if (n == 0) return;
+    const uint64_t nodes_off = sizeof(AccelHeader);
+    const uint64_t xf_off = nodes_off + TlasNodesBytes(n);
+    const uint64_t table_off = xf_off + TlasTransformsBytes(n);
+    const TlasScratchLayout L = LayoutTlas(n);
+    // The scene bounds the instance kernel accumulates are not used by a refit.
+    float* boxes = At<float>(scratch, L.boxes);
+    TlasInstanceKernel<<<Blocks(n), kBlock, 0, s>>>(instances, n, At<Transform>(result, xf_off),
+                                                    At<uint64_t>(result, table_off), boxes, At<BuildHead>(scratch, L.head));
+    RefitSceneKernel<<<Blocks(n), kBlock, 0, s>>>(boxes, n, At<BvhNode>(result, nodes_off));
+}
 
 void BuildTlas(const InstanceUpload* instances, uint32_t n, void* scratch, void* result, hipStream_t s) {
     AccelHeader* hdr = static_cast<AccelHeader*>(result);

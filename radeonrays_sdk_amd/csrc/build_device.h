@@ -26,7 +26,9 @@ inline MeshView ViewOf(const MeshDesc& m) {
 // vertex so that a bad index buffer cannot read out of bounds.
 __device__ __forceinline__ void LoadTriangle(const MeshView& m, uint32_t t, float v[9]) {
     uint32_t i0, i1, i2;
-    if (m.index16) {
+    if (!m.indices) {  // unindexed: triangle t is vertices 3t, 3t+1, 3t+2
+        i0 = 3 * t; i1 = 3 * t + 1; i2 = 3 * t + 2;
+    } else if (m.index16) {
         const uint16_t* ix = static_cast<const uint16_t*>(m.indices) + 3 * (size_t)t;
         i0 = ix[0]; i1 = ix[1]; i2 = ix[2];
     } else {

@@ -174,8 +174,17 @@ def test_memory_requirements(tris):
     assert size.value >= 1000 * 256
 
 
-def test_multiple_meshes_per_build_rejected():
+def test_multiple_meshes_per_build_input():
+    # Two 10-triangle meshes build one 20-triangle BLAS (the reference asserts
+    # a single mesh, intersector.cpp:110,146); refit needs scratch for them.
     lib = rr.load()
-    prim, inp = _mesh_input(10, count=2)
-    req = rr.RRMemoryRequirements()
-    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, byref(req)) == rr.RR_ERROR_INVALID_PARAMETER
+    prims = (rr.RRTriangleMeshPrimitive * 2)(*[rr.RRTriangleMeshPrimitive(None, 30, 12, None, 10, rr.RR_INDEX_TYPE_UINT32)
+                                               for _ in range(2)])
+    inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_TRIANGLE_MESH, 2, ctypes.cast(prims, c_void_p))
+    req, one = rr.RRMemoryRequirements(), rr.RRMemoryRequirements()
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, byref(req)) == rr.RR_SUCCESS
+    p20, i20 = _mesh_input(20)
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(i20), None, byref(one)) == rr.RR_SUCCESS
+    assert req.result_buffer_size == one.result_buffer_size
+    assert req.temporary_build_buffer_size > one.temporary_build_buffer_size
+    assert req.temporary_update_buffer_size >= 20 * 36

@@ -1,0 +1,118 @@
+"""GPU tests of the parts of the API the reference declares but does not
+implement: uint16 indices, RR_BUILD_FLAG_BITS_ALLOW_UPDATE, several meshes
+per build input, and scene (TLAS) refit (rrxCmdUpdateScene)."""
+import numpy as np
+import pytest
+
+import oracle
+from radeonrays_sdk_amd import api, rr, scenes
+from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _soup_of(meshes):
+    """Triangles of several indexed meshes as one unindexed soup (mesh 0 first)."""
+    v = np.concatenate([vv[ii.astype(np.int64)].reshape(-1, 3) for vv, ii in meshes]).astype(np.float32)
+    return v, np.arange(v.shape[0], dtype=np.int32).reshape(-1, 3)
+
+
+def test_uint16_indices_equal_uint32(ctx, dev):
+    import torch
+
+    say("uint16 vs uint32 indices, build and refit")
+    v, i = scenes.heightfield_mesh(150)  # 22,500 vertices
+    a16 = ctx.build_geometry(to_dev(v, dev), to_dev(i.astype(np.uint16).view(np.int16), dev), restructure=True)
+    a32 = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True)
+    torch.cuda.synchronize()
+    ok, msg = nodes_equal(a16.buffer.cpu().numpy(), a32.buffer.cpu().numpy())
+    assert ok, msg
+    v2 = scenes.perturbed_heights(v, 3)
+    ctx.update_geometry(a16, to_dev(v2, dev), to_dev(i.astype(np.uint16).view(np.int16), dev))
+    torch.cuda.synchronize()
+    ok, msg = nodes_equal(a16.buffer.cpu().numpy(), oracle.refit(a32.buffer.cpu().numpy(), v2, i))
+    assert ok, msg
+
+
+def test_allow_update_flag_is_accepted(ctx, dev):
+    import torch
+
+    say("RR_BUILD_FLAG_BITS_ALLOW_UPDATE")
+    v, i = scenes.triangle_soup(5000, scenes.rng_for(50))
+    a = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), extra_flags=rr.RR_BUILD_FLAG_BITS_ALLOW_UPDATE)
+    b = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev))
+    torch.cuda.synchronize()
+    ok, msg = nodes_equal(a.buffer.cpu().numpy(), b.buffer.cpu().numpy())
+    assert ok, msg
+
+
+@pytest.mark.parametrize("restructure", [False, True])
+def test_several_meshes_per_build_input(ctx, dev, restructure):
+    import torch
+
+    say(f"three meshes in one build input, restructure={restructure}")
+    m0 = scenes.triangle_soup(3000, scenes.rng_for(51))
+    hv, hi = scenes.heightfield_mesh(40)
+    m1 = (hv, hi.astype(np.uint16))
+    m2 = scenes.triangle_soup(777, scenes.rng_for(52))
+    meshes = [m0, m1, m2]
+    dm = [(to_dev(v, dev), to_dev(i.view(np.int16) if i.dtype == np.uint16 else i, dev)) for v, i in meshes]
+    accel = ctx.build_geometry_multi(dm, restructure=restructure)
+    torch.cuda.synchronize()
+    buf = accel.buffer.cpu().numpy()
+    sv, si = _soup_of(meshes)
+    ok, msg = nodes_equal(buf, oracle.build_blas(sv, si, restructure))
+    assert ok, msg
+    # Primitive ids run over the meshes in order: hits equal brute force over the soup.
+    rays = scenes.camera_rays(64, 64, (0.5, 0.5, -1.5))
+    hits = ctx.intersect(accel, to_dev(rays, dev), "closest", "full_t")
+    torch.cuda.synchronize()
+    ok, msg = hits_equal(api.decode_hits(hits, "full_t"), oracle.brute_force(buf, rays))
+    assert ok, msg
+    # Refit of the moved meshes.
+    rng = scenes.rng_for(53)
+    moved = [(v + rng.normal(0, 1e-3, v.shape).astype(np.float32), i) for v, i in meshes]
+    dmoved = [(to_dev(v, dev), d[1]) for (v, _), d in zip(moved, dm)]
+    ctx.build_geometry_multi(dmoved, update=accel)
+    torch.cuda.synchronize()
+    mv, mi = _soup_of(moved)
+    ok, msg = nodes_equal(accel.buffer.cpu().numpy(), oracle.refit(buf, mv, mi))
+    assert ok, msg
+
+
+def test_scene_update_follows_moved_instances(ctx, dev):
+    import torch
+
+    say("rrxCmdUpdateScene: 37 instances moved")
+    rng = scenes.rng_for(54)
+    blases, host = [], {}
+    for k in range(4):
+        v, i = scenes.triangle_soup(int(rng.integers(200, 3000)), rng, sigma=0.08)
+        b = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True)
+        blases.append(b)
+    pick = rng.integers(0, 4, size=37)
+    xf0 = scenes.random_transforms(rng, 37, trans=5.0)
+    tlas = ctx.build_scene([blases[k] for k in pick], xf0)
+    torch.cuda.synchronize()
+    host = {b.address: b.buffer.cpu().numpy() for b in blases}
+    xf1 = xf0.copy()
+    xf1[:, :, 3] += rng.uniform(-1, 1, size=(37, 3)).astype(np.float32)
+    ctx.build_scene([blases[k] for k in pick], xf1, update=tlas)
+    fresh = ctx.build_scene([blases[k] for k in pick], xf1)
+    torch.cuda.synchronize()
+    buf, fb = tlas.buffer.cpu().numpy(), fresh.buffer.cpu().numpy()
+    assert oracle.is_valid(buf) and oracle.check_consistency(buf)
+    # Transforms (inverse, forward per instance) and the BLAS table as a fresh build's.
+    xo, to = int(buf[24:32].view(np.uint64)[0]), int(buf[32:40].view(np.uint64)[0])
+    assert np.array_equal(buf[xo:xo + 37 * 96], fb[xo:xo + 37 * 96])
+    assert np.array_equal(buf[to:to + 37 * 8], fb[to:to + 37 * 8])
+    o = rng.uniform(-8, 8, size=(20000, 3)).astype(np.float32)
+    rays = scenes.make_rays(o, scenes.unit_sphere(20000, rng))
+    hits = ctx.intersect(tlas, to_dev(rays, dev), "closest", "full_t")
+    torch.cuda.synchronize()
+    g = api.decode_hits(hits, "full_t")
+    ok, msg = hits_equal(g, oracle.trace(buf, rays, blas_map=host))
+    assert ok, msg
+    ok, msg = hits_equal({k: a[:3000] for k, a in g.items()}, oracle.brute_force(fb, rays[:3000], blas_map=host))
+    assert ok, "brute force on the fresh scene: " + msg
+    assert (g["inst_id"] != RR_INVALID).sum() > 50
