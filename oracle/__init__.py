@@ -125,6 +125,19 @@ def build_blas(v, i, restructure=False) -> np.ndarray:
     return buf
 
 
+def build_blas_aabbs(boxes, restructure=False) -> np.ndarray:
+    """BLAS over an AABB list [n, 6] (min xyz, max xyz): the LBVH of the
+    unindexed triangles (min, max, min), whose bounds, centroids and leaf
+    boxes are the boxes', flagged as an AABB BLAS (header flag 2)."""
+    b = _c(boxes, np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    v = np.stack([b[:, 0:3], b[:, 3:6], b[:, 0:3]], axis=1).reshape(-1, 3)
+    buf = build_blas(v, np.arange(3 * n, dtype=np.uint32).reshape(-1, 3), restructure)
+    hdr = buf[:128].view(np.uint32)
+    hdr[10] |= 2
+    return buf
+
+
 def restructure(buf: np.ndarray, tris: int) -> np.ndarray:
     out = buf.copy()
     load().orc_restructure(tris, _p(out[128:]))

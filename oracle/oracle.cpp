@@ -60,6 +60,7 @@ static_assert(sizeof(Header) == 128, "header size");
 
 constexpr uint32_t kMagic = 0x53415252u;
 constexpr uint32_t kBlas = 1, kTlas = 2;
+constexpr uint32_t kFlagAabbs = 2;  // leaves are boxes (AABB list)
 
 size_t Align(size_t v, size_t a) { return (v + a - 1) / a * a; }
 uint32_t NodeCount(uint32_t n) { return n ? 2 * n - 1 : 0; }
@@ -491,6 +492,24 @@ bool Slab(const RayState& r, float limit, const float lo[3], const float hi[3], 
 
 float Widen(float t) { return std::fma(kLimitWiden, std::fabs(t), t); }
 
+// Leaf of an AABB-list BLAS: entry distance into the box (exact slab, no
+// widening), t = max(entry, tmin), a hit when t <= min(exit, tmax); the
+// leaf stores the box's min corner in a0min and its max corner in a0max.
+bool BoxLeaf(const RayState& r, float tmax, const float lo[3], const float hi[3], float& t) {
+    float n3[3], f3[3];
+    for (int a = 0; a < 3; ++a) {
+        const float l = (lo[a] - r.o[a]) * r.id[a];
+        const float h = (hi[a] - r.o[a]) * r.id[a];
+        const bool neg = r.id[a] < 0.0f;
+        n3[a] = neg ? h : l;
+        f3[a] = neg ? l : h;
+    }
+    const float a = Max(Max(Max(n3[0], n3[1]), n3[2]), r.tmin);
+    const float b = Min(Min(Min(f3[0], f3[1]), f3[2]), tmax);
+    t = a;
+    return a <= b;
+}
+
 struct Tri3 {
     float V, W, det, t;
 };
@@ -525,6 +544,17 @@ bool Watertight(const RayState& r, const float v0[3], const float v1[3], const f
     h.det = det;
     h.t = T / det;
     return true;
+}
+
+// Leaf test by the BLAS's primitive kind.
+bool Leaf(bool boxes, const RayState& r, float tmax, const Node& l, Tri3& h) {
+    if (boxes) {
+        h.V = 0.0f;
+        h.W = 0.0f;
+        h.det = 1.0f;
+        return BoxLeaf(r, tmax, l.a0min, l.a0max, h.t);
+    }
+    return Watertight(r, l.a0min, l.a0max, l.a1min, h);
 }
 
 bool Closer(float t, uint32_t inst, uint32_t prim, float bt, uint32_t binst, uint32_t bprim) {
@@ -577,6 +607,7 @@ OrcHit TraceOne(const void* accel, const void* const* blas_host, const float* ra
     stack.reserve(128);
     const Node* nodes = top;
     uint32_t inst = kInv, node = 0;
+    bool boxes = !two && (h->flags & kFlagAabbs) != 0;
     bool alive = true;
     while (alive) {
         const Node& nd = nodes[node];
@@ -602,6 +633,7 @@ OrcHit TraceOne(const void* accel, const void* const* blas_host, const float* ra
                 Setup(r, oo, ray[3], dd);
                 inst = id;
                 nodes = NodesOf(blas);
+                boxes = (Hdr(blas)->flags & kFlagAabbs) != 0;
                 stack.push_back(kSentinel);
                 node = 0;
                 continue;
@@ -610,7 +642,7 @@ OrcHit TraceOne(const void* accel, const void* const* blas_host, const float* ra
             s.leaves++;
             s.tri_tests++;
             Tri3 th;
-            if (Watertight(r, nd.a0min, nd.a0max, nd.a1min, th) && th.t >= r.tmin && th.t <= best_t) {
+            if (Leaf(boxes, r, best_t, nd, th) && th.t >= r.tmin && th.t <= best_t) {
                 const uint32_t hi = two ? inst : 0u;
                 if (any) {
                     best_t = th.t; best_prim = nd.c1; best_inst = hi;
@@ -631,6 +663,7 @@ OrcHit TraceOne(const void* accel, const void* const* blas_host, const float* ra
             if (two && node == kSentinel) {
                 inst = kInv;
                 nodes = top;
+                boxes = false;
                 Setup(r, ray, ray[3], ray + 4);
                 continue;
             }
@@ -656,10 +689,11 @@ void BruteBlas(const void* blas, const RayState& r, uint32_t inst, float& best_t
     const Node* nodes = NodesOf(blas);
     const uint32_t n = h->prim_count;
     const uint32_t first = n - 1;
+    const bool boxes = (h->flags & kFlagAabbs) != 0;
     for (uint32_t i = 0; i < n; ++i) {
         const Node& l = nodes[first + i];
         Tri3 th;
-        if (Watertight(r, l.a0min, l.a0max, l.a1min, th) && th.t >= r.tmin && th.t <= best_t &&
+        if (Leaf(boxes, r, best_t, l, th) && th.t >= r.tmin && th.t <= best_t &&
             Closer(th.t, inst, l.c1, best_t, best_inst, best_prim)) {
             best_t = th.t; best_inst = inst; best_prim = l.c1;
             bV = th.V; bW = th.W; bdet = th.det;

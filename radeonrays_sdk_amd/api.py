@@ -154,6 +154,27 @@ class Context:
             rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_UPDATE, byref(inp), None, None, r,
                                                  self.cs), "rrCmdBuildGeometry(update)")
 
+    def build_aabbs(self, boxes, restructure: bool = False, update: Accel = None) -> Accel:
+        """BLAS over an AABB list: boxes float32 [n, 6] (min xyz, max xyz) on the
+        device (RR_PRIMITIVE_TYPE_AABB_LIST); with update=<accel>, refit it."""
+        boxes = boxes.contiguous()
+        n = int(boxes.shape[0])
+        prim0 = rr.RRAABBListPrimitive(None, n, 24)
+        inp0 = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_AABB_LIST, 1, c_void_p(ctypes.addressof(prim0)))
+        req = rr.RRMemoryRequirements()
+        opts = self._options(restructure)
+        rr.check(self.lib.rrGetGeometryBuildMemoryRequirements(self.ctx, byref(inp0), byref(opts), byref(req)),
+                 "rrGetGeometryBuildMemoryRequirements")
+        result = update.buffer if update is not None else self.empty(req.result_buffer_size)
+        scratch = self.empty(req.temporary_update_buffer_size if update is not None else req.temporary_build_buffer_size)
+        with self.ptrs(boxes.data_ptr() if n else None, scratch.data_ptr(), result.data_ptr()) as (b, s, r):
+            prim = rr.RRAABBListPrimitive(b, n, 24)
+            inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_AABB_LIST, 1, c_void_p(ctypes.addressof(prim)))
+            op = rr.RR_BUILD_OPERATION_UPDATE if update is not None else rr.RR_BUILD_OPERATION_BUILD
+            rr.check(self.lib.rrCmdBuildGeometry(self.ctx, op, byref(inp), byref(opts), s, r, self.cs),
+                     "rrCmdBuildGeometry(aabbs)")
+        return Accel(result, "blas", n)
+
     def build_geometry_multi(self, meshes, restructure: bool = False, update: Accel = None) -> Accel:
         """One BLAS over several meshes [(vertices, indices)] (one build input with
         primitive_count = len(meshes)); with update=<accel>, refit it instead."""

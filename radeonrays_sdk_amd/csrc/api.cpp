@@ -333,6 +333,22 @@ long WaitSpinMicros() {
     return us;
 }
 
+// An AABB list (radeonrays.h:201-206; the reference declares it and answers
+// RR_ERROR_NOT_IMPLEMENTED, radeonrays.cpp:286-289).  One list per input.
+bool IsAabbList(const RRGeometryBuildInput* in) { return in->primitive_type == RR_PRIMITIVE_TYPE_AABB_LIST; }
+
+RRError AabbsFromInput(const RRGeometryBuildInput* in, rr::AabbDesc& a, const char* name, bool resolve) {
+    RR_REQUIRE(in->primitive_count == 1 && in->aabb_primitives, name);
+    const RRAABBListPrimitive& p = in->aabb_primitives[0];
+    RR_REQUIRE(p.aabb_count < (1u << 30), name);
+    a.count = p.aabb_count;
+    if (!resolve) return RR_SUCCESS;
+    if (p.aabb_count > 0) RR_REQUIRE(p.aabbs && p.aabb_stride >= 24 && p.aabb_stride % 4 == 0, name);
+    a.aabbs = Addr(p.aabbs);
+    a.stride = p.aabb_stride;
+    return RR_SUCCESS;
+}
+
 // Several triangle meshes in one build input (primitive_count > 1; the
 // reference asserts a single mesh, intersector.cpp:110,146).
 bool IsMultiMesh(const RRGeometryBuildInput* in) {
@@ -463,6 +479,15 @@ RRError rrGetGeometryBuildMemoryRequirements(RRContext context, const RRGeometry
                                              RRMemoryRequirements* memory_requirements) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrGetGeometryBuildMemoryRequirements");
     RR_REQUIRE(context && build_input && memory_requirements, "rrGetGeometryBuildMemoryRequirements");
+    if (IsAabbList(build_input)) {
+        rr::AabbDesc a;
+        const RRError e = AabbsFromInput(build_input, a, "rrGetGeometryBuildMemoryRequirements", false);
+        if (e != RR_SUCCESS) return e;
+        memory_requirements->result_buffer_size = rr::BlasResultSize(a.count);
+        memory_requirements->temporary_build_buffer_size = rr::AabbScratchBytes(a.count, true, WantsRestructure(build_options));
+        memory_requirements->temporary_update_buffer_size = rr::AabbScratchBytes(a.count, false, false);
+        return RR_SUCCESS;
+    }
     if (IsMultiMesh(build_input)) {
         std::vector<rr::SubMesh> subs;
         uint32_t tris = 0;
@@ -513,6 +538,28 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
     RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildGeometry");
     RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
                "rrCmdBuildGeometry");
+    if (IsAabbList(build_input)) {
+        rr::AabbDesc a;
+        const RRError e = AabbsFromInput(build_input, a, "rrCmdBuildGeometry", true);
+        if (e != RR_SUCCESS) return e;
+        RR_REQUIRE(geometry_buffer && temporary_buffer, "rrCmdBuildGeometry");
+        return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
+            Context* c = AsCtx(context);
+            DeviceGuard g(c->device);
+            void* result = Addr(geometry_buffer);
+            void* scratch = Addr(temporary_buffer);
+            const bool restructure = WantsRestructure(build_options);
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelBlas;
+            }
+            if (build_operation == RR_BUILD_OPERATION_BUILD)
+                Record(AsStream(command_stream), [=](hipStream_t s) { rr::BuildAabbs(a, restructure, scratch, result, s); });
+            else
+                Record(AsStream(command_stream), [=](hipStream_t s) { rr::UpdateAabbs(a, scratch, result, s); });
+            return RR_SUCCESS;
+        });
+    }
     if (IsMultiMesh(build_input)) {
         std::vector<rr::SubMesh> subs;
         uint32_t tris = 0;

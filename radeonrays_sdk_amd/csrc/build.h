@@ -16,7 +16,23 @@ struct MeshDesc {
     const void* indices = nullptr;
     uint32_t    index16 = 0;      // 1: uint16 indices, 0: uint32
     uint32_t    tri_count = 0;
+    uint32_t    accel_flags = 0;  // extra AccelHeader flags of the result (kAccelFlagAabbs)
 };
+
+// An AABB list in device memory (RRAABBListPrimitive, radeonrays.h:201-206):
+// count boxes of 6 floats (min xyz, max xyz) every `stride` bytes.
+struct AabbDesc {
+    const void* aabbs = nullptr;
+    uint32_t    count = 0;
+    uint32_t    stride = 24;
+};
+// An AABB list is built as the unindexed "triangles" (min, max, min), whose
+// bounds, centroids, leaf boxes and treelet boxes are exactly the boxes';
+// the result is flagged kAccelFlagAabbs so traversal tests leaf boxes.
+// Scratch: the gathered corners (36 bytes per box), then the mesh build's.
+size_t AabbScratchBytes(uint32_t count, bool build, bool restructure);
+void BuildAabbs(const AabbDesc& boxes, bool restructure, void* scratch, void* result, hipStream_t stream);
+void UpdateAabbs(const AabbDesc& boxes, void* scratch, void* result, hipStream_t stream);
 
 // One instance as uploaded for a TLAS build: forward object-to-world
 // transform and the device address of the instance's BLAS buffer.

@@ -610,6 +610,19 @@ __global__ __launch_bounds__(kBlock) void GatherSubMeshesKernel(const SubMesh* _
     for (int k = 0; k < 9; ++k) o[k] = v[k];
 }
 
+// Box i of an AABB list as the unindexed triangle (min, max, min).
+__global__ __launch_bounds__(kBlock) void GatherAabbsKernel(const uint8_t* __restrict__ aabbs, uint32_t count,
+                                                            uint32_t stride, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const float* b = reinterpret_cast<const float*>(aabbs + (size_t)i * stride);
+    float* o = out + 9 * (size_t)i;
+    const float x0 = b[0], y0 = b[1], z0 = b[2], x1 = b[3], y1 = b[4], z1 = b[5];
+    o[0] = x0; o[1] = y0; o[2] = z0;
+    o[3] = x1; o[4] = y1; o[5] = z1;
+    o[6] = x0; o[7] = y0; o[8] = z0;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 
@@ -630,7 +643,8 @@ void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* resu
     const SortScratch sc = CarveSortScratch(At<void>(scratch, L.sort), n);
     const bool do_restructure = restructure && n >= 64;
     InitBuildKernel<<<1, kBlock, 0, s>>>(hdr, kAccelBlas, n, sizeof(AccelHeader), 0, 0,
-                                         do_restructure ? kAccelFlagRestructured : 0u, head, sc.hist, sc.tickets);
+                                         (do_restructure ? kAccelFlagRestructured : 0u) | mesh.accel_flags, head,
+                                         sc.hist, sc.tickets);
     if (n == 0) return;
     const MeshView m = ViewOf(mesh);
     MeshBoundsKernel<<<std::min(kMaxGrid, Blocks(n)), kBlock, 0, s>>>(m, head);
@@ -669,6 +683,28 @@ void BuildMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, bool r
     if (tris) GatherSubMeshesKernel<<<Blocks(tris), kBlock, 0, s>>>(table, meshes, tris, static_cast<float*>(scratch));
     void* rest = static_cast<uint8_t*>(scratch) + MultiMeshScratchBytes(meshes, tris, false, false);
     BuildBlas(GatheredMesh(scratch, tris), restructure, rest, result, s);
+}
+
+size_t AabbScratchBytes(uint32_t count, bool build, bool restructure) {
+    return AlignUp(size_t(count) * 36, 256) + (build ? BlasBuildScratchBytes(count, restructure) : 0);
+}
+
+void BuildAabbs(const AabbDesc& boxes, bool restructure, void* scratch, void* result, hipStream_t s) {
+    const uint32_t n = boxes.count;
+    if (n)
+        GatherAabbsKernel<<<Blocks(n), kBlock, 0, s>>>(static_cast<const uint8_t*>(boxes.aabbs), n, boxes.stride,
+                                                       static_cast<float*>(scratch));
+    MeshDesc m = GatheredMesh(scratch, n);
+    m.accel_flags = kAccelFlagAabbs;
+    BuildBlas(m, restructure, static_cast<uint8_t*>(scratch) + AabbScratchBytes(n, false, false), result, s);
+}
+
+void UpdateAabbs(const AabbDesc& boxes, void* scratch, void* result, hipStream_t s) {
+    const uint32_t n = boxes.count;
+    if (!n) return;
+    GatherAabbsKernel<<<Blocks(n), kBlock, 0, s>>>(static_cast<const uint8_t*>(boxes.aabbs), n, boxes.stride,
+                                                   static_cast<float*>(scratch));
+    UpdateBlas(GatheredMesh(scratch, n), result, s);
 }
 
 void UpdateMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, void* scratch, void* result,

@@ -118,6 +118,25 @@ RR_DEV bool SlabTest(const RayState& r, float limit, float bx0, float by0, float
 
 RR_DEV float WidenLimit(float t) { return fmaf(kLimitWiden, fabsf(t), t); }
 
+// Leaf test of an AABB-list primitive: the ray's entry distance into the
+// box (exact slab test, no widening), t = max(entry, tmin), a hit when
+// t <= min(exit, tmax).  Same plane selection and NaN handling as SlabTest.
+RR_DEV bool BoxLeafTest(const RayState& r, float tmax, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+                        float& t) {
+    const float lx = (bx0 - r.ox) * r.ix, hx = (bx1 - r.ox) * r.ix;
+    const float ly = (by0 - r.oy) * r.iy, hy = (by1 - r.oy) * r.iy;
+    const float lz = (bz0 - r.oz) * r.iz, hz = (bz1 - r.oz) * r.iz;
+    const bool sx = r.ix < 0.0f, sy = r.iy < 0.0f, sz = r.iz < 0.0f;
+    const float nx = sx ? hx : lx, fx = sx ? lx : hx;
+    const float ny = sy ? hy : ly, fy = sy ? ly : hy;
+    const float nz = sz ? hz : lz, fz = sz ? lz : hz;
+    const float a = fmaxf(fmaxf(fmaxf(nx, ny), nz), r.tmin);
+    const float b = fminf(fminf(fminf(fx, fy), fz), tmax);
+    t = a;
+    return a <= b;
+}
+
+
 // Watertight ray/triangle test (Woop et al. 2013, with the paper's double
 // precision fallback for zero edge functions, no backface culling).  On a
 // hit returns true with the barycentric numerators and denominator: the hit
@@ -153,6 +172,20 @@ RR_DEV bool WatertightTest(const RayState& r, float v0x, float v0y, float v0z, f
     h.V = V; h.W = W; h.det = det;
     h.t = T / det;
     return true;
+}
+
+// Leaf test by primitive kind: a triangle (watertight) or, for an AABB
+// list, the leaf's box (stored as its min corner in v0 and max corner in
+// v1), reported with barycentrics 0.
+RR_DEV bool LeafTest(bool boxes, const RayState& r, float tmax, float v0x, float v0y, float v0z, float v1x, float v1y,
+                     float v1z, float v2x, float v2y, float v2z, TriHit& h) {
+    if (boxes) {
+        h.V = 0.0f;
+        h.W = 0.0f;
+        h.det = 1.0f;
+        return BoxLeafTest(r, tmax, v0x, v0y, v0z, v1x, v1y, v1z, h.t);
+    }
+    return WatertightTest(r, v0x, v0y, v0z, v1x, v1y, v1z, v2x, v2y, v2z, h);
 }
 
 // Closest-hit ordering: (t, inst, prim) lexicographic, so equal distances

@@ -56,6 +56,9 @@ struct AccelHeader {
 static_assert(sizeof(AccelHeader) == 128, "AccelHeader must be 128 bytes");
 
 constexpr uint32_t kAccelFlagRestructured = 1u;
+// The BLAS's primitives are axis-aligned boxes (RR_PRIMITIVE_TYPE_AABB_LIST):
+// each leaf stores its box's min corner in v0 and max corner in v1 and v2.
+constexpr uint32_t kAccelFlagAabbs = 2u;
 
 // Row-major 3x4 affine transform, 48 bytes (vlk/kernels/common.h:45-50).
 struct Transform {

@@ -95,6 +95,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
     const uint32_t top_count = hdr->node_count;
     const BvhNode* top_nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
+    bool boxes = !TWO_LEVEL && (hdr->flags & kAccelFlagAabbs) != 0;  // leaf kind of the level traversed
     const Transform* xforms = nullptr;
     const uint64_t* blas_table = nullptr;
     if (TWO_LEVEL) {
@@ -153,6 +154,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
                 inst = c1;
                 nodes = reinterpret_cast<const BvhNode*>(blas + bh->nodes_offset);
                 node_count = bh->node_count;
+                boxes = (bh->flags & kAccelFlagAabbs) != 0;
                 st.push(kTopLevelSentinel);
                 node = 0;
                 continue;
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
         } else {
             if (STATS) { ++n_leaf_visits; ++n_tri_tests; }
             TriHit h;
-            if (WatertightTest(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) && h.t >= r.tmin &&
+            if (LeafTest(boxes, r, best_t, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) && h.t >= r.tmin &&
                 h.t <= best_t) {
                 const uint32_t hit_inst = TWO_LEVEL ? inst : 0u;
                 if (ANY) {
@@ -184,6 +186,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
                 inst = kInvalid;
                 nodes = top_nodes;
                 node_count = top_count;
+                boxes = false;
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
                 continue;
             }
@@ -264,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     const uint32_t total = hdr->node_count;
     const uint32_t n_int = hdr->prim_count ? hdr->prim_count - 1 : 0;
     const BvhNode* nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
+    const bool boxes = (hdr->flags & kAccelFlagAabbs) != 0;
     uint32_t* counters = scratch;
     const uint32_t lane = __lane_id();
     const uint64_t lt = LaneMaskLt();
@@ -367,8 +371,8 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 LoadNode(nodes, leaf, q0, q1, q2, q3);
                 const uint32_t prim = __float_as_uint(q1.w);
                 TriHit h;
-                if (WatertightTest(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) && h.t >= r.tmin &&
-                    h.t <= best_t) {
+                if (LeafTest(boxes, r, best_t, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) &&
+                    h.t >= r.tmin && h.t <= best_t) {
                     if (ANY) {
                         best_t = h.t; best_prim = prim;
                         bV = h.V; bW = h.W; bdet = h.det;
@@ -437,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
 
     // Lane state.
     uint32_t rid = kInvalid, lreq = 0, lflags = 0, n_int = 0, total = 0;
+    bool boxes = false;
     const BvhNode* nodes = nullptr;
     RayState r;
     float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
@@ -477,6 +482,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 total = hdr->node_count;
                 n_int = hdr->prim_count ? hdr->prim_count - 1 : 0;
                 nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
+                boxes = (hdr->flags & kAccelFlagAabbs) != 0;
                 const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
                 tmax = ray.b.w;
@@ -531,8 +537,8 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 LoadNode(nodes, leaf, q0, q1, q2, q3);
                 const uint32_t prim = __float_as_uint(q1.w);
                 TriHit h;
-                if (WatertightTest(r, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) && h.t >= r.tmin &&
-                    h.t <= best_t) {
+                if (LeafTest(boxes, r, best_t, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) &&
+                    h.t >= r.tmin && h.t <= best_t) {
                     if (any) {
                         best_t = h.t; best_prim = prim;
                         bV = h.V; bW = h.W; bdet = h.det;

@@ -128,13 +128,21 @@ def test_tuning_validation():
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 8) == rr.RR_SUCCESS
 
 
-def test_aabb_list_is_not_implemented():
-    # radeonrays.cpp:286-289.
+def test_aabb_list_requirements_and_unknown_primitive_type():
+    # AABB lists are built (the reference answers RR_ERROR_NOT_IMPLEMENTED,
+    # radeonrays.cpp:286-289); an unknown primitive type still is not.
     lib = rr.load()
-    prim, inp = _mesh_input(10, ptype=rr.RR_PRIMITIVE_TYPE_AABB_LIST)
-    req = rr.RRMemoryRequirements()
-    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, byref(req)) == rr.RR_ERROR_NOT_IMPLEMENTED
-    assert (lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(inp), None, c_void_p(1), c_void_p(1),
+    boxes = rr.RRAABBListPrimitive(None, 10, 24)
+    inp = rr.RRGeometryBuildInput(rr.RR_PRIMITIVE_TYPE_AABB_LIST, 1, ctypes.cast(ctypes.pointer(boxes), c_void_p))
+    req, tri = rr.RRMemoryRequirements(), rr.RRMemoryRequirements()
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(inp), None, byref(req)) == rr.RR_SUCCESS
+    p10, i10 = _mesh_input(10)
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(i10), None, byref(tri)) == rr.RR_SUCCESS
+    assert req.result_buffer_size == tri.result_buffer_size
+    assert req.temporary_update_buffer_size >= 10 * 36
+    prim, bad = _mesh_input(10, ptype=7)
+    assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(bad), None, byref(req)) == rr.RR_ERROR_NOT_IMPLEMENTED
+    assert (lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(bad), None, c_void_p(1), c_void_p(1),
                                    c_void_p(1)) == rr.RR_ERROR_NOT_IMPLEMENTED)
 
 

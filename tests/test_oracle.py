@@ -399,3 +399,66 @@ def test_reference_bvh_analyzer_agrees_on_validity_and_sah():
         stats = dict(line.split(": ", 1) for line in out.stdout.splitlines() if ": " in line)
         assert stats["is_valid"].strip() == "1"
         assert math.isclose(float(stats["sah"]), oracle.sah(buf), rel_tol=1e-5)
+
+
+# --- AABB-list primitives ------------------------------------------------------
+def _ray(o, d, tmin=0.0, tmax=1e30):
+    return scenes.make_rays(np.asarray([o], np.float32), np.asarray([d], np.float32), tmin, tmax)
+
+
+def test_aabb_leaf_known_answers():
+    b = np.array([[0, 0, 0, 1, 1, 1], [0, 0, 3, 1, 1, 4]], np.float32)
+    buf = oracle.build_blas_aabbs(b)
+    assert oracle.is_valid(buf)
+    h = oracle.trace(buf, _ray((0.5, 0.5, -2), (0, 0, 1)))[0]
+    assert (h["prim_id"], h["t"], h["u"], h["v"]) == (0, 2.0, 0.0, 0.0)
+    h = oracle.trace(buf, _ray((0.5, 0.5, 2), (0, 0, 1)))[0]  # between the boxes: the far one at t = 1
+    assert (h["prim_id"], h["t"]) == (1, 1.0)
+    h = oracle.trace(buf, _ray((0.5, 0.5, 0.5), (0, 0, 1), tmin=0.25))[0]  # inside: t = tmin
+    assert (h["prim_id"], h["t"]) == (0, 0.25)
+    h = oracle.trace(buf, _ray((0.5, 0.5, -2), (0, 0, -1)))[0]  # pointing away
+    assert h["prim_id"] == 0xFFFFFFFF
+    h = oracle.trace(buf, _ray((0.5, 0.5, -2), (0, 0, 1), tmax=1.5))[0]  # too short
+    assert h["prim_id"] == 0xFFFFFFFF
+    h = oracle.trace(buf, _ray((1.0, 0.5, -2), (0, 0, 1)))[0]  # on a face plane, parallel to it
+    assert (h["prim_id"], h["t"]) == (0, 2.0)
+
+
+def _boxes(n, rng):
+    c = rng.uniform(0, 1, size=(n, 3))
+    e = rng.uniform(0.001, 0.03, size=(n, 3))
+    return np.concatenate([c - e, c + e], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("restructure", [False, True])
+def test_aabb_traversal_equals_fp64_brute_force(restructure):
+    rng = scenes.rng_for(60)
+    b = _boxes(3000, rng)
+    buf = oracle.build_blas_aabbs(b, restructure)
+    assert oracle.is_valid(buf) and oracle.check_consistency(buf)
+    o = rng.uniform(-0.5, 1.5, size=(4000, 3)).astype(np.float32)
+    rays = scenes.make_rays(o, scenes.unit_sphere(4000, rng))
+    got = oracle.trace(buf, rays)
+    # Float64 slab test over every box, independent of the BVH.
+    O = rays[:, 0:3].astype(np.float64)[:, None, :]
+    D = rays[:, 4:7].astype(np.float64)[:, None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0 = (b[None, :, 0:3] - O) / D
+        t1 = (b[None, :, 3:6] - O) / D
+    near = np.nanmax(np.minimum(t0, t1), axis=2)
+    far = np.nanmin(np.maximum(t0, t1), axis=2)
+    near = np.maximum(near, 0.0)
+    hit = near <= far
+    tt = np.where(hit, near, np.inf)
+    best = tt.argmin(axis=1)
+    bt = tt[np.arange(len(rays)), best]
+    anyhit = np.isfinite(bt)
+    assert np.array_equal(got["prim_id"] != 0xFFFFFFFF, anyhit)
+    with np.errstate(invalid="ignore"):
+        sel = anyhit & (np.sort(tt, axis=1)[:, 1] - bt > 1e-5 * np.maximum(bt, 1))  # no near-ties
+    assert np.array_equal(got["prim_id"][sel], best[sel])
+    assert np.allclose(got["t"][anyhit], bt[anyhit], rtol=1e-6, atol=1e-7)
+    assert (got["u"] == 0).all() and (got["v"] == 0).all()
+    bf = oracle.brute_force(buf, rays)
+    for f in ("prim_id", "t"):
+        assert np.array_equal(got[f], bf[f]), f
