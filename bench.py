@@ -396,7 +396,7 @@ def main():
         "any_mrays_s": round(n_rays / any_ms / 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(),
-                     "kernel": "TracePersistentKernel<closest, FULL_HIT>", "kernel_ms": round(closest_ms, 4),
+                     "kernel": "TracePersistentKernel<closest, FULL_HIT, LDS stack 8>", "kernel_ms": round(closest_ms, 4),
                      "algorithmic_bytes": int(alg_bytes),
                      "visits_per_ray": round(visits / n_rays, 3)},
     }

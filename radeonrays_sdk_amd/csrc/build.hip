@@ -27,6 +27,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxGrid = 4096;
+// Bounds blocks: each ends with six device atomics on the same six words,
+// so fewer, longer-running blocks contend less.
+constexpr uint32_t kBoundsGrid = 1024;
 
 inline uint32_t Blocks(size_t n, int per = kBlock) { return (uint32_t)std::max<size_t>(1, (n + per - 1) / per); }
 
@@ -237,6 +240,31 @@ __device__ __forceinline__ void NodeBox(const BvhNode* __restrict__ nodes, uint3
     }
 }
 
+// Agent-scope node accessors for the restructuring kernel (see step 5).
+__device__ __forceinline__ uint32_t LdU(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void StU(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void NodeBoxAgent(const BvhNode* __restrict__ nodes, uint32_t idx, uint32_t n_internal,
+                                             float lo[3], float hi[3]) {
+    const BvhNode& nd = nodes[idx];
+    for (int a = 0; a < 3; ++a) {
+        const float p = LoadAgent(&nd.a0_min_or_v0[a]), q = LoadAgent(&nd.a0_max_or_v1[a]);
+        const float r = LoadAgent(&nd.a1_min_or_v2[a]);
+        float l = fminf(fminf(p, q), r);
+        float h = fmaxf(fmaxf(p, q), r);
+        if (idx < n_internal) {
+            const float w = LoadAgent(&nd.a1_max_or_v3[a]);
+            l = fminf(l, w);
+            h = fmaxf(h, w);
+        }
+        lo[a] = l;
+        hi[a] = h;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void InitTreeletCountsKernel(uint32_t n, uint32_t* counts, uint32_t* counter) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n - 1) counts[i] = 0;
@@ -292,11 +320,11 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
             float area[kTreeletLeaves];
             float lo[3], hi[3];
             s_inner[0] = root;
-            s_leaf[0] = nodes[root].child0;
-            s_leaf[1] = nodes[root].child1;
+            s_leaf[0] = LdU(&nodes[root].child0);
+            s_leaf[1] = LdU(&nodes[root].child1);
             bool ok = s_leaf[0] < 2 * n - 1 && s_leaf[1] < 2 * n - 1;
             for (int k = 0; k < 2 && ok; ++k) {
-                NodeBox(nodes, s_leaf[k], n_int, lo, hi);
+                NodeBoxAgent(nodes, s_leaf[k], n_int, lo, hi);
                 area[k] = SurfaceArea(lo, hi);
             }
             for (int size = 2; ok && size < kTreeletLeaves; ++size) {
@@ -311,18 +339,18 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
                 if (sel < 0) { ok = false; break; }
                 const uint32_t opened = s_leaf[sel];
                 s_inner[size - 1] = opened;
-                const uint32_t c0 = nodes[opened].child0, c1 = nodes[opened].child1;
+                const uint32_t c0 = LdU(&nodes[opened].child0), c1 = LdU(&nodes[opened].child1);
                 if (c0 >= 2 * n - 1 || c1 >= 2 * n - 1) { ok = false; break; }
                 s_leaf[sel] = c0;
                 s_leaf[size] = c1;
-                NodeBox(nodes, c0, n_int, lo, hi);
+                NodeBoxAgent(nodes, c0, n_int, lo, hi);
                 area[sel] = SurfaceArea(lo, hi);
-                NodeBox(nodes, c1, n_int, lo, hi);
+                NodeBoxAgent(nodes, c1, n_int, lo, hi);
                 area[size] = SurfaceArea(lo, hi);
             }
             if (ok) {
                 for (int k = 0; k < kTreeletLeaves; ++k) {
-                    NodeBox(nodes, s_leaf[k], n_int, lo, hi);
+                    NodeBoxAgent(nodes, s_leaf[k], n_int, lo, hi);
                     for (int a = 0; a < 3; ++a) { s_box[k][a] = lo[a]; s_box[k][3 + a] = hi[a]; }
                 }
             }
@@ -405,33 +433,45 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
                     }
                     BvhNode& nd = nodes[node];
                     for (int a = 0; a < 3; ++a) {
-                        nd.a0_min_or_v0[a] = box[0][a];
-                        nd.a0_max_or_v1[a] = box[0][3 + a];
-                        nd.a1_min_or_v2[a] = box[1][a];
-                        nd.a1_max_or_v3[a] = box[1][3 + a];
+                        StoreAgent(&nd.a0_min_or_v0[a], box[0][a]);
+                        StoreAgent(&nd.a0_max_or_v1[a], box[0][3 + a]);
+                        StoreAgent(&nd.a1_min_or_v2[a], box[1][a]);
+                        StoreAgent(&nd.a1_max_or_v3[a], box[1][3 + a]);
                     }
-                    nd.child0 = child[0];
-                    nd.child1 = child[1];
-                    nodes[child[0]].parent = node;
-                    nodes[child[1]].parent = node;
+                    StU(&nd.child0, child[0]);
+                    StU(&nd.child1, child[1]);
+                    StU(&nodes[child[0]].parent, node);
+                    StU(&nodes[child[1]].parent, node);
                 }
             }
         }
         // 5. Climb: the last of the two subtrees to finish below the parent
         //    refits it and restructures there next (restructure_bvh.comp:497-540).
+        //    Hand-off as in ClimbRefit: every node access of this kernel is
+        //    an agent-scope (write-through / coherent) load or store, the
+        //    wave waits for its stores before the arrival counter's relaxed
+        //    atomic, and the second arriver then reads what the first wrote.
         if (lane == 0) {
-            const uint32_t parent = nodes[s_root].parent;
+            const uint32_t parent = LdU(&nodes[s_root].parent);
             uint32_t go = 0;
             if (parent < n_int) {
+                WaitStores();
                 const uint32_t old =
-                    __hip_atomic_fetch_add(&counts[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(&counts[parent], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (old != 0) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     float lo[3], hi[3];
                     BvhNode& pn = nodes[parent];
-                    NodeBox(nodes, pn.child0, n_int, lo, hi);
-                    for (int a = 0; a < 3; ++a) { pn.a0_min_or_v0[a] = lo[a]; pn.a0_max_or_v1[a] = hi[a]; }
-                    NodeBox(nodes, pn.child1, n_int, lo, hi);
-                    for (int a = 0; a < 3; ++a) { pn.a1_min_or_v2[a] = lo[a]; pn.a1_max_or_v3[a] = hi[a]; }
+                    NodeBoxAgent(nodes, LdU(&pn.child0), n_int, lo, hi);
+                    for (int a = 0; a < 3; ++a) {
+                        StoreAgent(&pn.a0_min_or_v0[a], lo[a]);
+                        StoreAgent(&pn.a0_max_or_v1[a], hi[a]);
+                    }
+                    NodeBoxAgent(nodes, LdU(&pn.child1), n_int, lo, hi);
+                    for (int a = 0; a < 3; ++a) {
+                        StoreAgent(&pn.a1_min_or_v2[a], lo[a]);
+                        StoreAgent(&pn.a1_max_or_v3[a], hi[a]);
+                    }
                     s_root = parent;
                     go = 1;
                 }
@@ -647,7 +687,7 @@ void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* resu
                                          sc.hist, sc.tickets);
     if (n == 0) return;
     const MeshView m = ViewOf(mesh);
-    MeshBoundsKernel<<<std::min(kMaxGrid, Blocks(n)), kBlock, 0, s>>>(m, head);
+    MeshBoundsKernel<<<std::min(kBoundsGrid, Blocks(n)), kBlock, 0, s>>>(m, head);
     uint32_t* keys = At<uint32_t>(scratch, L.keys);
     uint32_t* vals = At<uint32_t>(scratch, L.vals);
     const int passes = SortPasses(30);

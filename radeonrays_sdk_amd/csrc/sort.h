@@ -56,12 +56,15 @@ void LaunchSortClear(const SortScratch& sc, hipStream_t stream);
 // a mesh in triangle order) are mostly equal across a wave, and same-address
 // LDS atomics serialise, so for those a wave whose valid keys all share the
 // digit adds once; lower digits are near-random and go straight to atomics.
+// With CHECK_TOP false (keys of no particular order) every digit goes
+// straight to the atomics.
+template <bool CHECK_TOP = true>
 __device__ __forceinline__ void AccumulateDigits(uint32_t key, bool valid, uint32_t (*s_h)[kSortRadix], int passes) {
     const uint64_t act = __ballot(valid);
     const uint32_t lane = __lane_id();
     for (int p = 0; p < passes; ++p) {
         const uint32_t d = (key >> (p * kSortRadixBits)) & (kSortRadix - 1);
-        if (p + 2 < passes) {
+        if (!CHECK_TOP || p + 2 < passes) {
             if (valid) atomicAdd(&s_h[p][d], 1u);
             continue;
         }

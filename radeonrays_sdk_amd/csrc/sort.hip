@@ -84,17 +84,17 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
         for (int j = 0; j < kHistKeysPerThread / 4; ++j) q[j] = src[j * kBlock + threadIdx.x];
 #pragma unroll
         for (int j = 0; j < kHistKeysPerThread / 4; ++j) {
-            AccumulateDigits(q[j].x, true, s_h, passes);
-            AccumulateDigits(q[j].y, true, s_h, passes);
-            AccumulateDigits(q[j].z, true, s_h, passes);
-            AccumulateDigits(q[j].w, true, s_h, passes);
+            AccumulateDigits<false>(q[j].x, true, s_h, passes);
+            AccumulateDigits<false>(q[j].y, true, s_h, passes);
+            AccumulateDigits<false>(q[j].z, true, s_h, passes);
+            AccumulateDigits<false>(q[j].w, true, s_h, passes);
         }
     }
     // The rest, one key per lane (whole waves iterate together).
     for (uint32_t b = full * kStep + blockIdx.x * kBlock + (threadIdx.x & ~63u); b < n; b += gridDim.x * kBlock) {
         const uint32_t i = b + __lane_id();
         const bool valid = i < n;
-        AccumulateDigits(valid ? keys[i] : 0u, valid, s_h, passes);
+        AccumulateDigits<false>(valid ? keys[i] : 0u, valid, s_h, passes);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kSortRadix; i += kBlock) {
