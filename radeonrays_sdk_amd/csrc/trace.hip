@@ -659,10 +659,20 @@ uint32_t PersistentBlocks() {
 
 uint32_t BatchBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatchKernel), 1); }
 
+// Rays per queue atomic for n rays over `blocks` blocks: the tuned value,
+// cut (to a multiple of 32, at least 32) when fewer rays than waves x fetch
+// would leave waves without any.
+uint32_t FetchFor(uint32_t n, uint32_t blocks) {
+    const uint64_t waves = (uint64_t)blocks * (kBlock / 64);
+    const uint64_t per = (n + waves - 1) / waves;
+    const uint32_t f = (uint32_t)std::max<uint64_t>(32, (per + 31) / 32 * 32);
+    return std::min(RaysPerFetch(), f);
+}
+
 template <bool ANY, int DEPTH>
 void LaunchPersistentDepth(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r,
                            uint32_t n, const uint32_t* ind, void* hits, uint32_t* sc) {
-    const uint32_t mb = MinBusyLanes(), f = RaysPerFetch();
+    const uint32_t mb = MinBusyLanes(), f = FetchFor(n, blocks);
     switch (out) {
         case kOutFullHit:
             TracePersistentKernel<ANY, kOutFullHit, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
