@@ -14,6 +14,7 @@ written for 2^7 masks and 9 leaves would take 4x the LDS per treelet).
 """
 from __future__ import annotations
 
+import contextlib
 import itertools
 from ctypes import byref, c_uint32
 
@@ -46,6 +47,18 @@ def apply(lib, cfg: dict) -> None:
     set_(lib, rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, cfg["rays_per_fetch"])
     set_(lib, rr.RRX_TUNING_TRACE_LDS_STACK, cfg["lds_stack"])
     set_(lib, rr.RRX_TUNING_TRACE_MIN_BUSY, cfg["min_busy"])
+
+
+@contextlib.contextmanager
+def plain_traversal(lib, plain: bool = True):
+    """Run the enclosed queries on the plain kernels (one lane per ray for its
+    whole traversal) instead of the persistent ones, then restore the mode."""
+    before = get(lib, rr.RRX_TUNING_TRACE_PLAIN)
+    set_(lib, rr.RRX_TUNING_TRACE_PLAIN, int(plain))
+    try:
+        yield
+    finally:
+        set_(lib, rr.RRX_TUNING_TRACE_PLAIN, before)
 
 
 def tune_traversal(ctx, accel, rays, queries=("closest", "any"), reps: int = 5, fetch=FETCH, depth=DEPTH,

@@ -1202,6 +1202,10 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             return rr::SetRaysPerFetch(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
         case RRX_TUNING_TRACE_LDS_STACK:
             return rr::SetLdsStackDepth(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_TRACE_PLAIN:
+            RR_REQUIRE(value <= 1, "rrxSetTuning");
+            rr::SetTraceMode(value ? rr::kTraceModePlain : rr::kTraceModePersistent);
+            return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1212,6 +1216,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_TRACE_MIN_BUSY: *value = rr::MinBusyLanes(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_RAYS_PER_FETCH: *value = rr::RaysPerFetch(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_LDS_STACK: *value = rr::LdsStackDepth(); return RR_SUCCESS;
+        case RRX_TUNING_TRACE_PLAIN: *value = rr::TraceMode() == rr::kTraceModePlain ? 1u : 0u; return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

@@ -58,7 +58,7 @@ struct RayState {
     float tmin;           // lower bound of the accepted interval
     float ix, iy, iz;     // reciprocal direction for slab tests
     float sx, sy, sz;     // watertight shear constants
-    int   kx, ky, kz;     // watertight axis permutation
+    uint32_t k;           // watertight axis permutation: kx | ky << 2 | kz << 4
 };
 
 // Conservative widening of slab intervals: slab distances (b - o) * (1/d)
@@ -78,17 +78,25 @@ RR_DEV void SetupRay(RayState& r, float ox, float oy, float oz, float tmin, floa
     r.dx = dx; r.dy = dy; r.dz = dz;
     // Exact reciprocals: a zero component gives +-inf (sign of the zero), so
     // an axis-parallel ray has an unbounded slab on that axis.
-    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    // One division at a time (a scheduling barrier between them): ray set-up
+    // runs while a lane's whole traversal state is live, and three
+    // interleaved divisions would set the kernel's register peak.
+    r.ix = 1.0f / dx;
+    __builtin_amdgcn_sched_barrier(0);
+    r.iy = 1.0f / dy;
+    __builtin_amdgcn_sched_barrier(0);
+    r.iz = 1.0f / dz;
     // Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013:
     // kz = dimension of largest |d| (first on ties), kx/ky the next two in
     // cyclic order, swapped when d[kz] < 0 to keep the winding.
     float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
-    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
-    int kx = kz == 2 ? 0 : kz + 1;
-    int ky = kx == 2 ? 0 : kx + 1;
+    const int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
     const float dkz = Perm(kz, dx, dy, dz);
-    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
-    r.kx = kx; r.ky = ky; r.kz = kz;
+    // (kx, ky) = (1, 2), (2, 0), (0, 1) for kz = 0, 1, 2, swapped when d[kz] < 0.
+    constexpr uint32_t kFwd0 = 1u | 2u << 2, kFwd1 = 2u | 0u << 2 | 1u << 4, kFwd2 = 0u | 1u << 2 | 2u << 4;
+    constexpr uint32_t kRev0 = 2u | 1u << 2, kRev1 = 0u | 2u << 2 | 1u << 4, kRev2 = 1u | 0u << 2 | 2u << 4;
+    r.k = dkz < 0.0f ? (kz == 0 ? kRev0 : (kz == 1 ? kRev1 : kRev2)) : (kz == 0 ? kFwd0 : (kz == 1 ? kFwd1 : kFwd2));
+    const int kx = (int)(r.k & 3u), ky = (int)((r.k >> 2) & 3u);
     // Shear constants from the reciprocal already computed for the slab test.
     r.sz = Perm(kz, r.ix, r.iy, r.iz);
     r.sx = Perm(kx, dx, dy, dz) * r.sz;
@@ -150,9 +158,10 @@ RR_DEV bool WatertightTest(const RayState& r, float v0x, float v0y, float v0z, f
     const float Ax0 = v0x - r.ox, Ay0 = v0y - r.oy, Az0 = v0z - r.oz;
     const float Ax1 = v1x - r.ox, Ay1 = v1y - r.oy, Az1 = v1z - r.oz;
     const float Ax2 = v2x - r.ox, Ay2 = v2y - r.oy, Az2 = v2z - r.oz;
-    const float akx = Perm(r.kx, Ax0, Ay0, Az0), aky = Perm(r.ky, Ax0, Ay0, Az0), akz = Perm(r.kz, Ax0, Ay0, Az0);
-    const float bkx = Perm(r.kx, Ax1, Ay1, Az1), bky = Perm(r.ky, Ax1, Ay1, Az1), bkz = Perm(r.kz, Ax1, Ay1, Az1);
-    const float ckx = Perm(r.kx, Ax2, Ay2, Az2), cky = Perm(r.ky, Ax2, Ay2, Az2), ckz = Perm(r.kz, Ax2, Ay2, Az2);
+    const int kx = (int)(r.k & 3u), ky = (int)((r.k >> 2) & 3u), kz = (int)(r.k >> 4);
+    const float akx = Perm(kx, Ax0, Ay0, Az0), aky = Perm(ky, Ax0, Ay0, Az0), akz = Perm(kz, Ax0, Ay0, Az0);
+    const float bkx = Perm(kx, Ax1, Ay1, Az1), bky = Perm(ky, Ax1, Ay1, Az1), bkz = Perm(kz, Ax1, Ay1, Az1);
+    const float ckx = Perm(kx, Ax2, Ay2, Az2), cky = Perm(ky, Ax2, Ay2, Az2), ckz = Perm(kz, Ax2, Ay2, Az2);
     const float Ax = akx - r.sx * akz, Ay = aky - r.sy * akz;
     const float Bx = bkx - r.sx * bkz, By = bky - r.sy * bkz;
     const float Cx = ckx - r.sx * ckz, Cy = cky - r.sy * ckz;
@@ -212,10 +221,35 @@ RR_DEV T WaveMax(T v) {
     return v;
 }
 
+// Loads and stores through the global address space.  A pointer read from
+// memory (a BLAS address in a TLAS's table, a batched request's buffers) is
+// generic to the compiler, which then emits flat instructions; those count
+// against both the vector-memory and the LDS counters, so each wait for one
+// also waits for the other (the LDS stack's accesses among them).  Every
+// buffer here is device memory, so the cast to address space 1 is exact.
+typedef float RrFloat4V __attribute__((ext_vector_type(4)));
+template <typename T>
+using GlobalPtr = const T __attribute__((address_space(1)))*;
+template <typename T>
+using GlobalPtrW = T __attribute__((address_space(1)))*;
+template <typename T>
+RR_DEV T LdG(const T* p) { return *(GlobalPtr<T>)p; }
+RR_DEV float4 LdG4(const void* p) {
+    const RrFloat4V v = *(GlobalPtr<RrFloat4V>)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <typename T>
+RR_DEV void StG(T* p, T v) { *(GlobalPtrW<T>)p = v; }
+RR_DEV void StG4(void* p, float4 v) {
+    RrFloat4V w;
+    w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
+    *(GlobalPtrW<RrFloat4V>)p = w;
+}
+
 // Node loads as four 16-byte vector loads.
 RR_DEV void LoadNode(const BvhNode* __restrict__ nodes, uint32_t i, float4& q0, float4& q1, float4& q2, float4& q3) {
     const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    q0 = p[0]; q1 = p[1]; q2 = p[2]; q3 = p[3];
+    q0 = LdG4(p); q1 = LdG4(p + 1); q2 = LdG4(p + 2); q3 = LdG4(p + 3);
 }
 
 }  // namespace rr

@@ -21,9 +21,9 @@ constexpr int kSpillEntries = kStackCapacity - kMinLdsStackEntries;
 
 size_t TraceScratchBytes(uint32_t ray_count);
 
-// Single-level queries run the persistent kernel unless the mode is plain
-// (one lane per ray, the exact depth-first loop); RR_TRACE_MODE=plain selects
-// it at start-up.
+// Queries run the persistent kernels (single- or two-level) unless the mode
+// is plain (one lane per ray, the exact depth-first loop); RR_TRACE_MODE=plain
+// selects it at start-up.
 enum TraceModeKind : int { kTraceModePersistent = 0, kTraceModePlain = 1 };
 int TraceMode();
 void SetTraceMode(int mode);

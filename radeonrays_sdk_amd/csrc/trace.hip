@@ -43,8 +43,8 @@ struct RayIn {
 
 __device__ __forceinline__ RayIn LoadRay(const float4* __restrict__ rays, uint32_t i) {
     RayIn r;
-    r.a = rays[2 * (size_t)i];
-    r.b = rays[2 * (size_t)i + 1];
+    r.a = LdG4(rays + 2 * (size_t)i);
+    r.b = LdG4(rays + 2 * (size_t)i + 1);
     return r;
 }
 
@@ -62,7 +62,7 @@ struct LaneStack {
     __device__ __forceinline__ void push(uint32_t v) {
         if (sp == kStackCapacity) return;  // full: drop (tree deeper than the stack)
         if (sp - gbase == DEPTH) {
-            spill[gbase] = lds[gbase & (DEPTH - 1)][threadIdx.x];
+            StG(spill + gbase, lds[gbase & (DEPTH - 1)][threadIdx.x]);
             ++gbase;
         }
         lds[sp & (DEPTH - 1)][threadIdx.x] = v;
@@ -75,7 +75,7 @@ struct LaneStack {
         uint32_t v = lds[sp & (DEPTH - 1)][threadIdx.x];
         if (sp < gbase) {
             gbase = sp;
-            v = spill[sp];
+            v = LdG(spill + sp);
         }
         return v;
     }
@@ -140,9 +140,10 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
         } else if (TWO_LEVEL && inst == kInvalid) {
             // Enter instance c1 (isect_2l.comp:222-237).
             if (STATS) ++n_leaf_visits;
-            const uint8_t* blas = reinterpret_cast<const uint8_t*>(blas_table[c1]);
+            const uint8_t* blas = reinterpret_cast<const uint8_t*>(LdG(blas_table + c1));
             const AccelHeader* bh = reinterpret_cast<const AccelHeader*>(blas);
-            if (bh->node_count != 0) {
+            const uint32_t bn = LdG(&bh->node_count);
+            if (bn != 0) {
                 const Transform x = xforms[2 * (size_t)c1];
                 const float ox = ((x.m[0][0] * ray.a.x + x.m[0][1] * ray.a.y) + x.m[0][2] * ray.a.z) + x.m[0][3];
                 const float oy = ((x.m[1][0] * ray.a.x + x.m[1][1] * ray.a.y) + x.m[1][2] * ray.a.z) + x.m[1][3];
@@ -152,9 +153,9 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
                 const float dz = (x.m[2][0] * ray.b.x + x.m[2][1] * ray.b.y) + x.m[2][2] * ray.b.z;
                 SetupRay(r, ox, oy, oz, ray.a.w, dx, dy, dz);
                 inst = c1;
-                nodes = reinterpret_cast<const BvhNode*>(blas + bh->nodes_offset);
-                node_count = bh->node_count;
-                boxes = (bh->flags & kAccelFlagAabbs) != 0;
+                nodes = reinterpret_cast<const BvhNode*>(blas + LdG(&bh->nodes_offset));
+                node_count = bn;
+                boxes = (LdG(&bh->flags) & kAccelFlagAabbs) != 0;
                 st.push(kTopLevelSentinel);
                 node = 0;
                 continue;
@@ -220,15 +221,16 @@ template <int OUT, bool TWO_LEVEL>
 __device__ __forceinline__ void WriteHit(void* __restrict__ hits, uint32_t i, bool hit, float u, float v,
                                          uint32_t inst, uint32_t prim, float t, float tmax) {
     if (OUT == kOutFullHit) {
-        reinterpret_cast<float4*>(hits)[i] = hit ? make_float4(u, v, __uint_as_float(inst), __uint_as_float(prim))
-                                                 : make_float4(0.0f, 0.0f, __uint_as_float(kInvalid), __uint_as_float(kInvalid));
+        StG4(reinterpret_cast<float4*>(hits) + i,
+             hit ? make_float4(u, v, __uint_as_float(inst), __uint_as_float(prim))
+                 : make_float4(0.0f, 0.0f, __uint_as_float(kInvalid), __uint_as_float(kInvalid)));
     } else if (OUT == kOutInstanceId) {
-        reinterpret_cast<uint32_t*>(hits)[i] = hit ? (TWO_LEVEL ? inst : prim) : kInvalid;
+        StG(reinterpret_cast<uint32_t*>(hits) + i, hit ? (TWO_LEVEL ? inst : prim) : kInvalid);
     } else {
         float4* o = reinterpret_cast<float4*>(hits) + 2 * (size_t)i;
-        o[0] = hit ? make_float4(u, v, __uint_as_float(inst), __uint_as_float(prim))
-                   : make_float4(0.0f, 0.0f, __uint_as_float(kInvalid), __uint_as_float(kInvalid));
-        o[1] = make_float4(hit ? t : tmax, 0.0f, 0.0f, 0.0f);
+        StG4(o, hit ? make_float4(u, v, __uint_as_float(inst), __uint_as_float(prim))
+                    : make_float4(0.0f, 0.0f, __uint_as_float(kInvalid), __uint_as_float(kInvalid)));
+        StG4(o + 1, make_float4(hit ? t : tmax, 0.0f, 0.0f, 0.0f));
     }
 }
 
@@ -405,6 +407,226 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent two-level traversal: the single-level kernel's ray queues and
+// while-while loop over a TLAS whose leaves are instances.  A lane that
+// reaches an instance leaf enters it at once (world ray to object space,
+// sentinel pushed, BLAS root next; isect_2l.comp:222-237).  Triangle leaves
+// are postponed as in the single-level kernel, and a lane never pops the
+// sentinel (leaves the instance) while it holds a postponed leaf, so every
+// leaf is tested against its own instance's ray and nodes, and leaves are
+// still tested in depth-first order: closest and any hits are those of the
+// plain kernel.  Inner nodes and leaves are told apart by index (inner nodes
+// [0, N-1), leaves [N-1, 2N-1) at both levels).
+
+template <bool ANY, int OUT, int DEPTH>
+__global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t* __restrict__ accel,
+                                                                  const float4* __restrict__ rays, uint32_t ray_count,
+                                                                  const uint32_t* __restrict__ indirect,
+                                                                  void* __restrict__ hits,
+                                                                  uint32_t* __restrict__ scratch, uint32_t min_busy,
+                                                                  uint32_t fetch) {
+    __shared__ uint32_t s_stack[DEPTH][kBlock];
+    __shared__ float s_world[6][kBlock];  // each lane's world ray: origin xyz, direction xyz
+    uint32_t n = ray_count;
+    if (indirect) n = min(n, *indirect);
+    const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
+    const uint32_t top_total = hdr->node_count;
+    const uint32_t top_prims = hdr->prim_count;
+    const uint32_t top_int = top_prims ? top_prims - 1 : 0;
+    const BvhNode* top_nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
+    const Transform* xforms = reinterpret_cast<const Transform*>(accel + hdr->transforms_offset);
+    const uint64_t* blas_table = reinterpret_cast<const uint64_t*>(accel + hdr->blas_table_offset);
+    uint32_t* counters = scratch;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = LaneMaskLt();
+    const uint32_t part = (n + kQueues - 1) / kQueues;
+
+    // Wave-uniform queue state.
+    uint32_t q = blockIdx.x % kQueues, tries = 0, q_next = 0, q_end = 0;
+    bool drained = n == 0;
+
+    // Lane state: the ray of the level being traversed (the world ray waits
+    // in LDS), the best hit (best_t is the ray's t_max until a hit), and the
+    // level (inst == kInvalid at the top).
+    uint32_t rid = kInvalid;
+    RayState r;
+    float best_t = 0.0f, bu = 0.0f, bv = 0.0f;  // the culling limit is WidenLimit(best_t), recomputed per node
+    uint32_t best_prim = kInvalid, best_inst = kInvalid;
+    uint32_t node = kInvalid, leaf = kInvalid;
+    uint32_t inst = kInvalid;
+    const BvhNode* cur = top_nodes;
+    uint32_t cur_int = top_int, cur_total = top_total;
+    bool boxes = false;
+    LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries,
+                        0, 0};
+
+    for (;;) {
+        // Refill idle lanes.
+        const uint64_t idle = __ballot(rid == kInvalid);
+        if (idle && !drained) {
+            if (q_next == q_end) {
+                for (;;) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(&counters[q * kQueueStride], fetch);
+                    b = __shfl(b, 0, 64);
+                    const uint32_t end = min((q + 1) * part, n);
+                    if (b < end) {
+                        q_next = b;
+                        q_end = min(b + fetch, end);
+                        break;
+                    }
+                    if (++tries == kQueues) {
+                        drained = true;
+                        break;
+                    }
+                    q = q + 1 == kQueues ? 0 : q + 1;
+                }
+            }
+            const uint32_t take = min((uint32_t)__popcll(idle), q_end - q_next);
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (rid == kInvalid && rank < take) {
+                rid = q_next + rank;
+                const RayIn ray = LoadRay(rays, rid);
+                s_world[0][threadIdx.x] = ray.a.x; s_world[1][threadIdx.x] = ray.a.y; s_world[2][threadIdx.x] = ray.a.z;
+                s_world[3][threadIdx.x] = ray.b.x; s_world[4][threadIdx.x] = ray.b.y; s_world[5][threadIdx.x] = ray.b.z;
+                SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
+                best_t = ray.b.w;
+                best_prim = kInvalid;
+                best_inst = kInvalid;
+                bu = 0.0f; bv = 0.0f;
+                st.sp = 0;
+                st.gbase = 0;
+                inst = kInvalid;
+                cur = top_nodes;
+                cur_int = top_int;
+                cur_total = top_total;
+                boxes = false;
+                node = top_total ? 0u : kInvalid;
+                leaf = kInvalid;
+            }
+            q_next += take;
+        }
+        if (__ballot(rid != kInvalid) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (rid == kInvalid) continue;
+
+        for (;;) {
+            // Phase A: inner nodes, instance entries and exits, until every
+            // traversing lane has postponed a leaf or too few still traverse.
+            for (;;) {
+                if (node < cur_int) {
+                    float4 q0, q1, q2, q3;
+                    LoadNode(cur, node, q0, q1, q2, q3);
+                    const uint32_t c0 = __float_as_uint(q0.w);
+                    const uint32_t c1 = __float_as_uint(q1.w);
+                    float ta, tb;
+                    const float limit = WidenLimit(best_t);
+                    const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
+                    const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
+                    if (h0 || h1) {
+                        const bool c1first = h1 && (!h0 || ta > tb);
+                        if (h0 && h1) st.push(c1first ? c0 : c1);
+                        node = c1first ? c1 : c0;
+                    } else {
+                        node = st.empty() ? kInvalid : st.pop();
+                    }
+                } else if (inst == kInvalid && node < cur_total) {
+                    // Instance leaf: enter it (isect_2l.comp:222-237).
+                    const uint32_t id = __float_as_uint(LdG4(&top_nodes[node].a0_max_or_v1).w);
+                    const uint8_t* blas = id < top_prims ? reinterpret_cast<const uint8_t*>(LdG(blas_table + id)) : nullptr;
+                    const AccelHeader* bh = reinterpret_cast<const AccelHeader*>(blas);
+                    const uint32_t bn = blas ? LdG(&bh->node_count) : 0u;
+                    if (bn != 0) {
+                        const float4* xp = reinterpret_cast<const float4*>(xforms + 2 * (size_t)id);
+                        const float4 x0 = LdG4(xp), x1 = LdG4(xp + 1), x2 = LdG4(xp + 2);
+                        const float wox = s_world[0][threadIdx.x], woy = s_world[1][threadIdx.x], woz = s_world[2][threadIdx.x];
+                        const float wdx = s_world[3][threadIdx.x], wdy = s_world[4][threadIdx.x], wdz = s_world[5][threadIdx.x];
+                        const float ox = ((x0.x * wox + x0.y * woy) + x0.z * woz) + x0.w;
+                        const float oy = ((x1.x * wox + x1.y * woy) + x1.z * woz) + x1.w;
+                        const float oz = ((x2.x * wox + x2.y * woy) + x2.z * woz) + x2.w;
+                        const float dx = (x0.x * wdx + x0.y * wdy) + x0.z * wdz;
+                        const float dy = (x1.x * wdx + x1.y * wdy) + x1.z * wdz;
+                        const float dz = (x2.x * wdx + x2.y * wdy) + x2.z * wdz;
+                        SetupRay(r, ox, oy, oz, r.tmin, dx, dy, dz);
+                        inst = id;
+                        cur = reinterpret_cast<const BvhNode*>(blas + LdG(&bh->nodes_offset));
+                        cur_total = bn;
+                        cur_int = (bn - 1) / 2;
+                        boxes = (LdG(&bh->flags) & kAccelFlagAabbs) != 0;
+                        st.push(kTopLevelSentinel);
+                        node = 0;
+                    } else {
+                        node = st.empty() ? kInvalid : st.pop();
+                    }
+                } else if (node == kTopLevelSentinel && leaf == kInvalid) {
+                    // Leave the instance, restore the world ray (isect_2l.comp:278-286).
+                    inst = kInvalid;
+                    cur = top_nodes;
+                    cur_int = top_int;
+                    cur_total = top_total;
+                    boxes = false;
+                    SetupRay(r, s_world[0][threadIdx.x], s_world[1][threadIdx.x], s_world[2][threadIdx.x], r.tmin,
+                             s_world[3][threadIdx.x], s_world[4][threadIdx.x], s_world[5][threadIdx.x]);
+                    node = st.empty() ? kInvalid : st.pop();
+                } else if (inst != kInvalid && node < cur_total && leaf == kInvalid) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                } else {
+                    break;  // a second leaf or the sentinel behind a postponed leaf, or done
+                }
+                // An index past the level's nodes (corrupt buffer) ends the ray.
+                if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
+                if (!__any(leaf == kInvalid) || (uint32_t)__popcll(__ballot(1)) < min_busy) break;
+            }
+            // Phase B: the postponed leaf, then the leaves that follow it in order.
+            if (leaf == kInvalid && inst != kInvalid && node >= cur_int && node < cur_total) {
+                leaf = node;
+                node = st.empty() ? kInvalid : st.pop();
+                if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
+            }
+            bool finished = false;
+            while (leaf != kInvalid) {
+                float4 q0, q1, q2, q3;
+                LoadNode(cur, leaf, q0, q1, q2, q3);
+                const uint32_t prim = __float_as_uint(q1.w);
+                TriHit h;
+                if (LeafTest(boxes, r, best_t, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) &&
+                    h.t >= r.tmin && h.t <= best_t) {
+                    if (ANY) {
+                        best_t = h.t; best_prim = prim; best_inst = inst;
+                        if (OUT != kOutInstanceId) { bu = h.V / h.det; bv = h.W / h.det; }
+                        finished = true;
+                        break;
+                    }
+                    if (CloserHit(h.t, inst, prim, best_t, best_inst, best_prim)) {
+                        best_t = h.t; best_prim = prim; best_inst = inst;
+                        if (OUT != kOutInstanceId) { bu = h.V / h.det; bv = h.W / h.det; }
+                    }
+                }
+                leaf = kInvalid;
+                if (node >= cur_int && node < cur_total) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                    if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
+                }
+            }
+            if (finished || (node == kInvalid && leaf == kInvalid)) {
+                const bool hit = best_prim != kInvalid;
+                WriteHit<OUT, true>(hits, rid, hit, bu, bv, best_inst,
+                                    best_prim, best_t, best_t);
+                rid = kInvalid;
+                node = kInvalid;
+                leaf = kInvalid;
+                break;
+            }
+            if ((uint32_t)__popcll(__ballot(1)) < min_busy) break;
+        }
+    }
+}
+
 template <bool ANY, bool TWO_LEVEL, bool STATS>
 void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, const float4* rays, uint32_t count,
                const uint32_t* indirect, void* hits, uint32_t* scratch, uint32_t* stats) {
@@ -479,10 +701,11 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 lflags = rq.flags;
                 const uint8_t* accel = reinterpret_cast<const uint8_t*>(rq.accel);
                 const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
-                total = hdr->node_count;
-                n_int = hdr->prim_count ? hdr->prim_count - 1 : 0;
-                nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
-                boxes = (hdr->flags & kAccelFlagAabbs) != 0;
+                total = LdG(&hdr->node_count);
+                const uint32_t prims = LdG(&hdr->prim_count);
+                n_int = prims ? prims - 1 : 0;
+                nodes = reinterpret_cast<const BvhNode*>(accel + LdG(&hdr->nodes_offset));
+                boxes = (LdG(&hdr->flags) & kAccelFlagAabbs) != 0;
                 const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
                 tmax = ray.b.w;
@@ -616,15 +839,18 @@ bool SetLdsStackDepth(uint32_t v) {
     return true;
 }
 
-static int g_trace_mode = -1;
+static std::atomic<int> g_trace_mode{-1};
 int TraceMode() {
-    if (g_trace_mode < 0) {
+    int m = g_trace_mode.load(std::memory_order_relaxed);
+    if (m < 0) {
         const char* e = std::getenv("RR_TRACE_MODE");
-        g_trace_mode = (e && std::strcmp(e, "plain") == 0) ? kTraceModePlain : kTraceModePersistent;
+        m = (e && std::strcmp(e, "plain") == 0) ? kTraceModePlain : kTraceModePersistent;
+        int expected = -1;
+        if (!g_trace_mode.compare_exchange_strong(expected, m, std::memory_order_relaxed)) m = expected;
     }
-    return g_trace_mode;
+    return m;
 }
-void SetTraceMode(int mode) { g_trace_mode = mode; }
+void SetTraceMode(int mode) { g_trace_mode.store(mode, std::memory_order_relaxed); }
 
 namespace {
 
@@ -632,7 +858,7 @@ namespace {
 // Resident blocks of `kernel` on the current device (cached per device;
 // which = 0 for the single-request kernel, 1 for the batch kernel).
 uint32_t ResidentBlocks(const void* kernel, int which) {
-    static std::atomic<uint32_t> blocks[4][64];
+    static std::atomic<uint32_t> blocks[8][64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) dev = 0;
@@ -647,9 +873,16 @@ uint32_t ResidentBlocks(const void* kernel, int which) {
     return b;
 }
 
-// Resident blocks of the persistent kernel with the current LDS stack depth
-// (the occupancy depends on its LDS).
-uint32_t PersistentBlocks() {
+// Resident blocks of the single- or two-level persistent kernel with the
+// current LDS stack depth (the occupancy depends on its LDS).
+uint32_t PersistentBlocks(bool two) {
+    if (two) {
+        switch (LdsStackDepth()) {
+            case 8: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistent2LKernel<false, kOutFullHit, 8>), 4);
+            case 32: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistent2LKernel<false, kOutFullHit, 32>), 5);
+            default: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistent2LKernel<false, kOutFullHit, 16>), 6);
+        }
+    }
     switch (LdsStackDepth()) {
         case 8: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit, 8>), 2);
         case 32: return ResidentBlocks(reinterpret_cast<const void*>(&TracePersistentKernel<false, kOutFullHit, 32>), 3);
@@ -669,31 +902,32 @@ uint32_t FetchFor(uint32_t n, uint32_t blocks) {
     return std::min(RaysPerFetch(), f);
 }
 
-template <bool ANY, int DEPTH>
+template <bool ANY, bool TWO, int OUT, int DEPTH>
+void LaunchPersistentKernel(uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
+                            const uint32_t* ind, void* hits, uint32_t* sc, uint32_t mb, uint32_t f) {
+    if (TWO) TracePersistent2LKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
+    else TracePersistentKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
+}
+
+template <bool ANY, bool TWO, int DEPTH>
 void LaunchPersistentDepth(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r,
                            uint32_t n, const uint32_t* ind, void* hits, uint32_t* sc) {
     const uint32_t mb = MinBusyLanes(), f = FetchFor(n, blocks);
     switch (out) {
-        case kOutFullHit:
-            TracePersistentKernel<ANY, kOutFullHit, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
-            break;
-        case kOutInstanceId:
-            TracePersistentKernel<ANY, kOutInstanceId, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
-            break;
-        default:
-            TracePersistentKernel<ANY, kOutFullHitT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
-            break;
+        case kOutFullHit: LaunchPersistentKernel<ANY, TWO, kOutFullHit, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f); break;
+        case kOutInstanceId: LaunchPersistentKernel<ANY, TWO, kOutInstanceId, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f); break;
+        default: LaunchPersistentKernel<ANY, TWO, kOutFullHitT, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f); break;
     }
 }
 
-template <bool ANY>
+template <bool ANY, bool TWO>
 void LaunchPersistent(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
                       const uint32_t* ind, void* hits, uint32_t* sc) {
     QueueInitKernel<<<1, 64, 0, s>>>(sc, n, ind);
     switch (LdsStackDepth()) {
-        case 8: LaunchPersistentDepth<ANY, 8>(out, blocks, s, a, r, n, ind, hits, sc); break;
-        case 32: LaunchPersistentDepth<ANY, 32>(out, blocks, s, a, r, n, ind, hits, sc); break;
-        default: LaunchPersistentDepth<ANY, 16>(out, blocks, s, a, r, n, ind, hits, sc); break;
+        case 8: LaunchPersistentDepth<ANY, TWO, 8>(out, blocks, s, a, r, n, ind, hits, sc); break;
+        case 32: LaunchPersistentDepth<ANY, TWO, 32>(out, blocks, s, a, r, n, ind, hits, sc); break;
+        default: LaunchPersistentDepth<ANY, TWO, 16>(out, blocks, s, a, r, n, ind, hits, sc); break;
     }
 }
 
@@ -706,12 +940,17 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
     const float4* r = static_cast<const float4*>(rays);
     uint32_t* sc = static_cast<uint32_t*>(scratch);
     const bool two = kind == kAccelTlas;
-    if (!two && TraceMode() != kTraceModePlain) {
+    if (TraceMode() != kTraceModePlain) {
         // At most ceil(ray_count / kBlock) blocks: TraceScratchBytes holds
         // spill slots for that many lanes.
-        const uint32_t blocks = std::min(PersistentBlocks(), (ray_count + kBlock - 1) / kBlock);
-        if (query == kQueryAny) LaunchPersistent<true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
-        else LaunchPersistent<false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        const uint32_t blocks = std::min(PersistentBlocks(two), (ray_count + kBlock - 1) / kBlock);
+        if (two) {
+            if (query == kQueryAny) LaunchPersistent<true, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+            else LaunchPersistent<false, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        } else {
+            if (query == kQueryAny) LaunchPersistent<true, false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+            else LaunchPersistent<false, false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        }
         return;
     }
     const dim3 g((ray_count + kBlock - 1) / kBlock);

@@ -53,6 +53,7 @@ RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T = 2
 RRX_TUNING_TRACE_MIN_BUSY = 0
 RRX_TUNING_TRACE_RAYS_PER_FETCH = 1
 RRX_TUNING_TRACE_LDS_STACK = 2
+RRX_TUNING_TRACE_PLAIN = 3
 
 RRX_SERVE_BATCHED = 0
 RRX_SERVE_PLAIN = 1

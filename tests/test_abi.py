@@ -126,6 +126,12 @@ def test_tuning_validation():
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 32) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, byref(v)) == rr.RR_SUCCESS and v.value == 32
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 8) == rr.RR_SUCCESS
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, 2) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_PLAIN, byref(v)) == rr.RR_SUCCESS
+    before = v.value
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, 1) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_PLAIN, byref(v)) == rr.RR_SUCCESS and v.value == 1
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, before) == rr.RR_SUCCESS
 
 
 def test_aabb_list_requirements_and_unknown_primitive_type():

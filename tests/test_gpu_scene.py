@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from radeonrays_sdk_amd import api, scenes
+from radeonrays_sdk_amd import api, autotune, scenes
 from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, to_dev
 
 pytestmark = pytest.mark.gpu
@@ -60,3 +60,39 @@ def test_tlas_build_and_trace(ctx, dev, n_inst):
     torch.cuda.synchronize()
     r = oracle.trace(buf, rays, blas_map=host)
     assert np.array_equal(api.decode_hits(ids, "instance")["id"], r["inst_id"])
+
+
+@pytest.mark.parametrize("depth", [8, 16, 32])
+def test_persistent_two_level_equals_plain(ctx, dev, depth):
+    """The persistent two-level kernel (rays handed out from queues, leaves
+    postponed, instances entered and left mid-wave) returns exactly the plain
+    kernel's hits, for every query and output and each LDS stack depth, on a
+    scene whose instances overlap heavily (many instance entries per ray)."""
+    say(f"persistent vs plain two-level traversal, LDS stack {depth}")
+    import torch
+
+    tlas, blases, pick, xf, host = _scene(ctx, dev, 6, 500, 1, 3000, 77)
+    rng = scenes.rng_for(2, 2077)
+    n = 50000
+    o = rng.uniform(-8, 8, size=(n, 3)).astype(np.float32)
+    rays = scenes.make_rays(o, scenes.unit_sphere(n, rng))
+    rays[::7, 7] = -1.0  # some rays with an empty interval
+    drays = to_dev(rays, dev)
+    lib = ctx.lib
+    saved = autotune.current(lib)
+    try:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_LDS_STACK, depth)
+        for query in ("closest", "any"):
+            for out in ("full", "instance", "full_t"):
+                got = ctx.intersect(tlas, drays, query, out)
+                with autotune.plain_traversal(lib):
+                    ref = ctx.intersect(tlas, drays, query, out)
+                torch.cuda.synchronize()
+                assert torch.equal(got, ref), f"{query}/{out}"
+        buf = tlas.buffer.cpu().numpy()
+        hits = ctx.intersect(tlas, drays, "any", "full_t")
+        torch.cuda.synchronize()
+        ok, msg = hits_equal(api.decode_hits(hits, "full_t"), oracle.trace(buf, rays, any_hit=True, blas_map=host))
+        assert ok, msg
+    finally:
+        autotune.apply(lib, saved)
