@@ -1150,6 +1150,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
         const uint32_t n_small = (uint32_t)plan.small.size();
         rr::BatchMesh* host = nullptr;
         if (n_small) host = static_cast<rr::BatchMesh*>(AcquireStaging(c, cs, sizeof(rr::BatchMesh) * n_small));
+        uint32_t chunks = 0;
         for (uint32_t k = 0; k < n_small; ++k) {
             const rr::MeshDesc& m = plan.meshes[plan.small[k]];
             rr::BatchMesh b{};
@@ -1161,7 +1162,9 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
             b.vertex_stride = m.vertex_stride;
             b.index16 = m.index16;
             b.tri_count = m.tri_count;
-            b.pad[0] = (restructure && m.tri_count >= 64) ? rr::kAccelFlagRestructured : 0u;
+            b.flags = (restructure && m.tri_count >= 64) ? rr::kAccelFlagRestructured : 0u;
+            b.first_chunk = chunks;
+            chunks += rr::RefitChunks(m.tri_count);
             host[k] = b;
         }
         auto cmd = [=](hipStream_t s) {
@@ -1170,9 +1173,9 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
                 Check(hipMemcpyAsync(table, host, sizeof(rr::BatchMesh) * n_small, hipMemcpyHostToDevice, s),
                       "hipMemcpyAsync");
                 if (update) {
-                    rr::BatchRefit(table, n_small, scratch, s);
+                    rr::BatchRefit(table, n_small, chunks, scratch, s);
                 } else {
-                    rr::BatchBuild(table, n_small, scratch, s);
+                    rr::BatchBuild(table, n_small, chunks, scratch, s);
                     // Treelet restructuring, mesh by mesh, in each mesh's own work area.
                     if (restructure)
                         for (uint32_t k = 0; k < n_small; ++k)

@@ -23,7 +23,9 @@ struct BatchMesh {
     uint32_t vertex_stride;
     uint32_t index16;
     uint32_t tri_count;
-    uint32_t pad[4];
+    uint32_t flags;        // header flags of the result (kAccelFlag*)
+    uint32_t first_chunk;  // refit chunks (kRefitChunk leaves) of the meshes before this one
+    uint32_t pad[2];
 };
 static_assert(sizeof(BatchMesh) == 64, "BatchMesh must be 64 bytes");
 
@@ -37,15 +39,20 @@ inline size_t BatchHeadBytes(uint32_t meshes) {
     return kMeshTableOffset + AlignUp(size_t(meshes) * sizeof(BatchMesh), 256);
 }
 
+// Leaves per refit work item.
+constexpr uint32_t kRefitChunk = 256;
+inline uint32_t RefitChunks(uint32_t tris) { return (tris + kRefitChunk - 1) / kRefitChunk; }
+
 // LBVH build of every mesh in the table (already in device memory at
 // `table`, inside `scratch`): each workgroup takes the next mesh from a
-// queue and runs bounds, Morton codes, a stable four-pass radix sort,
-// Karras emission and refit for it alone.  The results equal BuildBlas with
-// restructure off, mesh by mesh.
-void BatchBuild(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t stream);
+// queue and runs bounds, Morton codes, a stable four-pass radix sort and
+// Karras emission for it alone; then one workgroup per refit chunk (of any
+// mesh) writes its leaves and climbs.  The results equal BuildBlas with
+// restructure off, mesh by mesh.  total_chunks = sum of RefitChunks.
+void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
 
 // Refit of every mesh in the table (the topology of each BLAS is kept), as
-// UpdateBlas mesh by mesh.
-void BatchRefit(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t stream);
+// UpdateBlas mesh by mesh: one workgroup per refit chunk.
+void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
 
 }  // namespace rr

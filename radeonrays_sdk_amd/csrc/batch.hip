@@ -54,8 +54,10 @@ struct BatchShared {
     uint32_t wave_hist[kWaves][kSortRadix];
 };
 
-__device__ __forceinline__ MeshView ViewOfBatch(const BatchMesh& b) {
-    return MeshView{reinterpret_cast<const uint8_t*>(b.vertices), reinterpret_cast<const void*>(b.indices),
+// The mesh's buffers, addressed from `arg` (a kernel argument) so that
+// accesses compile to global-memory instructions.
+__device__ __forceinline__ MeshView ViewOfBatch(const BatchMesh& b, const void* arg) {
+    return MeshView{RebaseOn<const uint8_t>(arg, b.vertices), RebaseOn<const void>(arg, b.indices),
                     b.vertex_stride, b.index16, b.vertex_count, b.tri_count};
 }
 
@@ -148,9 +150,9 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
         __syncthreads();
         if (mi >= meshes) break;
         const BatchMesh bm = table[mi];
-        const MeshView m = ViewOfBatch(bm);
+        const MeshView m = ViewOfBatch(bm, queue);
         const uint32_t n = bm.tri_count;
-        uint8_t* result = reinterpret_cast<uint8_t*>(bm.result);
+        uint8_t* result = RebaseOn<uint8_t>(queue, bm.result);
         AccelHeader* hdr = reinterpret_cast<AccelHeader*>(result);
         BvhNode* nodes = reinterpret_cast<BvhNode*>(result + sizeof(AccelHeader));
         if (tid == 0) {
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
             h.prim_count = n;
             h.node_count = NodeCount(n);
             h.nodes_offset = sizeof(AccelHeader);
-            h.flags = bm.pad[0];
+            h.flags = bm.flags;
             h.build_serial = serial;
             *hdr = h;
         }
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
         }
 
         // Morton codes of the triangle-box centroids, values = triangle index.
-        uint32_t* keys = reinterpret_cast<uint32_t*>(bm.work);
+        uint32_t* keys = RebaseOn<uint32_t>(queue, bm.work);
         uint32_t* vals = keys + AlignUp(n, 64);
         uint32_t* ktmp = vals + AlignUp(n, 64);
         uint32_t* vtmp = ktmp + AlignUp(n, 64);
@@ -223,31 +225,34 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
         __syncthreads();
         for (uint32_t i = tid; i < n; i += kBlock) EmitNode(keys, vals, n, i, nodes);
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += kBlock) RefitLeaf(m, nodes, i);
-        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
-                                                           uint32_t* queue) {
-    __shared__ uint32_t s_mesh;
-    for (;;) {
-        if (threadIdx.x == 0) s_mesh = atomicAdd(queue, 1u);
-        __syncthreads();
-        const uint32_t mi = s_mesh;
-        __syncthreads();
-        if (mi >= meshes) break;
-        const BatchMesh bm = table[mi];
-        const MeshView m = ViewOfBatch(bm);
-        BvhNode* nodes = reinterpret_cast<BvhNode*>(reinterpret_cast<uint8_t*>(bm.result) + sizeof(AccelHeader));
-        for (uint32_t i = threadIdx.x; i < bm.tri_count; i += kBlock) RefitLeaf(m, nodes, i);
+// Refit chunk blockIdx.x: leaves [k * kRefitChunk, (k + 1) * kRefitChunk)
+// of the mesh that owns it.  Chunks of one mesh run on many CUs at once and
+// meet in shared ancestors through ClimbRefit's agent-scope hand-off, so a
+// large mesh's refit is not one workgroup's serial loop.
+static_assert(kRefitChunk == kBlock, "one leaf per thread");
+__global__ __launch_bounds__(kBlock) void BatchRefitChunksKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                                                 uint32_t* scratch) {
+    const uint32_t c = blockIdx.x;
+    uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (table[mid].first_chunk <= c) lo = mid;
+        else hi = mid;
     }
+    const BatchMesh bm = table[lo];
+    const uint32_t i = (c - bm.first_chunk) * kRefitChunk + threadIdx.x;
+    if (i >= bm.tri_count) return;
+    const MeshView m = ViewOfBatch(bm, scratch);
+    BvhNode* nodes = reinterpret_cast<BvhNode*>(RebaseOn<uint8_t>(scratch, bm.result) + sizeof(AccelHeader));
+    RefitLeaf(m, nodes, i);
 }
 
-// Resident blocks of `kernel` on the current device, cached per device
-// (which: 0 build, 1 refit).
+// Resident blocks of `kernel` on the current device, cached per device.
 uint32_t ResidentBlocks(const void* kernel, int which) {
-    static std::atomic<uint32_t> cache[2][64];
+    static std::atomic<uint32_t> cache[1][64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) dev = 0;
@@ -268,20 +273,18 @@ size_t BatchWorkBytes(uint32_t tris) {
     return std::max(AlignUp(size_t(AlignUp(tris, 64)) * 16, 256), RestructureScratchBytes(tris));
 }
 
-void BatchBuild(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t s) {
+void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t s) {
     if (meshes == 0) return;
     uint32_t* queue = static_cast<uint32_t*>(scratch);
     BatchQueueInitKernel<<<1, 64, 0, s>>>(queue);
     const uint32_t blocks = std::min(meshes, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildKernel), 0));
     BatchBuildKernel<<<blocks, kBlock, 0, s>>>(table, meshes, queue);
+    BatchRefit(table, meshes, total_chunks, scratch, s);
 }
 
-void BatchRefit(const BatchMesh* table, uint32_t meshes, void* scratch, hipStream_t s) {
-    if (meshes == 0) return;
-    uint32_t* queue = static_cast<uint32_t*>(scratch);
-    BatchQueueInitKernel<<<1, 64, 0, s>>>(queue);
-    const uint32_t blocks = std::min(meshes, ResidentBlocks(reinterpret_cast<const void*>(&BatchRefitKernel), 1));
-    BatchRefitKernel<<<blocks, kBlock, 0, s>>>(table, meshes, queue);
+void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t s) {
+    if (meshes == 0 || total_chunks == 0) return;
+    BatchRefitChunksKernel<<<total_chunks, kBlock, 0, s>>>(table, meshes, static_cast<uint32_t*>(scratch));
 }
 
 }  // namespace rr

@@ -564,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void TlasInstanceKernel(const InstanceUploa
         table[i] = u.blas;
         float blo[3], bhi[3];
         float wlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, whi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        if (BlasRootBox(reinterpret_cast<const uint8_t*>(u.blas), blo, bhi)) {
+        if (BlasRootBox(RebaseOn<const uint8_t>(inst, u.blas), blo, bhi)) {
             float mag[3] = {0.0f, 0.0f, 0.0f};
             for (int c = 0; c < 8; ++c) {
                 const float x = (c & 4) ? bhi[0] : blo[0];
@@ -642,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void GatherSubMeshesKernel(const SubMesh* _
         else hi = mid;
     }
     const SubMesh& sm = table[lo];
-    const MeshView m{reinterpret_cast<const uint8_t*>(sm.vertices), reinterpret_cast<const void*>(sm.indices),
+    const MeshView m{RebaseOn<const uint8_t>(out, sm.vertices), RebaseOn<const void>(out, sm.indices),
                      sm.vertex_stride, sm.index16, sm.vertex_count, sm.tri_count};
     float v[9];
     LoadTriangle(m, t - sm.first_tri, v);

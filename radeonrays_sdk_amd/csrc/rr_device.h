@@ -246,6 +246,18 @@ RR_DEV void StG4(void* p, float4 v) {
     *(GlobalPtrW<RrFloat4V>)p = w;
 }
 
+// A device address read from memory (a table of meshes), re-expressed as
+// the same address reached from a kernel-argument pointer: the compiler then
+// knows it points to global memory and emits global rather than flat
+// instructions for every access through it, also in helpers that take plain
+// pointers.  Null stays null.
+template <typename T>
+RR_DEV T* RebaseOn(const void* kernel_arg, uint64_t addr) {
+    if (addr == 0) return nullptr;
+    char* b = const_cast<char*>(static_cast<const char*>(kernel_arg));
+    return reinterpret_cast<T*>(b + (int64_t)(addr - reinterpret_cast<uint64_t>(b)));
+}
+
 // Node loads as four 16-byte vector loads.
 RR_DEV void LoadNode(const BvhNode* __restrict__ nodes, uint32_t i, float4& q0, float4& q1, float4& q2, float4& q3) {
     const float4* p = reinterpret_cast<const float4*>(nodes + i);
