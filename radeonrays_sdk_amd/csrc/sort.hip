@@ -17,6 +17,7 @@
 #include "rr_device.h"
 #include "sort.h"
 
+#include <cstdio>
 #include <cstdlib>
 
 namespace rr {
@@ -275,14 +276,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// Tile shapes the pass kernel is instantiated for.
+static bool KnownShape(int block, int items) {
+    return (block == 1024 && (items == 16 || items == 8)) || (block == 512 && (items == 16 || items == 8)) ||
+           (block == 256 && (items == 16 || items == 8 || items == 4));
+}
+
 SortShape ChooseSortShape(uint32_t n) {
-    // RR_SORT_ITEMS=4|8|16 fixes the keys per thread (tuning runs).
-    static const int forced = [] {
-        const char* e = std::getenv("RR_SORT_ITEMS");
-        const int v = e ? std::atoi(e) : 0;
-        return (v == 4 || v == 8 || v == 16) ? v : 0;
+    // RR_SORT_SHAPE=<threads>x<keys per thread> (one of KnownShape) fixes
+    // the tile shape (tuning runs).
+    static const SortShape forced = [] {
+        const char* e = std::getenv("RR_SORT_SHAPE");
+        int b = 0, i = 0;
+        if (e && std::sscanf(e, "%dx%d", &b, &i) == 2 && KnownShape(b, i)) return SortShape{b, i};
+        return SortShape{0, 0};
     }();
-    if (forced) return {forced == 16 ? 1024 : kBlock, forced};
+    if (forced.block) return forced;
     // Large inputs: 16384-key tiles (fewer tiles, so fewer look-back rounds,
     // and 64-key digit runs per tile on average, i.e. whole cache lines).
     if (n >= (1u << 22)) return {1024, 16};
@@ -332,17 +341,20 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         uint32_t* nst = p + 1 < passes ? st + tiles * kSortRadix : nullptr;
         const uint32_t* h = sc.hist + p * kSortRadix;
         const uint32_t shift = (uint32_t)(p * kSortRadixBits);
-        switch (shape.items) {
-            case 16:
-                SortPassKernel<1024, 16><<<(uint32_t)tiles, 1024, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
-                break;
-            case 8:
-                SortPassKernel<kBlock, 8><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
-                break;
-            default:
-                SortPassKernel<kBlock, 4><<<(uint32_t)tiles, kBlock, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst);
-                break;
+#define RR_SORT_PASS(B, I) \
+    SortPassKernel<B, I><<<(uint32_t)tiles, B, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst)
+        if (shape.block == 1024) {
+            if (shape.items == 16) RR_SORT_PASS(1024, 16);
+            else RR_SORT_PASS(1024, 8);
+        } else if (shape.block == 512) {
+            if (shape.items == 16) RR_SORT_PASS(512, 16);
+            else RR_SORT_PASS(512, 8);
+        } else {
+            if (shape.items == 16) RR_SORT_PASS(kBlock, 16);
+            else if (shape.items == 8) RR_SORT_PASS(kBlock, 8);
+            else RR_SORT_PASS(kBlock, 4);
         }
+#undef RR_SORT_PASS
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
