@@ -88,6 +88,40 @@ def cpu_baseline(buf, rays, threads, budget_s=20.0):
             "sample": f"{reps} x (closest + any) over all {n} rays of the same BVH, oracle/oracle.cpp OpenMP"}
 
 
+def cpu_baseline_reference(ctx, accel, rays_dev, threads, side=1024):
+    """The reference's own CPU traversal (bvh_analyzer, built from its sources
+    into oracle/_ref), timed as-is on a dump of the same BVH and the first
+    side*side rays: closest hit, OpenMP over rays.  Its hits are not used (its
+    rcp and triangle-t bugs make them wrong); the time of a 1x1-ray run
+    (file reads, validity and SAH checks) is subtracted.  None when the
+    analyzer has not been built."""
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "oracle" / "_ref" / "bvh_analyzer"
+    if not exe.exists():
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        cfg = Path(d) / "config.txt"
+        ctx.write_analyzer_dump(accel, Path(d) / "bvh.bin", cfg, rays=rays_dev[: side * side], width=side,
+                                height=side, rays_path=Path(d) / "rays.bin")
+        lines = cfg.read_text().splitlines()
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+
+        def run(wd, ht):
+            cfg.write_text("\n".join(lines[:5] + [str(wd), str(ht)]) + "\n")
+            t0 = time.perf_counter()
+            subprocess.run([str(exe), str(cfg)], cwd=d, env=env, check=True, capture_output=True)
+            return time.perf_counter() - t0
+
+        fixed = min(run(1, 1) for _ in range(2))
+        full = min(run(side, side) for _ in range(2))
+    n = side * side
+    return {"value": round(n / max(full - fixed, 1e-9) / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+            "kind": "reference", "sample": f"bvh_analyzer (oracle/_ref, the reference's sources) closest hit over the "
+                                            f"first {n} rays of the same BVH; best of 2 minus a 1x1-ray run"}
+
+
 def serving_bench(ctx, dev, n_requests, log):
     """The serving stream: 32 resident scenes (log-uniform 10k-2M triangles,
     soups and heightfields), requests of log-uniform 1k-1M rays sliced from
@@ -459,6 +493,9 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         buf = accel.buffer.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(buf, w.rays, threads)
+        ref = cpu_baseline_reference(ctx, accel, rays, threads)
+        if ref is not None:
+            out["cpu_baseline_reference"] = ref
     ctx.close()
     print(json.dumps(out), flush=True)
 
