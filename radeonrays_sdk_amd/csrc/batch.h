@@ -39,20 +39,20 @@ inline size_t BatchHeadBytes(uint32_t meshes) {
     return kMeshTableOffset + AlignUp(size_t(meshes) * sizeof(BatchMesh), 256);
 }
 
-// Leaves per refit work item.
-constexpr uint32_t kRefitChunk = 256;
+// Leaves per refit queue entry.
+constexpr uint32_t kRefitChunk = 64;
 inline uint32_t RefitChunks(uint32_t tris) { return (tris + kRefitChunk - 1) / kRefitChunk; }
 
 // LBVH build of every mesh in the table (already in device memory at
 // `table`, inside `scratch`): each workgroup takes the next mesh from a
 // queue and runs bounds, Morton codes, a stable four-pass radix sort and
-// Karras emission for it alone; then one workgroup per refit chunk (of any
-// mesh) writes its leaves and climbs.  The results equal BuildBlas with
+// Karras emission for it alone; then one thread per leaf (in chunks of
+// kRefitChunk leaves of any mesh) writes the leaves and climbs.  The results equal BuildBlas with
 // restructure off, mesh by mesh.  total_chunks = sum of RefitChunks.
 void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
 
 // Refit of every mesh in the table (the topology of each BLAS is kept), as
-// UpdateBlas mesh by mesh: one workgroup per refit chunk.
+// UpdateBlas mesh by mesh (one launch over the refit chunks of all meshes).
 void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
 
 }  // namespace rr

@@ -228,14 +228,16 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
     }
 }
 
-// Refit chunk blockIdx.x: leaves [k * kRefitChunk, (k + 1) * kRefitChunk)
-// of the mesh that owns it.  Chunks of one mesh run on many CUs at once and
-// meet in shared ancestors through ClimbRefit's agent-scope hand-off, so a
-// large mesh's refit is not one workgroup's serial loop.
-static_assert(kRefitChunk == kBlock, "one leaf per thread");
-__global__ __launch_bounds__(kBlock) void BatchRefitChunksKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
-                                                                 uint32_t* scratch) {
-    const uint32_t c = blockIdx.x;
+// Refit of every mesh in the table, one thread per leaf: thread t of block b
+// takes leaf t % kRefitChunk of refit chunk b * (kBlock / kRefitChunk) +
+// t / kRefitChunk, in the mesh that owns the chunk (binary search over
+// first_chunk).  Chunks of one mesh run on many CUs at once and meet in
+// shared ancestors through ClimbRefit's agent-scope hand-off.
+static_assert(kBlock % kRefitChunk == 0, "whole chunks per block");
+__global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                                           uint32_t total_chunks, uint32_t* scratch) {
+    const uint32_t c = blockIdx.x * (kBlock / kRefitChunk) + threadIdx.x / kRefitChunk;
+    if (c >= total_chunks) return;
     uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
@@ -243,11 +245,10 @@ __global__ __launch_bounds__(kBlock) void BatchRefitChunksKernel(const BatchMesh
         else hi = mid;
     }
     const BatchMesh bm = table[lo];
-    const uint32_t i = (c - bm.first_chunk) * kRefitChunk + threadIdx.x;
-    if (i >= bm.tri_count) return;
-    const MeshView m = ViewOfBatch(bm, scratch);
+    const uint32_t i = (c - bm.first_chunk) * kRefitChunk + threadIdx.x % kRefitChunk;
+    if (c < bm.first_chunk || i >= bm.tri_count) return;
     BvhNode* nodes = reinterpret_cast<BvhNode*>(RebaseOn<uint8_t>(scratch, bm.result) + sizeof(AccelHeader));
-    RefitLeaf(m, nodes, i);
+    RefitLeaf(ViewOfBatch(bm, scratch), nodes, i);
 }
 
 // Resident blocks of `kernel` on the current device, cached per device.
@@ -284,7 +285,8 @@ void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, 
 
 void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t s) {
     if (meshes == 0 || total_chunks == 0) return;
-    BatchRefitChunksKernel<<<total_chunks, kBlock, 0, s>>>(table, meshes, static_cast<uint32_t*>(scratch));
+    const uint32_t blocks = (uint32_t)(((uint64_t)total_chunks * kRefitChunk + kBlock - 1) / kBlock);
+    BatchRefitKernel<<<blocks, kBlock, 0, s>>>(table, meshes, total_chunks, static_cast<uint32_t*>(scratch));
 }
 
 }  // namespace rr

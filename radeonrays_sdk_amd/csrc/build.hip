@@ -204,6 +204,14 @@ __global__ __launch_bounds__(kBlock) void RefitMeshKernel(MeshView m, BvhNode* _
     if (i < m.tri_count) RefitLeaf(m, nodes, i);
 }
 
+// Refit of the BLAS in `result` (nodes after the header; topology and leaf
+// links in place), one thread per leaf.
+void LaunchMeshRefit(const MeshView& m, void* result, hipStream_t s) {
+    const uint32_t n = m.tri_count;
+    if (n == 0) return;
+    RefitMeshKernel<<<Blocks(n), kBlock, 0, s>>>(m, At<BvhNode>(result, sizeof(AccelHeader)));
+}
+
 
 // ---------------------------------------------------------------------------
 // Treelet restructuring (Karras & Aila, "Fast Parallel Construction of
@@ -249,16 +257,14 @@ __device__ __forceinline__ void StU(uint32_t* p, uint32_t v) {
 }
 __device__ __forceinline__ void NodeBoxAgent(const BvhNode* __restrict__ nodes, uint32_t idx, uint32_t n_internal,
                                              float lo[3], float hi[3]) {
-    const BvhNode& nd = nodes[idx];
+    float q[4][3];
+    LoadNodeBoxesAgent(nodes + idx, q);
     for (int a = 0; a < 3; ++a) {
-        const float p = LoadAgent(&nd.a0_min_or_v0[a]), q = LoadAgent(&nd.a0_max_or_v1[a]);
-        const float r = LoadAgent(&nd.a1_min_or_v2[a]);
-        float l = fminf(fminf(p, q), r);
-        float h = fmaxf(fmaxf(p, q), r);
+        float l = fminf(fminf(q[0][a], q[1][a]), q[2][a]);
+        float h = fmaxf(fmaxf(q[0][a], q[1][a]), q[2][a]);
         if (idx < n_internal) {
-            const float w = LoadAgent(&nd.a1_max_or_v3[a]);
-            l = fminf(l, w);
-            h = fmaxf(h, w);
+            l = fminf(l, q[3][a]);
+            h = fmaxf(h, q[3][a]);
         }
         lo[a] = l;
         hi[a] = h;
@@ -432,12 +438,10 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
                                 }
                     }
                     BvhNode& nd = nodes[node];
-                    for (int a = 0; a < 3; ++a) {
-                        StoreAgent(&nd.a0_min_or_v0[a], box[0][a]);
-                        StoreAgent(&nd.a0_max_or_v1[a], box[0][3 + a]);
-                        StoreAgent(&nd.a1_min_or_v2[a], box[1][a]);
-                        StoreAgent(&nd.a1_max_or_v3[a], box[1][3 + a]);
-                    }
+                    StoreBoxAgent(nd.a0_min_or_v0, box[0][0], box[0][1], box[0][2]);
+                    StoreBoxAgent(nd.a0_max_or_v1, box[0][3], box[0][4], box[0][5]);
+                    StoreBoxAgent(nd.a1_min_or_v2, box[1][0], box[1][1], box[1][2]);
+                    StoreBoxAgent(nd.a1_max_or_v3, box[1][3], box[1][4], box[1][5]);
                     StU(&nd.child0, child[0]);
                     StU(&nd.child1, child[1]);
                     StU(&nodes[child[0]].parent, node);
@@ -463,15 +467,11 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
                     float lo[3], hi[3];
                     BvhNode& pn = nodes[parent];
                     NodeBoxAgent(nodes, LdU(&pn.child0), n_int, lo, hi);
-                    for (int a = 0; a < 3; ++a) {
-                        StoreAgent(&pn.a0_min_or_v0[a], lo[a]);
-                        StoreAgent(&pn.a0_max_or_v1[a], hi[a]);
-                    }
+                    StoreBoxAgent(pn.a0_min_or_v0, lo[0], lo[1], lo[2]);
+                    StoreBoxAgent(pn.a0_max_or_v1, hi[0], hi[1], hi[2]);
                     NodeBoxAgent(nodes, LdU(&pn.child1), n_int, lo, hi);
-                    for (int a = 0; a < 3; ++a) {
-                        StoreAgent(&pn.a1_min_or_v2[a], lo[a]);
-                        StoreAgent(&pn.a1_max_or_v3[a], hi[a]);
-                    }
+                    StoreBoxAgent(pn.a1_min_or_v2, lo[0], lo[1], lo[2]);
+                    StoreBoxAgent(pn.a1_max_or_v3, hi[0], hi[1], hi[2]);
                     s_root = parent;
                     go = 1;
                 }
@@ -697,7 +697,7 @@ void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* resu
     const uint32_t* sk = in_place ? keys : At<uint32_t>(scratch, L.keys_tmp);
     const uint32_t* sv = in_place ? vals : At<uint32_t>(scratch, L.vals_tmp);
     EmitHierarchyKernel<<<Blocks(n), kBlock, 0, s>>>(sk, sv, n, nodes);
-    RefitMeshKernel<<<Blocks(n), kBlock, 0, s>>>(m, nodes);
+    LaunchMeshRefit(m, result, s);
     if (do_restructure) RestructureBlas(n, scratch, result, s);
 }
 
@@ -757,8 +757,7 @@ void UpdateMultiMesh(const SubMesh* table, uint32_t meshes, uint32_t tris, void*
 void UpdateBlas(const MeshDesc& mesh, void* result, hipStream_t s) {
     const uint32_t n = mesh.tri_count;
     if (n == 0) return;
-    BvhNode* nodes = At<BvhNode>(result, sizeof(AccelHeader));
-    RefitMeshKernel<<<Blocks(n), kBlock, 0, s>>>(ViewOf(mesh), nodes);
+    LaunchMeshRefit(ViewOf(mesh), result, s);
 }
 
 void RestructureBlas(uint32_t n, void* scratch, void* result, hipStream_t s) {

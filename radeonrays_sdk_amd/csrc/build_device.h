@@ -118,15 +118,45 @@ __device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, con
 // written with agent-scope (write-through) stores, the wave waits for them to
 // complete (vmcnt(0)) before the counter's atomic add, and the second
 // arriver reads the sibling's slot with agent-scope loads, which do not hit
-// stale cache lines.  Both threads' accesses to a slot are agent-scope
-// atomics, so the counter orders them.
-__device__ __forceinline__ void StoreAgent(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float LoadAgent(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// stale cache lines.  Both threads' accesses to a slot carry the agent-scope
+// cache policy, and the counter's atomic orders them.
 __device__ __forceinline__ void WaitStores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// A child box side (three floats, 16-byte aligned) as one 12-byte vector
+// store or load with the cache policy of agent-scope relaxed atomic accesses
+// (sc1: written through, read past stale lines), one memory transaction per
+// side instead of three.  The loads wait for their data before returning.
+typedef float RrBox3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ void StoreBoxAgent(float* p, float x, float y, float z) {
+    RrBox3 v;
+    v.x = x; v.y = y; v.z = z;
+    asm volatile("global_store_dwordx3 %0, %1, off sc1" : : "v"(p), "v"(v) : "memory");
+}
+// The four 12-byte box fields of a node (both child boxes, or a leaf's three
+// vertices and padding), agent-scope, waited for.
+__device__ __forceinline__ void LoadNodeBoxesAgent(const BvhNode* nd, float q[4][3]) {
+    RrBox3 a, b, c, d;
+    asm volatile(
+        "global_load_dwordx3 %0, %4, off sc1\n\tglobal_load_dwordx3 %1, %4, off offset:16 sc1\n\t"
+        "global_load_dwordx3 %2, %4, off offset:32 sc1\n\tglobal_load_dwordx3 %3, %4, off offset:48 sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
+        : "v"(nd)
+        : "memory");
+    q[0][0] = a.x; q[0][1] = a.y; q[0][2] = a.z;
+    q[1][0] = b.x; q[1][1] = b.y; q[1][2] = b.z;
+    q[2][0] = c.x; q[2][1] = c.y; q[2][2] = c.z;
+    q[3][0] = d.x; q[3][1] = d.y; q[3][2] = d.z;
+}
+__device__ __forceinline__ void LoadBoxesAgent(const float* a, const float* b, float lo[3], float hi[3]) {
+    RrBox3 x, y;
+    asm volatile("global_load_dwordx3 %0, %2, off sc1\n\tglobal_load_dwordx3 %1, %3, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=&v"(x), "=&v"(y)
+                 : "v"(a), "v"(b)
+                 : "memory");
+    lo[0] = x.x; lo[1] = x.y; lo[2] = x.z;
+    hi[0] = y.x; hi[1] = y.y; hi[2] = y.z;
+}
 
 __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t child, uint32_t p,
                                            float lo[3], float hi[3]) {
@@ -139,17 +169,17 @@ __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t
         float* slot_hi = c0 == child ? pn->a0_max_or_v1 : pn->a1_max_or_v3;
         const float* other_lo = c0 == child ? pn->a1_min_or_v2 : pn->a0_min_or_v0;
         const float* other_hi = c0 == child ? pn->a1_max_or_v3 : pn->a0_max_or_v1;
-        for (int a = 0; a < 3; ++a) {
-            StoreAgent(slot_lo + a, lo[a]);
-            StoreAgent(slot_hi + a, hi[a]);
-        }
+        StoreBoxAgent(slot_lo, lo[0], lo[1], lo[2]);
+        StoreBoxAgent(slot_hi, hi[0], hi[1], hi[2]);
         WaitStores();
         const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((old & 1u) == 0) return;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        float olo[3], ohi[3];
+        LoadBoxesAgent(other_lo, other_hi, olo, ohi);
         for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], LoadAgent(other_lo + a));
-            hi[a] = fmaxf(hi[a], LoadAgent(other_hi + a));
+            lo[a] = fminf(lo[a], olo[a]);
+            hi[a] = fmaxf(hi[a], ohi[a]);
         }
         child = p;
         p = pn->parent;
