@@ -54,15 +54,18 @@ struct LaneStack {
     static_assert((DEPTH & (DEPTH - 1)) == 0, "ring depth must be a power of two");
     static_assert(DEPTH >= kMinLdsStackEntries && kStackCapacity - DEPTH <= kSpillEntries, "spill slots per lane");
     uint32_t (*lds)[kBlock];
-    uint32_t* spill;
+    uint32_t* spill;  // this block's spill slots (uniform): lane t's are [t * kSpillEntries, (t + 1) * kSpillEntries)
     int sp;     // entries on the stack
     int gbase;  // entries [0, gbase) live in scratch, [gbase, sp) in LDS
+
+    // A 32-bit lane offset from the block's base: one register, not a 64-bit pointer per lane.
+    __device__ __forceinline__ uint32_t* slot(int k) const { return spill + (threadIdx.x * (uint32_t)kSpillEntries + (uint32_t)k); }
 
     __device__ __forceinline__ bool empty() const { return sp == 0; }
     __device__ __forceinline__ void push(uint32_t v) {
         if (sp == kStackCapacity) return;  // full: drop (tree deeper than the stack)
         if (sp - gbase == DEPTH) {
-            StG(spill + gbase, lds[gbase & (DEPTH - 1)][threadIdx.x]);
+            StG(slot(gbase), lds[gbase & (DEPTH - 1)][threadIdx.x]);
             ++gbase;
         }
         lds[sp & (DEPTH - 1)][threadIdx.x] = v;
@@ -75,7 +78,7 @@ struct LaneStack {
         uint32_t v = lds[sp & (DEPTH - 1)][threadIdx.x];
         if (sp < gbase) {
             gbase = sp;
-            v = LdG(spill + sp);
+            v = LdG(slot(sp));
         }
         return v;
     }
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     float limit = WidenLimit(tmax);
     uint32_t n_internal_visits = 0, n_leaf_visits = 0, n_box_tests = 0, n_tri_tests = 0;
 
-    LaneStack<> st{s_stack, scratch + kQueueHeadWords + (size_t)i * kSpillEntries, 0, 0};
+    LaneStack<> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries, 0, 0};
     const BvhNode* nodes = top_nodes;
     uint32_t node_count = top_count;  // of the level being traversed (bounds every node index)
     uint32_t inst = kInvalid;  // kInvalid while in the top level
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries,
+    LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries,
                         0, 0};
 
     for (;;) {
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
     const BvhNode* cur = top_nodes;
     uint32_t cur_int = top_int, cur_total = top_total;
     bool boxes = false;
-    LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries,
+    LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries,
                         0, 0};
 
     for (;;) {
@@ -666,10 +669,10 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
     bool boxes = false;
     const BvhNode* nodes = nullptr;
     RayState r;
-    float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
+    float best_t = 0.0f, bu = 0.0f, bv = 0.0f;  // best_t is t_max until a hit; culling limit WidenLimit(best_t)
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack<> st{s_stack, scratch + spill_base + ((size_t)blockIdx.x * kBlock + threadIdx.x) * kSpillEntries, 0, 0};
+    LaneStack<> st{s_stack, scratch + spill_base + (size_t)blockIdx.x * kBlock * kSpillEntries, 0, 0};
 
     for (;;) {
         const uint64_t idle = __ballot(rid == kInvalid);
@@ -708,11 +711,9 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 boxes = (LdG(&hdr->flags) & kAccelFlagAabbs) != 0;
                 const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
-                tmax = ray.b.w;
-                best_t = tmax;
+                best_t = ray.b.w;
                 best_prim = kInvalid;
-                bV = 0.0f; bW = 0.0f; bdet = 1.0f;
-                limit = WidenLimit(tmax);
+                bu = 0.0f; bv = 0.0f;
                 st.sp = 0;
                 st.gbase = 0;
                 node = total ? 0u : kInvalid;
@@ -734,6 +735,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 const uint32_t c0 = __float_as_uint(q0.w);
                 const uint32_t c1 = __float_as_uint(q1.w);
                 float ta, tb;
+                const float limit = WidenLimit(best_t);
                 const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
                 const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
                 if (h0 || h1) {
@@ -764,14 +766,13 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                     h.t >= r.tmin && h.t <= best_t) {
                     if (any) {
                         best_t = h.t; best_prim = prim;
-                        bV = h.V; bW = h.W; bdet = h.det;
+                        bu = h.V / h.det; bv = h.W / h.det;
                         finished = true;
                         break;
                     }
                     if (CloserHit(h.t, 0u, prim, best_t, 0u, best_prim)) {
                         best_t = h.t; best_prim = prim;
-                        bV = h.V; bW = h.W; bdet = h.det;
-                        limit = WidenLimit(best_t);
+                        bu = h.V / h.det; bv = h.W / h.det;
                     }
                 }
                 leaf = kInvalid;
@@ -782,12 +783,11 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
             }
             if (finished || (node == kInvalid && leaf == kInvalid)) {
                 const bool hit = best_prim != kInvalid;
-                const float u = hit ? bV / bdet : 0.0f, v = hit ? bW / bdet : 0.0f;
                 void* out = reinterpret_cast<void*>(table[lreq].hits);
-                switch ((lflags >> 1) & 3u) {
-                    case kOutFullHit: WriteHit<kOutFullHit, false>(out, rid, hit, u, v, 0u, best_prim, best_t, tmax); break;
-                    case kOutInstanceId: WriteHit<kOutInstanceId, false>(out, rid, hit, u, v, 0u, best_prim, best_t, tmax); break;
-                    default: WriteHit<kOutFullHitT, false>(out, rid, hit, u, v, 0u, best_prim, best_t, tmax); break;
+                switch ((lflags >> 1) & 3u) {  // a miss leaves best_t at t_max
+                    case kOutFullHit: WriteHit<kOutFullHit, false>(out, rid, hit, bu, bv, 0u, best_prim, best_t, best_t); break;
+                    case kOutInstanceId: WriteHit<kOutInstanceId, false>(out, rid, hit, bu, bv, 0u, best_prim, best_t, best_t); break;
+                    default: WriteHit<kOutFullHitT, false>(out, rid, hit, bu, bv, 0u, best_prim, best_t, best_t); break;
                 }
                 rid = kInvalid;
                 node = kInvalid;
