@@ -257,14 +257,14 @@ class Context:
         req = self.scene_requirements(n)
         result = update.buffer if update is not None else self.empty(req.result_buffer_size)
         scratch = self.empty(req.temporary_update_buffer_size if update is not None else req.temporary_build_buffer_size)
-        with self.ptrs(*[b.address for b in blases], scratch.data_ptr(), result.data_ptr()) as hs:
-            geo, s, r = hs[:n], hs[n], hs[n + 1]
+        uniq = list(dict.fromkeys(b.address for b in blases))  # one handle per distinct BLAS
+        with self.ptrs(*uniq, scratch.data_ptr(), result.data_ptr()) as hs:
+            handle = {addr: h.value for addr, h in zip(uniq, hs[:len(uniq)])}
+            s, r = hs[len(uniq)], hs[len(uniq) + 1]
             arr = (rr.RRInstance * max(n, 1))()
-            for k in range(n):
-                arr[k].geometry = geo[k]
-                for a in range(3):
-                    for b in range(4):
-                        arr[k].transform[a][b] = float(transforms[k, a, b])
+            view = np.frombuffer(arr, dtype=_INSTANCE_DTYPE)  # the same 56-byte records
+            view["geometry"][:n] = [handle[b.address] for b in blases]
+            view["transform"][:n] = transforms
             inp = rr.RRSceneBuildInput(arr, n)
             if update is not None:
                 rr.check(self.lib.rrxCmdUpdateScene(self.ctx, byref(inp), s, r, self.cs), "rrxCmdUpdateScene")
@@ -351,6 +351,11 @@ class Context:
         with self.ptrs(keys.data_ptr(), values.data_ptr(), kt.data_ptr(), vt.data_ptr(), scratch.data_ptr()) as (
                 k, v, a, b, s):
             rr.check(self.lib.rrxCmdSortPairs(self.ctx, k, v, a, b, n, key_bits, s, self.cs), "rrxCmdSortPairs")
+
+
+# RRInstance (radeonrays.h:228-232) as a numpy record: geometry handle, 3x4 transform.
+_INSTANCE_DTYPE = np.dtype([("geometry", "<u8"), ("transform", "<f4", (3, 4))])
+assert _INSTANCE_DTYPE.itemsize == ctypes.sizeof(rr.RRInstance)
 
 
 # -- decoding helpers ------------------------------------------------------------

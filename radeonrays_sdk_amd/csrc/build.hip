@@ -248,6 +248,13 @@ __device__ __forceinline__ void NodeBox(const BvhNode* __restrict__ nodes, uint3
     }
 }
 
+// The two child boxes held in an internal node's slots, as (lo xyz, hi xyz).
+__device__ __forceinline__ void SlotBoxes(const float4& q0, const float4& q1, const float4& q2, const float4& q3,
+                                          float b0[6], float b1[6]) {
+    b0[0] = q0.x; b0[1] = q0.y; b0[2] = q0.z; b0[3] = q1.x; b0[4] = q1.y; b0[5] = q1.z;
+    b1[0] = q2.x; b1[1] = q2.y; b1[2] = q2.z; b1[3] = q3.x; b1[4] = q3.y; b1[5] = q3.z;
+}
+
 // Agent-scope node accessors for the restructuring kernel (see step 5).
 __device__ __forceinline__ uint32_t LdU(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -308,6 +315,7 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
     __shared__ uint32_t s_leaf[kTreeletLeaves];
     __shared__ uint32_t s_inner[kTreeletLeaves - 1];
     __shared__ float s_box[kTreeletLeaves][6];
+    __shared__ float s_leaf_area[kTreeletLeaves];
     __shared__ float s_area[kTreeletMasks];
     __shared__ float s_cost[kTreeletMasks];
     __shared__ uint32_t s_part[kTreeletMasks];
@@ -321,18 +329,22 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
     for (uint32_t pass = 0; pass < n; ++pass) {
         // 1. Form a treelet: repeatedly open the treelet leaf with the largest
         //    surface area (restructure_bvh.comp:170-218).
+        //    A node's two child boxes are in its own slots (exact unions,
+        //    kept current by refit and by every finished treelet), so opening
+        //    a node is one load of that node: its child links and both boxes.
         if (lane == 0) {
             const uint32_t root = s_root;
-            float area[kTreeletLeaves];
-            float lo[3], hi[3];
+            float* area = s_leaf_area;  // LDS: indexed at run time (registers would spill)
+            float (*box)[6] = s_box;
             s_inner[0] = root;
-            s_leaf[0] = LdU(&nodes[root].child0);
-            s_leaf[1] = LdU(&nodes[root].child1);
+            float4 q0, q1, q2, q3;
+            LoadNodeAgent(nodes + root, q0, q1, q2, q3);
+            s_leaf[0] = __float_as_uint(q0.w);
+            s_leaf[1] = __float_as_uint(q1.w);
+            SlotBoxes(q0, q1, q2, q3, box[0], box[1]);
+            area[0] = SurfaceArea(box[0], box[0] + 3);
+            area[1] = SurfaceArea(box[1], box[1] + 3);
             bool ok = s_leaf[0] < 2 * n - 1 && s_leaf[1] < 2 * n - 1;
-            for (int k = 0; k < 2 && ok; ++k) {
-                NodeBoxAgent(nodes, s_leaf[k], n_int, lo, hi);
-                area[k] = SurfaceArea(lo, hi);
-            }
             for (int size = 2; ok && size < kTreeletLeaves; ++size) {
                 float largest = 0.0f;
                 int sel = -1;
@@ -345,20 +357,14 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
                 if (sel < 0) { ok = false; break; }
                 const uint32_t opened = s_leaf[sel];
                 s_inner[size - 1] = opened;
-                const uint32_t c0 = LdU(&nodes[opened].child0), c1 = LdU(&nodes[opened].child1);
+                LoadNodeAgent(nodes + opened, q0, q1, q2, q3);
+                const uint32_t c0 = __float_as_uint(q0.w), c1 = __float_as_uint(q1.w);
                 if (c0 >= 2 * n - 1 || c1 >= 2 * n - 1) { ok = false; break; }
                 s_leaf[sel] = c0;
                 s_leaf[size] = c1;
-                NodeBoxAgent(nodes, c0, n_int, lo, hi);
-                area[sel] = SurfaceArea(lo, hi);
-                NodeBoxAgent(nodes, c1, n_int, lo, hi);
-                area[size] = SurfaceArea(lo, hi);
-            }
-            if (ok) {
-                for (int k = 0; k < kTreeletLeaves; ++k) {
-                    NodeBoxAgent(nodes, s_leaf[k], n_int, lo, hi);
-                    for (int a = 0; a < 3; ++a) { s_box[k][a] = lo[a]; s_box[k][3 + a] = hi[a]; }
-                }
+                SlotBoxes(q0, q1, q2, q3, box[sel], box[size]);
+                area[sel] = SurfaceArea(box[sel], box[sel] + 3);
+                area[size] = SurfaceArea(box[size], box[size] + 3);
             }
             s_go = ok ? 1u : 0u;
         }

@@ -148,6 +148,21 @@ __device__ __forceinline__ void LoadNodeBoxesAgent(const BvhNode* nd, float q[4]
     q[2][0] = c.x; q[2][1] = c.y; q[2][2] = c.z;
     q[3][0] = d.x; q[3][1] = d.y; q[3][2] = d.z;
 }
+// A whole node (four 16-byte quads), agent-scope, waited for.
+__device__ __forceinline__ void LoadNodeAgent(const BvhNode* nd, float4& q0, float4& q1, float4& q2, float4& q3) {
+    RrFloat4V a, b, c, d;
+    asm volatile(
+        "global_load_dwordx4 %0, %4, off sc1\n\tglobal_load_dwordx4 %1, %4, off offset:16 sc1\n\t"
+        "global_load_dwordx4 %2, %4, off offset:32 sc1\n\tglobal_load_dwordx4 %3, %4, off offset:48 sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
+        : "v"(nd)
+        : "memory");
+    q0 = make_float4(a.x, a.y, a.z, a.w);
+    q1 = make_float4(b.x, b.y, b.z, b.w);
+    q2 = make_float4(c.x, c.y, c.z, c.w);
+    q3 = make_float4(d.x, d.y, d.z, d.w);
+}
 __device__ __forceinline__ void LoadBoxesAgent(const float* a, const float* b, float lo[3], float hi[3]) {
     RrBox3 x, y;
     asm volatile("global_load_dwordx3 %0, %2, off sc1\n\tglobal_load_dwordx3 %1, %3, off sc1\n\ts_waitcnt vmcnt(0)"

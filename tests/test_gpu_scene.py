@@ -72,6 +72,17 @@ def test_persistent_two_level_equals_plain(ctx, dev, depth):
     import torch
 
     tlas, blases, pick, xf, host = _scene(ctx, dev, 6, 500, 1, 3000, 77)
+    # Add instances of an empty BLAS and of a one-triangle BLAS (root = leaf).
+    empty = ctx.build_geometry(to_dev(np.zeros((3, 3), np.float32), dev),
+                               to_dev(np.zeros((0, 3), np.int32), dev).reshape(0, 3))
+    v1, i1 = scenes.triangle_soup(1, scenes.rng_for(2, 3077), sigma=0.5)
+    one = ctx.build_geometry(to_dev(v1, dev), to_dev(i1.reshape(-1, 3), dev))
+    extra = [empty, one] * 20
+    xf = np.concatenate([xf, scenes.random_transforms(scenes.rng_for(2, 4077), len(extra), trans=5.0)])
+    all_blases = [blases[k] for k in pick] + extra
+    tlas = ctx.build_scene(all_blases, xf)
+    torch.cuda.synchronize()
+    host.update({b.address: b.buffer.cpu().numpy() for b in (empty, one)})
     rng = scenes.rng_for(2, 2077)
     n = 50000
     o = rng.uniform(-8, 8, size=(n, 3)).astype(np.float32)
