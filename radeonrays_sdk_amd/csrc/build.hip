@@ -255,27 +255,9 @@ __device__ __forceinline__ void SlotBoxes(const float4& q0, const float4& q1, co
     b1[0] = q2.x; b1[1] = q2.y; b1[2] = q2.z; b1[3] = q3.x; b1[4] = q3.y; b1[5] = q3.z;
 }
 
-// Agent-scope node accessors for the restructuring kernel (see step 5).
-__device__ __forceinline__ uint32_t LdU(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Agent-scope store of a node link for the restructuring kernel (see step 5).
 __device__ __forceinline__ void StU(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void NodeBoxAgent(const BvhNode* __restrict__ nodes, uint32_t idx, uint32_t n_internal,
-                                             float lo[3], float hi[3]) {
-    float q[4][3];
-    LoadNodeBoxesAgent(nodes + idx, q);
-    for (int a = 0; a < 3; ++a) {
-        float l = fminf(fminf(q[0][a], q[1][a]), q[2][a]);
-        float h = fmaxf(fmaxf(q[0][a], q[1][a]), q[2][a]);
-        if (idx < n_internal) {
-            l = fminf(l, q[3][a]);
-            h = fmaxf(h, q[3][a]);
-        }
-        lo[a] = l;
-        hi[a] = h;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void InitTreeletCountsKernel(uint32_t n, uint32_t* counts, uint32_t* counter) {
@@ -320,6 +302,7 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
     __shared__ float s_cost[kTreeletMasks];
     __shared__ uint32_t s_part[kTreeletMasks];
     __shared__ uint32_t s_root;
+    __shared__ uint32_t s_parent;
     __shared__ uint32_t s_go;
     const uint32_t lane = threadIdx.x;
     const uint32_t n_int = n - 1;
@@ -339,6 +322,7 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
             s_inner[0] = root;
             float4 q0, q1, q2, q3;
             LoadNodeAgent(nodes + root, q0, q1, q2, q3);
+            s_parent = __float_as_uint(q2.w);
             s_leaf[0] = __float_as_uint(q0.w);
             s_leaf[1] = __float_as_uint(q1.w);
             SlotBoxes(q0, q1, q2, q3, box[0], box[1]);
@@ -456,13 +440,19 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
             }
         }
         // 5. Climb: the last of the two subtrees to finish below the parent
-        //    refits it and restructures there next (restructure_bvh.comp:497-540).
+        //    restructures there next (restructure_bvh.comp:497-540).  The
+        //    reference also refits the parent's two child boxes here; those
+        //    are unchanged (a restructured treelet covers the same leaves, so
+        //    its root's box, the exact union of them, is bit for bit the one
+        //    already in the parent's slot), so this climb only hands off.
         //    Hand-off as in ClimbRefit: every node access of this kernel is
         //    an agent-scope (write-through / coherent) load or store, the
         //    wave waits for its stores before the arrival counter's relaxed
         //    atomic, and the second arriver then reads what the first wrote.
+        //    The root's parent link was read with the root in step 1 (no
+        //    other wave writes it before this hand-off).
         if (lane == 0) {
-            const uint32_t parent = LdU(&nodes[s_root].parent);
+            const uint32_t parent = s_parent;
             uint32_t go = 0;
             if (parent < n_int) {
                 WaitStores();
@@ -470,14 +460,6 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
                     __hip_atomic_fetch_add(&counts[parent], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (old != 0) {
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                    float lo[3], hi[3];
-                    BvhNode& pn = nodes[parent];
-                    NodeBoxAgent(nodes, LdU(&pn.child0), n_int, lo, hi);
-                    StoreBoxAgent(pn.a0_min_or_v0, lo[0], lo[1], lo[2]);
-                    StoreBoxAgent(pn.a0_max_or_v1, hi[0], hi[1], hi[2]);
-                    NodeBoxAgent(nodes, LdU(&pn.child1), n_int, lo, hi);
-                    StoreBoxAgent(pn.a1_min_or_v2, lo[0], lo[1], lo[2]);
-                    StoreBoxAgent(pn.a1_max_or_v3, hi[0], hi[1], hi[2]);
                     s_root = parent;
                     go = 1;
                 }

@@ -132,22 +132,6 @@ __device__ __forceinline__ void StoreBoxAgent(float* p, float x, float y, float 
     v.x = x; v.y = y; v.z = z;
     asm volatile("global_store_dwordx3 %0, %1, off sc1" : : "v"(p), "v"(v) : "memory");
 }
-// The four 12-byte box fields of a node (both child boxes, or a leaf's three
-// vertices and padding), agent-scope, waited for.
-__device__ __forceinline__ void LoadNodeBoxesAgent(const BvhNode* nd, float q[4][3]) {
-    RrBox3 a, b, c, d;
-    asm volatile(
-        "global_load_dwordx3 %0, %4, off sc1\n\tglobal_load_dwordx3 %1, %4, off offset:16 sc1\n\t"
-        "global_load_dwordx3 %2, %4, off offset:32 sc1\n\tglobal_load_dwordx3 %3, %4, off offset:48 sc1\n\t"
-        "s_waitcnt vmcnt(0)"
-        : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d)
-        : "v"(nd)
-        : "memory");
-    q[0][0] = a.x; q[0][1] = a.y; q[0][2] = a.z;
-    q[1][0] = b.x; q[1][1] = b.y; q[1][2] = b.z;
-    q[2][0] = c.x; q[2][1] = c.y; q[2][2] = c.z;
-    q[3][0] = d.x; q[3][1] = d.y; q[3][2] = d.z;
-}
 // A whole node (four 16-byte quads), agent-scope, waited for.
 __device__ __forceinline__ void LoadNodeAgent(const BvhNode* nd, float4& q0, float4& q1, float4& q2, float4& q3) {
     RrFloat4V a, b, c, d;
