@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void InitBuildKernel(AccelHeader* hdr, uint
         *hdr = h;
     }
     if (t < 8) head->bounds[t] = (t < 4) ? 0xFFFFFFFFu : 0u;
-    for (uint32_t i = t; i < kSortMaxPasses * kSortRadix; i += kBlock) sort_hist[i] = 0;
+    for (uint32_t i = t; i < kSortHistCopies * kSortHistWords; i += kBlock) sort_hist[i] = 0;
     if (t < kSortMaxPasses) sort_tickets[t] = 0;
 }
 
@@ -177,9 +177,10 @@ __device__ __forceinline__ void MortonAndHistogram(uint32_t n, const BuildHead* 
     }
     AccumulateDigits(key, valid, s_h, passes);
     __syncthreads();
+    uint32_t* copy = HistCopyOfBlock(sort_hist);
     for (uint32_t d = threadIdx.x; d < (uint32_t)passes * kSortRadix; d += kBlock) {
         const uint32_t c = (&s_h[0][0])[d];
-        if (c) atomicAdd(&sort_hist[d], c);
+        if (c) atomicAdd(&copy[d], c);
     }
     for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += (size_t)gridDim.x * kBlock)
         sort_status[w] = 0;

@@ -14,6 +14,12 @@ namespace rr {
 constexpr int kSortRadixBits = 8;
 constexpr int kSortRadix = 1 << kSortRadixBits;
 constexpr int kSortMaxPasses = 4;
+// The global digit histogram is kept as kSortHistCopies copies: a block adds
+// its counts into copy blockIdx.x % kSortHistCopies (one per XCD, as blocks
+// are dispatched round-robin), which divides the same-address atomics of
+// thousands of blocks by that factor; readers sum the copies.
+constexpr int kSortHistCopies = 8;
+constexpr int kSortHistWords = kSortMaxPasses * kSortRadix;  // one copy
 
 // Tile shape of one pass kernel.  Small inputs use small tiles so that the
 // grid still spreads over all CUs.
@@ -32,7 +38,7 @@ size_t SortScratchBytes(uint32_t n);
 __host__ __device__ inline int SortPasses(int key_bits) { return (key_bits + kSortRadixBits - 1) / kSortRadixBits; }
 
 struct SortScratch {
-    uint32_t* hist;     // [kSortMaxPasses][256] global digit counts
+    uint32_t* hist;     // [kSortHistCopies][kSortMaxPasses][256] global digit counts
     uint32_t* tickets;  // [kSortMaxPasses] tile tickets
     uint32_t* status;   // [kSortMaxPasses][tiles][256] look-back words
     size_t status_words;  // words of pass 0's look-back region, which must be zero before a sort
@@ -50,6 +56,11 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
 
 // Launch helpers exposed for the build's fused histogram.
 void LaunchSortClear(const SortScratch& sc, hipStream_t stream);
+
+// This block's copy of the global histogram.
+__device__ __forceinline__ uint32_t* HistCopyOfBlock(uint32_t* hist) {
+    return hist + (blockIdx.x % kSortHistCopies) * kSortHistWords;
+}
 
 // Accumulate the 8-bit digits of `key` for `passes` passes into LDS
 // histograms.  The top two digits of spatially ordered keys (Morton codes of

@@ -61,7 +61,7 @@ __device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* s_w
 }  // namespace
 
 __global__ __launch_bounds__(kBlock) void SortClearKernel(uint32_t* hist, uint32_t* tickets) {
-    for (uint32_t i = threadIdx.x; i < kSortMaxPasses * kSortRadix; i += kBlock) hist[i] = 0;
+    for (uint32_t i = threadIdx.x; i < kSortHistCopies * kSortHistWords; i += kBlock) hist[i] = 0;
     if (threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0;
 }
 
@@ -98,9 +98,10 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
         AccumulateDigits<false>(valid ? keys[i] : 0u, valid, s_h, passes);
     }
     __syncthreads();
+    uint32_t* copy = HistCopyOfBlock(hist);
     for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kSortRadix; i += kBlock) {
         const uint32_t c = (&s_h[0][0])[i];
-        if (c) atomicAdd(&hist[i], c);
+        if (c) atomicAdd(&copy[i], c);
     }
     for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += (size_t)gridDim.x * kBlock)
         status[w] = 0;
@@ -135,8 +136,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (tid == 0) s_tile = atomicAdd(tickets, 1u);
     for (uint32_t i = tid; i < kWaves * kSortRadix; i += BLOCK) (&s_wave_hist[0][0])[i] = 0;
     if (digit_thread) s_count[tid] = 0;
-    // Global exclusive digit offsets from the upfront histogram.
-    const uint32_t hist_d = digit_thread ? hist[tid] : 0u;
+    // Global exclusive digit offsets from the upfront histogram (the sum of its copies).
+    uint32_t hist_d = 0;
+    if (digit_thread)
+#pragma unroll
+        for (int c = 0; c < kSortHistCopies; ++c) hist_d += hist[c * kSortHistWords + tid];
     const uint32_t hist_scan = BlockExclusiveScan<BLOCK>(hist_d, s_wave_tot);  // contains __syncthreads
     const uint32_t tile = s_tile;
     const uint32_t base = tile * kTile + wave * (64 * ITEMS) + lane;  // n <= 2^30: 32-bit indices
@@ -305,7 +309,7 @@ static size_t SortTiles(uint32_t n) {
 }
 
 size_t SortScratchBytes(uint32_t n) {
-    const size_t words = kSortMaxPasses * kSortRadix + 64 + (size_t)kSortMaxPasses * SortTiles(n) * kSortRadix;
+    const size_t words = kSortHistCopies * kSortHistWords + 64 + (size_t)kSortMaxPasses * SortTiles(n) * kSortRadix;
     return AlignUp(words * 4, 256);
 }
 
@@ -313,7 +317,7 @@ SortScratch CarveSortScratch(void* base, uint32_t n) {
     SortScratch s;
     uint32_t* p = static_cast<uint32_t*>(base);
     s.hist = p;
-    s.tickets = p + kSortMaxPasses * kSortRadix;
+    s.tickets = p + kSortHistCopies * kSortHistWords;
     s.status = s.tickets + 64;
     s.status_words = SortTiles(n) * kSortRadix;  // pass 0's; each pass clears the next pass's
     return s;
