@@ -46,17 +46,19 @@ RR_API RRError rrxCmdIntersectWithStats(RRContext        context,
                                         RRDevicePtr      stats,
                                         RRCommandStream  command_stream);
 
-/* Process-wide tuning of the persistent traversal kernel (the autotuner in
- * radeonrays_sdk_amd/autotune.py sweeps these).  Results do not depend on
- * them, only speed does. */
+/* Process-wide tuning of the persistent traversal kernel and of the sort
+ * (the autotuner in radeonrays_sdk_amd/autotune.py sweeps these).  Results
+ * do not depend on them, only speed does. */
 typedef enum RRxTuning
 {
     RRX_TUNING_TRACE_MIN_BUSY = 0,        /* refill a wave once fewer lanes traverse (1..64; default 24) */
     RRX_TUNING_TRACE_RAYS_PER_FETCH = 1,  /* ray indices per queue atomic (32, 64, 128; default 128) */
     RRX_TUNING_TRACE_LDS_STACK = 2,       /* LDS stack entries per lane (8, 16, 32; default 8) */
-    RRX_TUNING_TRACE_PLAIN = 3            /* 1: one lane per ray for its whole traversal, no
+    RRX_TUNING_TRACE_PLAIN = 3,           /* 1: one lane per ray for its whole traversal, no
                                              persistent kernels (default 0, or 1 when the
                                              environment sets RR_TRACE_MODE=plain) */
+    RRX_TUNING_SORT_DIGIT_BITS = 4        /* digit width of rrxCmdSortPairs on >= 2^20 keys:
+                                             8 (default) or 10 (three passes for 30-bit keys) */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

@@ -7,10 +7,14 @@ threshold (lanes still traversing below which a wave fetches new rays),
 checks that every combination returns the same hits as the current setting,
 and leaves the fastest one set.
 
-Not swept, with the reason: the radix digit width (8 bits; 4-bit digits
-double the passes over memory for a sort that is bandwidth-bound) and the
-treelet size (7 leaves; the treelet kernels' subset dynamic programme is
-written for 2^7 masks and 9 leaves would take 4x the LDS per treelet).
+The sort's radix digit width is swept by tune_sort: 8-bit digits (four
+passes over 30-bit Morton codes) or 10-bit digits (three passes, 1024-bin
+tiles), on a caller-given number of keys; both give the same stable order.
+
+Not swept, with the reason: 4-bit digits (they double the passes over
+memory of a sort that is bandwidth-bound) and the treelet size (7 leaves;
+the treelet kernels' subset dynamic programme is written for 2^7 masks and
+9 leaves would take 4x the LDS per treelet).
 """
 from __future__ import annotations
 
@@ -108,4 +112,59 @@ def tune_traversal(ctx, accel, rays, queries=("closest", "any"), reps: int = 5, 
     best = min(table, key=lambda r: r["ms"])
     default = next((r for r in table if all(r[k] == start[k] for k in start)), None)
     apply(lib, {k: best[k] for k in start})
+    return {"best": best, "default": default, "table": table}
+
+
+DIGIT_BITS = (8, 10)
+
+
+def tune_sort(ctx, n: int = 16 << 20, reps: int = 5, digit_bits=DIGIT_BITS, seed: int = 0x52520005) -> dict:
+    """Sweep the radix digit width on a sort of n random 30-bit keys with
+    index values (rrxCmdSortPairs); returns {"best", "default", "table"} and
+    leaves the fastest width set.  Raises if any width's output differs."""
+    import torch
+
+    lib = ctx.lib
+    start = get(lib, rr.RRX_TUNING_SORT_DIGIT_BITS)
+    dev = ctx.torch_device
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    keys = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, generator=g).to(dev)
+    vals = torch.arange(n, dtype=torch.int32, device=dev)
+    k2, v2 = keys.clone(), vals.clone()
+    tmp = (torch.empty_like(keys), torch.empty_like(vals))
+
+    def run():
+        k2.copy_(keys)
+        v2.copy_(vals)
+        ctx.sort_pairs(k2, v2, 30, tmp=tmp)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+    copy_ms = timed(lambda: (k2.copy_(keys), v2.copy_(vals)))
+    reference = None
+    table = []
+    for bits in digit_bits:
+        set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, bits)
+        run()
+        torch.cuda.synchronize()
+        got = (k2.cpu().numpy(), v2.cpu().numpy())
+        if reference is None:
+            reference = got
+        elif not (np.array_equal(got[0], reference[0]) and np.array_equal(got[1], reference[1])):
+            set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, start)
+            raise RuntimeError(f"sort output differs with {bits}-bit digits")
+        ms = timed(run) - copy_ms
+        table.append({"digit_bits": bits, "ms": round(ms, 4)})
+    best = min(table, key=lambda r: r["ms"])
+    default = next((r for r in table if r["digit_bits"] == start), None)
+    set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, best["digit_bits"])
     return {"best": best, "default": default, "table": table}

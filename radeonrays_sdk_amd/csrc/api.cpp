@@ -1209,6 +1209,8 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             RR_REQUIRE(value <= 1, "rrxSetTuning");
             rr::SetTraceMode(value ? rr::kTraceModePlain : rr::kTraceModePersistent);
             return RR_SUCCESS;
+        case RRX_TUNING_SORT_DIGIT_BITS:
+            return rr::SetSortDigitBits((int)value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1220,6 +1222,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_TRACE_RAYS_PER_FETCH: *value = rr::RaysPerFetch(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_LDS_STACK: *value = rr::LdsStackDepth(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_PLAIN: *value = rr::TraceMode() == rr::kTraceModePlain ? 1u : 0u; return RR_SUCCESS;
+        case RRX_TUNING_SORT_DIGIT_BITS: *value = (uint32_t)rr::SortDigitBits(); return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

@@ -99,7 +99,8 @@ __global__ __launch_bounds__(kBlock) void InitBuildKernel(AccelHeader* hdr, uint
         *hdr = h;
     }
     if (t < 8) head->bounds[t] = (t < 4) ? 0xFFFFFFFFu : 0u;
-    for (uint32_t i = t; i < kSortHistCopies * kSortHistWords; i += kBlock) sort_hist[i] = 0;
+    for (uint32_t i = t; i < kSortHistCopies * kSortHistWords; i += kBlock)
+        sort_hist[(i / kSortHistWords) * kSortHistStride + i % kSortHistWords] = 0;
     if (t < kSortMaxPasses) sort_tickets[t] = 0;
 }
 

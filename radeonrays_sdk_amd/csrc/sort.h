@@ -19,16 +19,27 @@ constexpr int kSortMaxPasses = 4;
 // are dispatched round-robin), which divides the same-address atomics of
 // thousands of blocks by that factor; readers sum the copies.
 constexpr int kSortHistCopies = 8;
-constexpr int kSortHistWords = kSortMaxPasses * kSortRadix;  // one copy
+constexpr int kSortHistWords = kSortMaxPasses * kSortRadix;  // 8-bit digits: words used in each copy
+// Large sorts may use 10-bit digits (three passes for 30-bit keys); a copy
+// is laid out for the larger of the two.
+constexpr int kSortWideBits = 10;
+constexpr int kSortWidePasses = 3;
+constexpr int kSortHistStride = kSortWidePasses << kSortWideBits;  // words per copy (3072 >= 1024)
 
 // Tile shape of one pass kernel.  Small inputs use small tiles so that the
 // grid still spreads over all CUs.
 struct SortShape {
     int block;   // threads per tile
     int items;   // keys per thread
+    int bits;    // digit width: 8, or 10 (1024-thread tiles)
     int tile() const { return block * items; }
 };
 SortShape ChooseSortShape(uint32_t n);
+
+// Digit width of sorts of at least 2^20 keys whose callers did not prepare
+// the histogram (rrxCmdSortPairs): 8 (default) or 10.  Process-wide.
+int SortDigitBits();
+bool SetSortDigitBits(int bits);
 
 // Scratch needed to sort n pairs: histograms, tile tickets and look-back
 // status words.  Everything in it is (re)initialised by the sort itself.
@@ -59,7 +70,7 @@ void LaunchSortClear(const SortScratch& sc, hipStream_t stream);
 
 // This block's copy of the global histogram.
 __device__ __forceinline__ uint32_t* HistCopyOfBlock(uint32_t* hist) {
-    return hist + (blockIdx.x % kSortHistCopies) * kSortHistWords;
+    return hist + (blockIdx.x % kSortHistCopies) * kSortHistStride;
 }
 
 // Accumulate the 8-bit digits of `key` for `passes` passes into LDS

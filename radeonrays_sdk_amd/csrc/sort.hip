@@ -17,6 +17,8 @@
 #include "rr_device.h"
 #include "sort.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 
@@ -60,20 +62,30 @@ __device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* s_w
 
 }  // namespace
 
-__global__ __launch_bounds__(kBlock) void SortClearKernel(uint32_t* hist, uint32_t* tickets) {
-    for (uint32_t i = threadIdx.x; i < kSortHistCopies * kSortHistWords; i += kBlock) hist[i] = 0;
-    if (threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0;
+// Clears the first `words` words of every histogram copy, and the tickets.
+__global__ __launch_bounds__(kBlock) void SortClearKernel(uint32_t* hist, uint32_t* tickets, uint32_t words) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < kSortHistCopies * words; i += gridDim.x * kBlock)
+        hist[(i / words) * kSortHistStride + i % words] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0;
 }
 
 // Digit counts of every pass in one read of the keys: each thread takes 16
 // consecutive keys (four 16-byte loads) per step of a grid-stride loop.
 constexpr int kHistKeysPerThread = 16;
 
+// Digits of `key` for `passes` passes of BITS-bit digits, counted in LDS.
+template <int BITS>
+__device__ __forceinline__ void CountDigits(uint32_t key, uint32_t* s_h, int passes) {
+    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[(p << BITS) | ((key >> (p * BITS)) & ((1u << BITS) - 1))], 1u);
+}
+
+template <int BITS>
 __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __restrict__ keys, uint32_t n, int passes,
                                                               uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
                                                               size_t status_words) {
-    __shared__ uint32_t s_h[kSortMaxPasses][kSortRadix];
-    for (uint32_t i = threadIdx.x; i < kSortMaxPasses * kSortRadix; i += kBlock) (&s_h[0][0])[i] = 0;
+    constexpr int kWords = (BITS == 8 ? kSortMaxPasses : kSortWidePasses) << BITS;
+    __shared__ uint32_t s_h[kWords];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kWords; i += kBlock) s_h[i] = 0;
     __syncthreads();
     constexpr uint32_t kStep = kBlock * kHistKeysPerThread;
     // Steps whose keys are all in range (none when keys is not 16-byte aligned).
@@ -85,22 +97,19 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
         for (int j = 0; j < kHistKeysPerThread / 4; ++j) q[j] = src[j * kBlock + threadIdx.x];
 #pragma unroll
         for (int j = 0; j < kHistKeysPerThread / 4; ++j) {
-            AccumulateDigits<false>(q[j].x, true, s_h, passes);
-            AccumulateDigits<false>(q[j].y, true, s_h, passes);
-            AccumulateDigits<false>(q[j].z, true, s_h, passes);
-            AccumulateDigits<false>(q[j].w, true, s_h, passes);
+            CountDigits<BITS>(q[j].x, s_h, passes);
+            CountDigits<BITS>(q[j].y, s_h, passes);
+            CountDigits<BITS>(q[j].z, s_h, passes);
+            CountDigits<BITS>(q[j].w, s_h, passes);
         }
     }
-    // The rest, one key per lane (whole waves iterate together).
-    for (uint32_t b = full * kStep + blockIdx.x * kBlock + (threadIdx.x & ~63u); b < n; b += gridDim.x * kBlock) {
-        const uint32_t i = b + __lane_id();
-        const bool valid = i < n;
-        AccumulateDigits<false>(valid ? keys[i] : 0u, valid, s_h, passes);
-    }
+    // The rest, one key per thread.
+    for (uint32_t i = full * kStep + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        CountDigits<BITS>(keys[i], s_h, passes);
     __syncthreads();
     uint32_t* copy = HistCopyOfBlock(hist);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)passes * kSortRadix; i += kBlock) {
-        const uint32_t c = (&s_h[0][0])[i];
+    for (uint32_t i = threadIdx.x; i < ((uint32_t)passes << BITS); i += kBlock) {
+        const uint32_t c = s_h[i];
         if (c) atomicAdd(&copy[i], c);
     }
     for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += (size_t)gridDim.x * kBlock)
@@ -114,33 +123,38 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
 // while this tile is still ranking.  Four waves per SIMD: with 16384-key
 // tiles (1024 threads x 16 keys) one tile fills a CU's LDS and at most 128
 // VGPRs keep all 16 waves resident.
-template <int BLOCK, int ITEMS>
+//
+// BITS is the digit width: 8, or 10 with 1024-thread tiles (three passes
+// for 30-bit keys instead of four; digit runs get shorter, so tiles stay
+// large).
+template <int BLOCK, int ITEMS, int BITS = kSortRadixBits>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, const uint32_t* __restrict__ hist, uint32_t* tickets,
     uint32_t* status, uint32_t* next_status) {
-    static_assert(BLOCK >= kSortRadix && BLOCK % 64 == 0, "one thread per digit");
+    constexpr int kRadix = 1 << BITS;
+    static_assert(BLOCK >= kRadix && BLOCK % 64 == 0, "one thread per digit");
     constexpr int kWaves = BLOCK / 64;
     constexpr int kTile = BLOCK * ITEMS;
     __shared__ uint32_t s_buf[kTile];
     __shared__ uint32_t s_vbuf[kTile];
-    __shared__ uint32_t s_wave_hist[kWaves][kSortRadix];
-    __shared__ uint32_t s_count[kSortRadix];
-    __shared__ uint32_t s_digit_start[kSortRadix];
-    __shared__ uint32_t s_global_off[kSortRadix];
+    __shared__ uint32_t s_wave_hist[kWaves][kRadix];
+    __shared__ uint32_t s_count[kRadix];
+    __shared__ uint32_t s_digit_start[kRadix];
+    __shared__ uint32_t s_global_off[kRadix];
     __shared__ uint32_t s_wave_tot[kWaves];
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const bool digit_thread = tid < (uint32_t)kSortRadix;  // thread d handles digit d
+    const bool digit_thread = tid < (uint32_t)kRadix;  // thread d handles digit d
     if (tid == 0) s_tile = atomicAdd(tickets, 1u);
-    for (uint32_t i = tid; i < kWaves * kSortRadix; i += BLOCK) (&s_wave_hist[0][0])[i] = 0;
+    for (uint32_t i = tid; i < kWaves * kRadix; i += BLOCK) (&s_wave_hist[0][0])[i] = 0;
     if (digit_thread) s_count[tid] = 0;
     // Global exclusive digit offsets from the upfront histogram (the sum of its copies).
     uint32_t hist_d = 0;
     if (digit_thread)
 #pragma unroll
-        for (int c = 0; c < kSortHistCopies; ++c) hist_d += hist[c * kSortHistWords + tid];
+        for (int c = 0; c < kSortHistCopies; ++c) hist_d += hist[c * kSortHistStride + tid];
     const uint32_t hist_scan = BlockExclusiveScan<BLOCK>(hist_d, s_wave_tot);  // contains __syncthreads
     const uint32_t tile = s_tile;
     const uint32_t base = tile * kTile + wave * (64 * ITEMS) + lane;  // n <= 2^30: 32-bit indices
@@ -160,7 +174,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // The tile's digit counts.  Padding keys (all ones) are counted in digit
     // 255 and are ranked last, so they never displace a real key.
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) atomicAdd(&s_count[(k[j] >> shift) & (kSortRadix - 1)], 1u);
+    for (int j = 0; j < ITEMS; ++j) atomicAdd(&s_count[(k[j] >> shift) & (kRadix - 1)], 1u);
     __syncthreads();
 
     // Decoupled look-back for digit `tid`, kLookback predecessors at a time:
@@ -173,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         if (tile == 0) {
             AtomicStoreAgent(&status[d], kFlagInclusive | count);
         } else {
-            AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagAggregate | count);
+            AtomicStoreAgent(&status[(size_t)tile * kRadix + d], kFlagAggregate | count);
             int64_t p = (int64_t)tile - 1;
             uint32_t spins = 0;
             bool done = false;
@@ -181,7 +195,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 uint32_t w[kLookback];
 #pragma unroll
                 for (int q = 0; q < kLookback; ++q)
-                    w[q] = (p - q >= 0) ? AtomicLoadAgent(&status[(size_t)(p - q) * kSortRadix + d]) : kFlagInclusive;
+                    w[q] = (p - q >= 0) ? AtomicLoadAgent(&status[(size_t)(p - q) * kRadix + d]) : kFlagInclusive;
                 int used = 0;
                 bool blocked = false;
 #pragma unroll
@@ -202,11 +216,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
-            AtomicStoreAgent(&status[(size_t)tile * kSortRadix + d], kFlagInclusive | (excl + count));
+            AtomicStoreAgent(&status[(size_t)tile * kRadix + d], kFlagInclusive | (excl + count));
         }
         s_global_off[d] = hist_scan + excl;
         // The next pass finds this tile's look-back words cleared.
-        if (next_status) next_status[(size_t)tile * kSortRadix + d] = 0;
+        if (next_status) next_status[(size_t)tile * kRadix + d] = 0;
     }
 
     // Rank within each wave: lanes holding the same digit are the AND over
@@ -214,11 +228,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // selects which), one three-input bit operation per half.
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
+        const uint32_t d = (k[j] >> shift) & (kRadix - 1);
         const uint32_t pre = s_wave_hist[wave][d];
         uint32_t m_lo = ~0u, m_hi = ~0u;
 #pragma unroll
-        for (int b = 0; b < kSortRadixBits; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)k[j], shift + b, 1);  // 0 or ~0
             const uint64_t bal = __ballot(neg != 0u);
             m_lo &= ~((uint32_t)bal ^ neg);
@@ -261,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // contiguously.
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
+        const uint32_t d = (k[j] >> shift) & (kRadix - 1);
         const uint32_t pos = s_digit_start[d] + s_wave_hist[wave][d] + rank[j];
         s_buf[pos] = k[j];
         s_vbuf[pos] = v[j];
@@ -271,7 +285,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t i = (uint32_t)j * BLOCK + tid;
         const uint32_t key = s_buf[i];
-        const uint32_t d = (key >> shift) & (kSortRadix - 1);
+        const uint32_t d = (key >> shift) & (kRadix - 1);
         const uint32_t gpos = s_global_off[d] + (i - s_digit_start[d]);  // n <= 2^30: fits in 32 bits
         if (gpos < n) {
             keys_out[gpos] = key;
@@ -281,35 +295,59 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 }
 
 // Tile shapes the pass kernel is instantiated for.
-static bool KnownShape(int block, int items) {
-    return (block == 1024 && (items == 16 || items == 8)) || (block == 512 && (items == 16 || items == 8)) ||
-           (block == 256 && (items == 16 || items == 8 || items == 4));
+static bool KnownShape(int block, int items, int bits) {
+    if (bits == kSortWideBits) return block == 1024 && (items == 8 || items == 4);
+    return bits == 8 && ((block == 1024 && (items == 16 || items == 8)) || (block == 512 && (items == 16 || items == 8)) ||
+                         (block == 256 && (items == 16 || items == 8 || items == 4)));
 }
 
-SortShape ChooseSortShape(uint32_t n) {
-    // RR_SORT_SHAPE=<threads>x<keys per thread> (one of KnownShape) fixes
-    // the tile shape (tuning runs).
+static std::atomic<int> g_digit_bits{8};
+int SortDigitBits() { return g_digit_bits.load(std::memory_order_relaxed); }
+bool SetSortDigitBits(int bits) {
+    if (bits != 8 && bits != kSortWideBits) return false;
+    g_digit_bits.store(bits, std::memory_order_relaxed);
+    return true;
+}
+
+// Inputs at least this large may take 10-bit digits (8192-key tiles).
+constexpr uint32_t kWideMinKeys = 1u << 20;
+
+// Shape of a sort of n keys.  wide_ok: 10-bit digits are allowed (not for
+// callers that prepared an 8-bit histogram themselves).
+static SortShape ChooseShape(uint32_t n, bool wide_ok) {
+    // RR_SORT_SHAPE=<threads>x<keys per thread>[x<digit bits>] (one of
+    // KnownShape) fixes the shape of sorts that allow it (tuning runs).
     static const SortShape forced = [] {
         const char* e = std::getenv("RR_SORT_SHAPE");
-        int b = 0, i = 0;
-        if (e && std::sscanf(e, "%dx%d", &b, &i) == 2 && KnownShape(b, i)) return SortShape{b, i};
-        return SortShape{0, 0};
+        int b = 0, i = 0, bits = 8;
+        if (e && std::sscanf(e, "%dx%dx%d", &b, &i, &bits) >= 2 && KnownShape(b, i, bits)) return SortShape{b, i, bits};
+        return SortShape{0, 0, 8};
     }();
-    if (forced.block) return forced;
+    if (forced.block && (wide_ok || forced.bits == 8)) return forced;
+    if (wide_ok && n >= kWideMinKeys && SortDigitBits() == kSortWideBits) return {1024, 8, kSortWideBits};
     // Large inputs: 16384-key tiles (fewer tiles, so fewer look-back rounds,
     // and 64-key digit runs per tile on average, i.e. whole cache lines).
-    if (n >= (1u << 22)) return {1024, 16};
-    if (n >= (1u << 19)) return {kBlock, 8};
-    return {kBlock, 4};
+    if (n >= (1u << 22)) return {1024, 16, 8};
+    if (n >= (1u << 19)) return {kBlock, 8, 8};
+    return {kBlock, 4, 8};
 }
 
-static size_t SortTiles(uint32_t n) {
-    const SortShape s = ChooseSortShape(n);
-    return n ? ((size_t)n + s.tile() - 1) / s.tile() : 1;
+SortShape ChooseSortShape(uint32_t n) { return ChooseShape(n, true); }
+
+static size_t TilesOf(uint32_t n, const SortShape& s) { return n ? ((size_t)n + s.tile() - 1) / s.tile() : 1; }
+static int PassesOf(int key_bits, const SortShape& s) { return (key_bits + s.bits - 1) / s.bits; }
+
+// Look-back words of all passes for one shape (32-bit keys).
+static size_t StatusWords(uint32_t n, const SortShape& s) {
+    return (size_t)PassesOf(32, s) * TilesOf(n, s) * ((size_t)1 << s.bits);
 }
 
 size_t SortScratchBytes(uint32_t n) {
-    const size_t words = kSortHistCopies * kSortHistWords + 64 + (size_t)kSortMaxPasses * SortTiles(n) * kSortRadix;
+    // Enough for any shape a sort of n keys may take, whatever the digit
+    // width is set to when it runs.
+    size_t status = std::max(StatusWords(n, ChooseShape(n, true)), StatusWords(n, ChooseShape(n, false)));
+    if (n >= kWideMinKeys) status = std::max(status, StatusWords(n, SortShape{1024, 8, kSortWideBits}));
+    const size_t words = (size_t)kSortHistCopies * kSortHistStride + 64 + status;
     return AlignUp(words * 4, 256);
 }
 
@@ -317,46 +355,59 @@ SortScratch CarveSortScratch(void* base, uint32_t n) {
     SortScratch s;
     uint32_t* p = static_cast<uint32_t*>(base);
     s.hist = p;
-    s.tickets = p + kSortHistCopies * kSortHistWords;
+    s.tickets = p + kSortHistCopies * kSortHistStride;
     s.status = s.tickets + 64;
-    s.status_words = SortTiles(n) * kSortRadix;  // pass 0's; each pass clears the next pass's
+    // Pass 0's look-back words for 8-bit digits (what callers that build the
+    // histogram themselves clear); each pass clears the next pass's.
+    const SortShape narrow = ChooseShape(n, false);
+    s.status_words = TilesOf(n, narrow) * ((size_t)1 << narrow.bits);
     return s;
 }
 
 void LaunchSortClear(const SortScratch& sc, hipStream_t stream) {
-    SortClearKernel<<<1, kBlock, 0, stream>>>(sc.hist, sc.tickets);
+    SortClearKernel<<<1, kBlock, 0, stream>>>(sc.hist, sc.tickets, kSortHistWords);
 }
 
 bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
                     const SortScratch& sc, bool hist_ready, hipStream_t stream) {
     if (n <= 1) return true;
-    const int passes = SortPasses(key_bits);
+    const SortShape shape = ChooseShape(n, !hist_ready);
+    const int passes = PassesOf(key_bits, shape);
+    const size_t tiles = TilesOf(n, shape);
+    const size_t radix = (size_t)1 << shape.bits;
     if (!hist_ready) {
-        LaunchSortClear(sc, stream);
+        const uint32_t words = (uint32_t)(passes * radix);
+        SortClearKernel<<<(kSortHistCopies * words + kBlock - 1) / kBlock, kBlock, 0, stream>>>(sc.hist, sc.tickets,
+                                                                                                words);
         const uint32_t blocks =
             (uint32_t)std::max<size_t>(1, std::min<size_t>(1024, (size_t)n / (kBlock * kHistKeysPerThread)));
-        SortHistogramKernel<<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, sc.status_words);
+        const size_t status0 = tiles * radix;
+        if (shape.bits == kSortWideBits)
+            SortHistogramKernel<kSortWideBits><<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, status0);
+        else
+            SortHistogramKernel<8><<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, status0);
     }
-    const SortShape shape = ChooseSortShape(n);
-    const size_t tiles = SortTiles(n);
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
     for (int p = 0; p < passes; ++p) {
-        uint32_t* st = sc.status + (size_t)p * tiles * kSortRadix;
-        uint32_t* nst = p + 1 < passes ? st + tiles * kSortRadix : nullptr;
-        const uint32_t* h = sc.hist + p * kSortRadix;
-        const uint32_t shift = (uint32_t)(p * kSortRadixBits);
-#define RR_SORT_PASS(B, I) \
-    SortPassKernel<B, I><<<(uint32_t)tiles, B, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst)
-        if (shape.block == 1024) {
-            if (shape.items == 16) RR_SORT_PASS(1024, 16);
-            else RR_SORT_PASS(1024, 8);
+        uint32_t* st = sc.status + (size_t)p * tiles * radix;
+        uint32_t* nst = p + 1 < passes ? st + tiles * radix : nullptr;
+        const uint32_t* h = sc.hist + p * radix;
+        const uint32_t shift = (uint32_t)(p * shape.bits);
+#define RR_SORT_PASS(B, I, D) \
+    SortPassKernel<B, I, D><<<(uint32_t)tiles, B, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst)
+        if (shape.bits == kSortWideBits) {
+            if (shape.items == 8) RR_SORT_PASS(1024, 8, kSortWideBits);
+            else RR_SORT_PASS(1024, 4, kSortWideBits);
+        } else if (shape.block == 1024) {
+            if (shape.items == 16) RR_SORT_PASS(1024, 16, 8);
+            else RR_SORT_PASS(1024, 8, 8);
         } else if (shape.block == 512) {
-            if (shape.items == 16) RR_SORT_PASS(512, 16);
-            else RR_SORT_PASS(512, 8);
+            if (shape.items == 16) RR_SORT_PASS(512, 16, 8);
+            else RR_SORT_PASS(512, 8, 8);
         } else {
-            if (shape.items == 16) RR_SORT_PASS(kBlock, 16);
-            else if (shape.items == 8) RR_SORT_PASS(kBlock, 8);
-            else RR_SORT_PASS(kBlock, 4);
+            if (shape.items == 16) RR_SORT_PASS(kBlock, 16, 8);
+            else if (shape.items == 8) RR_SORT_PASS(kBlock, 8, 8);
+            else RR_SORT_PASS(kBlock, 4, 8);
         }
 #undef RR_SORT_PASS
         std::swap(ki, ko);

@@ -813,16 +813,19 @@ size_t TraceScratchBytes(uint32_t ray_count) {
     return AlignUp((size_t)kQueueHeadWords * 4 + AlignUp((size_t)ray_count, kBlock) * kSpillEntries * 4, 256);
 }
 
-static uint32_t g_min_busy = 0;
+static std::atomic<uint32_t> g_min_busy{0};
 uint32_t MinBusyLanes() {
-    if (!g_min_busy) {
+    uint32_t m = g_min_busy.load(std::memory_order_relaxed);
+    if (!m) {
         const char* e = std::getenv("RR_TRACE_MIN_BUSY");
         const int v = e ? std::atoi(e) : 0;
-        g_min_busy = (v >= 1 && v <= 64) ? (uint32_t)v : kDefaultMinBusy;
+        uint32_t expected = 0;
+        m = (v >= 1 && v <= 64) ? (uint32_t)v : kDefaultMinBusy;
+        if (!g_min_busy.compare_exchange_strong(expected, m, std::memory_order_relaxed)) m = expected;
     }
-    return g_min_busy;
+    return m;
 }
-void SetMinBusyLanes(uint32_t v) { g_min_busy = (v >= 1 && v <= 64) ? v : kDefaultMinBusy; }
+void SetMinBusyLanes(uint32_t v) { g_min_busy.store((v >= 1 && v <= 64) ? v : kDefaultMinBusy, std::memory_order_relaxed); }
 
 static std::atomic<uint32_t> g_fetch{kDefaultFetch};
 static std::atomic<uint32_t> g_depth{kDefaultLdsStack};

@@ -25,3 +25,15 @@ def test_tuning_sweep_keeps_results(ctx, dev):
         assert autotune.current(ctx.lib) == {k: best[k] for k in ("rays_per_fetch", "lds_stack", "min_busy")}
     finally:
         autotune.apply(ctx.lib, start)
+
+
+def test_sort_digit_width_sweep_keeps_results(ctx, dev):
+    say("autotune sort digit width on 2M keys")
+    lib = ctx.lib
+    start = autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS)
+    try:
+        out = autotune.tune_sort(ctx, 2 << 20, reps=2)
+        assert [r["digit_bits"] for r in out["table"]] == [8, 10]
+        assert autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS) == out["best"]["digit_bits"]
+    finally:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start)

@@ -181,6 +181,35 @@ def test_sort_pairs(ctx, dev, n):
     assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])
 
 
+@pytest.mark.parametrize("n", [1 << 20, (1 << 20) + 12345, 5000000])
+def test_sort_pairs_ten_bit_digits(ctx, dev, n):
+    """Sorts of at least 2^20 keys with 10-bit digits (three passes for
+    30-bit keys): the same stable order as numpy, including a partial last
+    tile and many duplicate keys."""
+    say(f"onesweep sort with 10-bit digits, n={n}")
+    import torch
+
+    from radeonrays_sdk_amd import autotune
+
+    lib = ctx.lib
+    start = autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS)
+    try:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, 10)
+        rng = np.random.default_rng(n)
+        keys = rng.integers(0, 1 << 30, size=n, dtype=np.uint32)
+        keys[: n // 4] = keys[7]
+        vals = np.arange(n, dtype=np.uint32)
+        k = to_dev(keys.view(np.int32), dev)
+        v = to_dev(vals.view(np.int32), dev)
+        ctx.sort_pairs(k, v, 30)
+        torch.cuda.synchronize()
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(k.cpu().numpy().view(np.uint32), keys[order])
+        assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])
+    finally:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start)
+
+
 @pytest.mark.parametrize("query", ["closest", "any"])
 def test_visit_counters_match_oracle_stats(ctx, dev, query):
     say(f"plain-kernel hits and TraversalStats vs oracle, {query}")
