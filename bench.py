@@ -466,6 +466,13 @@ def main():
         autotune.set_(ctx.lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start_bits)
         out["autotune"]["sort_digit_bits"] = {"table": ts["table"], "best": ts["best"]["digit_bits"],
                                               "note": "identical sorted keys and values for every width"}
+        log("autotune sweep (treelet size)")
+        start_leaves = autotune.get(ctx.lib, autotune.rr.RRX_TUNING_TREELET_LEAVES)
+        tt = autotune.tune_treelet(ctx, v, idx, rays)
+        autotune.set_(ctx.lib, autotune.rr.RRX_TUNING_TREELET_LEAVES, start_leaves)
+        out["autotune"]["treelet_leaves"] = {"table": tt["table"], "best": tt["best"]["treelet_leaves"],
+                                             "note": "closest hits identical for every size; score = build + "
+                                                     "closest + any traversal of the heightfield rays"}
 
     if not args.no_extras:
         log("build / refit / sort timings")

@@ -38,8 +38,10 @@ def load():
             "orc_emit_hierarchy": ([P, P, c_uint32, P], None),
             "orc_refit": ([P, c_uint32, P, c_uint32, P], None),
             "orc_restructure": ([c_uint32, P], None),
+            "orc_restructure_leaves": ([c_uint32, P, c_int], None),
             "orc_blas_bytes": ([c_uint32], c_size_t),
             "orc_build_blas": ([P, c_uint32, P, c_uint32, c_int, P], None),
+            "orc_build_blas_leaves": ([P, c_uint32, P, c_uint32, c_int, c_int, P], None),
             "orc_tlas_bytes": ([c_uint32], c_size_t),
             "orc_build_tlas": ([P, P, P, c_uint32, P], None),
             "orc_trace": ([P, P, P, c_uint32, c_int, P, P, c_int], None),
@@ -117,11 +119,11 @@ def emit_hierarchy(sorted_codes, sorted_prims):
     return nodes
 
 
-def build_blas(v, i, restructure=False) -> np.ndarray:
+def build_blas(v, i, restructure=False, treelet_leaves=7) -> np.ndarray:
     v, i = _c(v, np.float32), _c(i, np.uint32)
     n = i.shape[0]
     buf = np.zeros(load().orc_blas_bytes(n), np.uint8)
-    load().orc_build_blas(_p(v), v.shape[1], _p(i), n, int(restructure), _p(buf))
+    load().orc_build_blas_leaves(_p(v), v.shape[1], _p(i), n, int(restructure), int(treelet_leaves), _p(buf))
     return buf
 
 
@@ -138,9 +140,9 @@ def build_blas_aabbs(boxes, restructure=False) -> np.ndarray:
     return buf
 
 
-def restructure(buf: np.ndarray, tris: int) -> np.ndarray:
+def restructure(buf: np.ndarray, tris: int, treelet_leaves=7) -> np.ndarray:
     out = buf.copy()
-    load().orc_restructure(tris, _p(out[128:]))
+    load().orc_restructure_leaves(tris, _p(out[128:]), int(treelet_leaves))
     return out
 
 

@@ -233,7 +233,7 @@ void Refit(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t n, N
 
 // --- treelet restructuring (restructure_bvh.comp:149-542) -----------------
 
-constexpr int kLeaves = 7;
+constexpr int kLeaves = 7;  // the reference's treelet size; 5 and 9 are tuning options
 constexpr int kMasks = 1 << kLeaves;
 constexpr float kCInt = 1.2f;
 
@@ -261,12 +261,16 @@ Box NodeBox(const Node* nodes, uint32_t x, uint32_t nint) {
     return b;
 }
 
-// Optimal partition of every subset of the 7 treelet leaves by increasing
-// size (restructure_bvh.comp:224-414): candidates of subset s are the
-// non-empty subsets p of s without its lowest leaf, in increasing order, the
-// first cheapest wins; cost = C_INT * area + best split.  Orientation as
-// restructure_bvh.comp:395-397 (sizes 2-5 flipped for even sizes).
-void TreeletDp(const Box lb[kLeaves], float sa[kMasks], float cost[kMasks], uint32_t part[kMasks]) {
+// Optimal partition of every subset of the L treelet leaves by increasing
+// size (restructure_bvh.comp:224-414, L = 7 there): candidates of subset s
+// are the non-empty subsets p of s without its lowest leaf, in increasing
+// order, the first cheapest wins; cost = C_INT * area + best split.
+// Orientation as restructure_bvh.comp:395-397 (sizes 2-5 flipped for even
+// sizes; larger sizes unflipped).
+template <int L>
+void TreeletDp(const Box lb[L], float sa[1 << L], float cost[1 << L], uint32_t part[1 << L]) {
+    constexpr int kLeaves = L;
+    constexpr int kMasks = 1 << L;
     auto union_of = [&](uint32_t m) {
         Box b;
         for (int a = 0; a < 3; ++a) { b.lo[a] = FLT_MAX; b.hi[a] = -FLT_MAX; }
@@ -305,7 +309,10 @@ void TreeletDp(const Box lb[kLeaves], float sa[kMasks], float cost[kMasks], uint
     }
 }
 
+template <int L>
 void RestructureTreelet(Node* nodes, uint32_t nint, uint32_t root) {
+    constexpr int kLeaves = L;
+    constexpr int kMasks = 1 << L;
     uint32_t leaf[kLeaves], inner[kLeaves - 1];
     float area[kLeaves];
     inner[0] = root;
@@ -323,7 +330,7 @@ void RestructureTreelet(Node* nodes, uint32_t nint, uint32_t root) {
                 largest = area[k];
                 sel = k;
             }
-        if (sel < 0) return;  // fewer than 7 leaves below: leave as is
+        if (sel < 0) return;  // fewer than L leaves below: leave as is
         const uint32_t opened = leaf[sel];
         inner[size - 1] = opened;
         const uint32_t c0 = nodes[opened].c0, c1 = nodes[opened].c1;
@@ -349,7 +356,7 @@ void RestructureTreelet(Node* nodes, uint32_t nint, uint32_t root) {
                 }
         return b;
     };
-    TreeletDp(lb, sa, cost, part);
+    TreeletDp<L>(lb, sa, cost, part);
     // Rewire top-down.
     uint32_t stack_mask[kLeaves], stack_node[kLeaves];
     int sp = 0, allocated = 1;
@@ -401,6 +408,7 @@ void RefitFromChildren(Node* nodes, uint32_t x, uint32_t nint) {
 // stops at (both children complete, >= min_prims primitives); every root
 // and every ancestor of a root is processed after its subtree, ancestors
 // being refit from their children first (restructure_bvh.comp:497-540).
+template <int L>
 void RestructureRound(Node* nodes, uint32_t n, uint32_t min_prims) {
     const uint32_t nint = n - 1, total = NodeCount(n);
     std::vector<uint32_t> count(total, 0);
@@ -438,7 +446,7 @@ void RestructureRound(Node* nodes, uint32_t n, uint32_t min_prims) {
     for (uint32_t x : order) {
         if (x >= nint || !marked[x]) continue;
         if (!root[x]) RefitFromChildren(nodes, x, nint);
-        RestructureTreelet(nodes, nint, x);
+        RestructureTreelet<L>(nodes, nint, x);
     }
 }
 
@@ -751,15 +759,25 @@ void orc_refit(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t 
     Refit(vtx, stride, idx, tris, static_cast<Node*>(nodes));
 }
 
-void orc_restructure(uint32_t tris, void* nodes) {
+// Three rounds with at least 64, 128, 256 primitives per treelet root
+// (restructure_hlbvh.cpp:34-35,151,276), treelets of `leaves` leaves
+// (5, 7 or 9; 7 in the reference).
+void orc_restructure_leaves(uint32_t tris, void* nodes, int leaves) {
     if (tris < 64) return;
-    uint32_t m = 64;  // restructure_hlbvh.cpp:34-35,151,276
-    for (int round = 0; round < 3; ++round, m *= 2) RestructureRound(static_cast<Node*>(nodes), tris, m);
+    uint32_t m = 64;
+    for (int round = 0; round < 3; ++round, m *= 2) {
+        if (leaves == 5) RestructureRound<5>(static_cast<Node*>(nodes), tris, m);
+        else if (leaves == 9) RestructureRound<9>(static_cast<Node*>(nodes), tris, m);
+        else RestructureRound<7>(static_cast<Node*>(nodes), tris, m);
+    }
 }
+
+void orc_restructure(uint32_t tris, void* nodes) { orc_restructure_leaves(tris, nodes, kLeaves); }
 
 size_t orc_blas_bytes(uint32_t tris) { return sizeof(Header) + Align((size_t)NodeCount(tris) * 64, 256); }
 
-void orc_build_blas(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t n, int restructure, void* buffer) {
+void orc_build_blas_leaves(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t n, int restructure,
+                           int leaves, void* buffer) {
     std::memset(buffer, 0, orc_blas_bytes(n));
     Header* h = static_cast<Header*>(buffer);
     h->magic = kMagic;
@@ -777,7 +795,11 @@ void orc_build_blas(const float* vtx, uint32_t stride, const uint32_t* idx, uint
     Node* nodes = NodesOf(buffer);
     Emit(codes.data(), prims.data(), n, nodes);
     Refit(vtx, stride, idx, n, nodes);
-    if (restructure) orc_restructure(n, nodes);
+    if (restructure) orc_restructure_leaves(n, nodes, leaves);
+}
+
+void orc_build_blas(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t n, int restructure, void* buffer) {
+    orc_build_blas_leaves(vtx, stride, idx, n, restructure, kLeaves, buffer);
 }
 
 size_t orc_tlas_bytes(uint32_t n) {
@@ -991,7 +1013,7 @@ void orc_treelet_dp(const float* boxes, float* cost, uint32_t* part) {
             lb[k].hi[a] = boxes[6 * k + 3 + a];
         }
     float sa[kMasks];
-    TreeletDp(lb, sa, cost, part);
+    TreeletDp<kLeaves>(lb, sa, cost, part);
 }
 
 int orc_is_valid(const void* accel) {

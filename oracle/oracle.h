@@ -49,8 +49,10 @@ void orc_emit_hierarchy(const uint32_t* sorted_codes, const uint32_t* sorted_pri
  * boxes of their children. */
 void orc_refit(const float* vertices, uint32_t stride_floats, const uint32_t* indices, uint32_t tris, void* nodes);
 
-/* Treelet restructuring, three rounds with minimum root sizes 64/128/256 (a9). */
+/* Treelet restructuring, three rounds with minimum root sizes 64/128/256 (a9),
+ * 7-leaf treelets; orc_restructure_leaves takes the treelet size (5, 7, 9). */
 void orc_restructure(uint32_t tris, void* nodes);
+void orc_restructure_leaves(uint32_t tris, void* nodes, int leaves);
 
 /* Dynamic programme of one 7-leaf treelet: boxes are 7 x (lo xyz, hi xyz);
  * writes the optimal cost and split of each of the 128 leaf subsets. */
@@ -60,6 +62,8 @@ void orc_treelet_dp(const float* boxes, float* cost, uint32_t* part);
 size_t orc_blas_bytes(uint32_t tris);
 void orc_build_blas(const float* vertices, uint32_t stride_floats, const uint32_t* indices, uint32_t tris,
                     int restructure, void* buffer);
+void orc_build_blas_leaves(const float* vertices, uint32_t stride_floats, const uint32_t* indices, uint32_t tris,
+                           int restructure, int leaves, void* buffer);
 
 /* TLAS over instances (a11): xforms are row-major 3x4 object-to-world,
  * blas_addr the device address recorded for each instance, blas_host the

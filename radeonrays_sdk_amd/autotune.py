@@ -11,10 +11,13 @@ The sort's radix digit width is swept by tune_sort: 8-bit digits (four
 passes over 30-bit Morton codes) or 10-bit digits (three passes, 1024-bin
 tiles), on a caller-given number of keys; both give the same stable order.
 
+The treelet size of the restructuring (5, 7 or 9 leaves) is swept by
+tune_treelet: it trades build time (the subset dynamic programme grows as
+3^leaves) against tree quality (traversal time); closest hits are the same
+for every size.
+
 Not swept, with the reason: 4-bit digits (they double the passes over
-memory of a sort that is bandwidth-bound) and the treelet size (7 leaves;
-the treelet kernels' subset dynamic programme is written for 2^7 masks and
-9 leaves would take 4x the LDS per treelet).
+memory of a sort that is bandwidth-bound).
 """
 from __future__ import annotations
 
@@ -167,4 +170,56 @@ def tune_sort(ctx, n: int = 16 << 20, reps: int = 5, digit_bits=DIGIT_BITS, seed
     best = min(table, key=lambda r: r["ms"])
     default = next((r for r in table if r["digit_bits"] == start), None)
     set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, best["digit_bits"])
+    return {"best": best, "default": default, "table": table}
+
+
+TREELET_LEAVES = (5, 7, 9)
+
+
+def tune_treelet(ctx, vertices, indices, rays, reps: int = 3, leaves=TREELET_LEAVES) -> dict:
+    """Sweep the treelet size on one mesh: for each size, the restructured
+    build time and the closest + any-hit traversal time of `rays`; the score
+    is their sum (one build per pass over the rays).  Closest hits must be
+    identical for every size (raises otherwise).  Leaves the best size set;
+    returns {"best", "default", "table"}."""
+    import torch
+
+    lib = ctx.lib
+    start = get(lib, rr.RRX_TUNING_TREELET_LEAVES)
+    n = int(rays.shape[0])
+    hits = ctx.empty(n * 16)
+    scratch = ctx.trace_scratch(n)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+    reference = None
+    table = []
+    for size in leaves:
+        set_(lib, rr.RRX_TUNING_TREELET_LEAVES, size)
+        accel = ctx.build_geometry(vertices, indices, restructure=True)
+        build_ms = timed(lambda: ctx.build_geometry(vertices, indices, restructure=True, result=accel.buffer))
+        ctx.intersect(accel, rays, "closest", "full", hits=hits, scratch=scratch)
+        torch.cuda.synchronize()
+        got = hits.cpu().numpy().copy()
+        if reference is None:
+            reference = got
+        elif not np.array_equal(got, reference):
+            set_(lib, rr.RRX_TUNING_TREELET_LEAVES, start)
+            raise RuntimeError(f"closest hits differ with {size}-leaf treelets")
+        trace_ms = timed(lambda: [ctx.intersect(accel, rays, q, "full", hits=hits, scratch=scratch)
+                                  for q in ("closest", "any")])
+        table.append({"treelet_leaves": size, "build_ms": round(build_ms, 4), "trace_ms": round(trace_ms, 4),
+                      "score_ms": round(build_ms + trace_ms, 4)})
+    best = min(table, key=lambda r: r["score_ms"])
+    default = next((r for r in table if r["treelet_leaves"] == start), None)
+    set_(lib, rr.RRX_TUNING_TREELET_LEAVES, best["treelet_leaves"])
     return {"best": best, "default": default, "table": table}

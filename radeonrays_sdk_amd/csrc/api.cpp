@@ -1211,6 +1211,8 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             return RR_SUCCESS;
         case RRX_TUNING_SORT_DIGIT_BITS:
             return rr::SetSortDigitBits((int)value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_TREELET_LEAVES:
+            return rr::SetTreeletLeaves((int)value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1223,6 +1225,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_TRACE_LDS_STACK: *value = rr::LdsStackDepth(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_PLAIN: *value = rr::TraceMode() == rr::kTraceModePlain ? 1u : 0u; return RR_SUCCESS;
         case RRX_TUNING_SORT_DIGIT_BITS: *value = (uint32_t)rr::SortDigitBits(); return RR_SUCCESS;
+        case RRX_TUNING_TREELET_LEAVES: *value = (uint32_t)rr::TreeletLeaves(); return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

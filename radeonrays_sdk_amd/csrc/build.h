@@ -96,4 +96,9 @@ inline size_t TlasUploadOffset() { return 0; }
 // treelet-root sizes 64, 128, 256 primitives).
 void RestructureBlas(uint32_t tris, void* scratch, void* result, hipStream_t stream);
 
+// Leaves per treelet of the restructuring (5, 7 or 9; default 7, the
+// reference's).  Process-wide; setters reject other values.
+int TreeletLeaves();
+bool SetTreeletLeaves(int leaves);
+
 }  // namespace rr

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 
 #include "build.h"
@@ -221,8 +222,9 @@ void LaunchMeshRefit(const MeshView& m, void* result, hipStream_t s) {
 // init_primitive_count.comp, find_treelet_roots.comp and
 // restructure_bvh.comp:149-542 with restructure_hlbvh.cpp:133-278.
 
-constexpr int kTreeletLeaves = 7;
-constexpr int kTreeletMasks = 1 << kTreeletLeaves;
+// Treelet size: 7 leaves in the reference (restructure_bvh.comp); 5 and 9
+// are tuning options (process-wide, SetTreeletLeaves).
+static std::atomic<int> g_treelet_leaves{7};
 constexpr float kCostInternal = 1.2f;  // C_INT, restructure_bvh.comp:38
 
 __device__ __forceinline__ float SurfaceArea(const float lo[3], const float hi[3]) {
@@ -293,8 +295,11 @@ __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* 
 // One wave per treelet root.  Lane 0 does the pointer-chasing parts (forming
 // the treelet, rewiring it, climbing); the 128 subset areas and the dynamic
 // programme over subsets run across the wave.
+template <int LEAVES>
 __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, uint32_t n, const uint32_t* counter,
                                                                 const uint32_t* __restrict__ roots, uint32_t* counts) {
+    constexpr int kTreeletLeaves = LEAVES;
+    constexpr uint32_t kTreeletMasks = 1u << LEAVES;
     if (blockIdx.x >= *counter) return;
     __shared__ uint32_t s_leaf[kTreeletLeaves];
     __shared__ uint32_t s_inner[kTreeletLeaves - 1];
@@ -750,6 +755,13 @@ void UpdateBlas(const MeshDesc& mesh, void* result, hipStream_t s) {
     LaunchMeshRefit(ViewOf(mesh), result, s);
 }
 
+int TreeletLeaves() { return g_treelet_leaves.load(std::memory_order_relaxed); }
+bool SetTreeletLeaves(int leaves) {
+    if (leaves != 5 && leaves != 7 && leaves != 9) return false;
+    g_treelet_leaves.store(leaves, std::memory_order_relaxed);
+    return true;
+}
+
 void RestructureBlas(uint32_t n, void* scratch, void* result, hipStream_t s) {
     if (n < 64) return;
     BvhNode* nodes = At<BvhNode>(result, sizeof(AccelHeader));
@@ -761,7 +773,12 @@ void RestructureBlas(uint32_t n, void* scratch, void* result, hipStream_t s) {
     for (int round = 0; round < 3; ++round, min_prims *= 2) {
         InitTreeletCountsKernel<<<Blocks(n), kBlock, 0, s>>>(n, counts, counter);
         FindTreeletRootsKernel<<<Blocks(n), kBlock, 0, s>>>(nodes, n, min_prims, counts, counter, roots);
-        RestructureTreeletsKernel<<<Blocks(n, (int)min_prims), 64, 0, s>>>(nodes, n, counter, roots, counts);
+        const dim3 grid(Blocks(n, (int)min_prims));
+        switch (TreeletLeaves()) {
+            case 5: RestructureTreeletsKernel<5><<<grid, 64, 0, s>>>(nodes, n, counter, roots, counts); break;
+            case 9: RestructureTreeletsKernel<9><<<grid, 64, 0, s>>>(nodes, n, counter, roots, counts); break;
+            default: RestructureTreeletsKernel<7><<<grid, 64, 0, s>>>(nodes, n, counter, roots, counts); break;
+        }
     }
 }
 

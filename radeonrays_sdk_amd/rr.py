@@ -55,6 +55,7 @@ RRX_TUNING_TRACE_RAYS_PER_FETCH = 1
 RRX_TUNING_TRACE_LDS_STACK = 2
 RRX_TUNING_TRACE_PLAIN = 3
 RRX_TUNING_SORT_DIGIT_BITS = 4
+RRX_TUNING_TREELET_LEAVES = 5
 
 RRX_SERVE_BATCHED = 0
 RRX_SERVE_PLAIN = 1

@@ -137,6 +137,12 @@ def test_tuning_validation():
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, 10) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 10
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, 8) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TREELET_LEAVES, byref(v)) == rr.RR_SUCCESS and v.value == 7
+    for bad in (6, 8, 11):
+        assert lib.rrxSetTuning(rr.RRX_TUNING_TREELET_LEAVES, bad) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TREELET_LEAVES, 9) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TREELET_LEAVES, byref(v)) == rr.RR_SUCCESS and v.value == 9
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TREELET_LEAVES, 7) == rr.RR_SUCCESS
 
 
 def test_aabb_list_requirements_and_unknown_primitive_type():

@@ -37,3 +37,53 @@ def test_sort_digit_width_sweep_keeps_results(ctx, dev):
         assert autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS) == out["best"]["digit_bits"]
     finally:
         autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start)
+
+
+@pytest.mark.parametrize("leaves", [5, 9])
+def test_treelet_sizes_match_oracle(ctx, dev, leaves):
+    """Restructuring with 5- or 9-leaf treelets equals the oracle's
+    restatement node for node, and its hits equal the oracle's."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from radeonrays_sdk_amd import api
+    from tests.gpu_util import hits_equal, nodes_equal
+
+    say(f"restructure with {leaves}-leaf treelets vs oracle")
+    lib = ctx.lib
+    start = autotune.get(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES)
+    try:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES, leaves)
+        for v, i in (scenes.triangle_soup(20000, scenes.rng_for(40, leaves)),
+                     scenes.heightfield_mesh(90, seed=scenes.SEED_BASE + leaves)):
+            i = i.reshape(-1, 3)
+            accel = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True)
+            torch.cuda.synchronize()
+            buf = accel.buffer.cpu().numpy()
+            ok, msg = nodes_equal(buf, oracle.build_blas(v, i, True, treelet_leaves=leaves))
+            assert ok, msg
+            lo, hi = v.min(axis=0), v.max(axis=0)
+            rng = np.random.default_rng(leaves)
+            o = rng.uniform(lo, hi, size=(20000, 3)).astype(np.float32)
+            rays = scenes.make_rays(o, scenes.unit_sphere(20000, rng))
+            for q in ("closest", "any"):
+                hits = ctx.intersect(accel, to_dev(rays, dev), q, "full_t")
+                torch.cuda.synchronize()
+                ok, msg = hits_equal(api.decode_hits(hits, "full_t"), oracle.trace(buf, rays, any_hit=(q == "any")))
+                assert ok, f"{q}: {msg}"
+    finally:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES, start)
+
+
+def test_treelet_sweep_keeps_closest_hits(ctx, dev):
+    say("autotune treelet size on a 256x256 heightfield")
+    w = scenes.heightfield(res=256, n_rays=100000)
+    lib = ctx.lib
+    start = autotune.get(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES)
+    try:
+        out = autotune.tune_treelet(ctx, to_dev(w.vertices, dev), to_dev(w.indices, dev), to_dev(w.rays, dev), reps=2)
+        assert [r["treelet_leaves"] for r in out["table"]] == [5, 7, 9]
+        assert autotune.get(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES) == out["best"]["treelet_leaves"]
+    finally:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES, start)

@@ -242,6 +242,26 @@ def test_restructure_keeps_validity_and_lowers_sah():
         assert sorted(leaves[:, 7].tolist()) == list(range(n))  # each primitive in exactly one leaf
 
 
+@pytest.mark.parametrize("leaves", [5, 9])
+def test_restructure_other_treelet_sizes(leaves):
+    """Treelets of 5 or 9 leaves (a tuning option; the reference uses 7):
+    valid, consistent, every primitive in one leaf, SAH not above the
+    unrestructured build, and traversal still equals brute force."""
+    for n, seed in ((64, 1), (3000, 4)):
+        v, i = scenes.triangle_soup(n, scenes.rng_for(12, seed))
+        base = oracle.build_blas(v, i, restructure=False)
+        opt = oracle.build_blas(v, i, restructure=True, treelet_leaves=leaves)
+        assert oracle.is_valid(opt) and oracle.check_consistency(opt)
+        assert oracle.sah(opt) <= oracle.sah(base) * (1 + 1e-6)
+        leaf_nodes = oracle.nodes_of(opt)[n - 1:]
+        assert sorted(leaf_nodes[:, 7].tolist()) == list(range(n))
+    w = scenes.soup_camera(n_tris=5000, width=48, height=48)
+    buf = oracle.build_blas(w.vertices, w.indices, True, treelet_leaves=leaves)
+    h, bf = oracle.trace(buf, w.rays), oracle.brute_force(buf, w.rays)
+    assert np.array_equal(h["prim_id"], bf["prim_id"])
+    assert np.array_equal(h["t"].view(np.uint32), bf["t"].view(np.uint32))
+
+
 # --- a12/a13/a15: traversal ----------------------------------------------------
 def test_watertight_known_answer():
     # Ray straight down onto the triangle (0,0,0),(1,0,0),(0,0,1) at x=0.25, z=0.5.
