@@ -324,6 +324,14 @@ def batched_bench(ctx, dev, log, frames=16):
     # GPU times from graph replays (the eager calls' host-side setup of
     # 8192-entry tables is not GPU work); the graphs replay the same calls.
     build_gpu_ms = timed(graph_of(build).replay, 3, warm=1)
+    rs_results = [ctx.empty(ctx.geometry_requirements(n, 3 * n, True).result_buffer_size) for _, n in spans]
+    rs_scratch = ctx.empty(ctx.batch_build_requirements(meshes, True))
+
+    def build_restructured():
+        ctx.build_geometry_batch(meshes, restructure=True, scratch=rs_scratch, results=rs_results)
+
+    build_rs_gpu_ms = timed(graph_of(build_restructured).replay, 3, warm=1)
+    del rs_results, rs_scratch
     refit_ms = timed(graph_of(refit).replay, 3, warm=1)
     query_ms = timed(graph_of(query).replay, 3, warm=1)
     g = graph_of(refit, query)
@@ -342,6 +350,7 @@ def batched_bench(ctx, dev, log, frames=16):
                         f"{frames} refit+query frames replayed from a HIP graph",
             "batch_build_ms": round(build_ms, 3), "batch_build_gpu_ms": round(build_gpu_ms, 3),
             "batch_build_mtris_s": round(tris / build_gpu_ms / 1e3, 1),
+            "batch_build_restructured_gpu_ms": round(build_rs_gpu_ms, 3),
             "per_call_build_ms_wall": round(per_call_ms, 1), "batch_equals_per_call": bool(same),
             "refit_ms": round(refit_ms, 3), "refit_gbs": round(tris * 304 / (refit_ms * 1e-3) / 1e9, 1),
             "query_ms": round(query_ms, 3), "query_mrays_s": round(R / query_ms / 1e3, 1),

@@ -1083,6 +1083,8 @@ struct BatchPlan {
     std::vector<uint32_t> small, large;  // indices into meshes; small sorted by size, largest first
     std::vector<size_t> work_offset;     // per small mesh
     size_t head = 0, total = 0;
+    size_t roots_offset = 0;             // treelet-root list of the batched restructuring
+    uint32_t max_roots[3] = {0, 0, 0};   // its bound per round
 };
 
 static RRError PlanBatch(const RRGeometryBuildInput* inputs, uint32_t count, bool restructure, bool resolve,
@@ -1101,6 +1103,13 @@ static RRError PlanBatch(const RRGeometryBuildInput* inputs, uint32_t count, boo
     for (size_t k = 0; k < plan.small.size(); ++k) {
         plan.work_offset[k] = o;
         o += rr::BatchWorkBytes(plan.meshes[plan.small[k]].tri_count);
+    }
+    if (restructure) {
+        // Every treelet root of round r covers at least 64 << r primitives.
+        for (uint32_t k : plan.small)
+            for (int r = 0; r < 3; ++r) plan.max_roots[r] += plan.meshes[k].tri_count / (64u << r);
+        plan.roots_offset = o;
+        o += rr::AlignUp(size_t(plan.max_roots[0]) * 8, 256);
     }
     plan.total = o;
     for (uint32_t i : plan.large) plan.total = std::max(plan.total, rr::BlasBuildScratchBytes(plan.meshes[i].tri_count, restructure));
@@ -1167,6 +1176,8 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
             chunks += rr::RefitChunks(m.tri_count);
             host[k] = b;
         }
+        const size_t roots_offset = plan.roots_offset;
+        const uint32_t max_roots[3] = {plan.max_roots[0], plan.max_roots[1], plan.max_roots[2]};
         auto cmd = [=](hipStream_t s) {
             if (n_small) {
                 auto* table = reinterpret_cast<rr::BatchMesh*>(static_cast<uint8_t*>(scratch) + rr::kMeshTableOffset);
@@ -1176,12 +1187,10 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
                     rr::BatchRefit(table, n_small, chunks, scratch, s);
                 } else {
                     rr::BatchBuild(table, n_small, chunks, scratch, s);
-                    // Treelet restructuring, mesh by mesh, in each mesh's own work area.
+                    // Treelet restructuring of all meshes at once, each in its own work area.
                     if (restructure)
-                        for (uint32_t k = 0; k < n_small; ++k)
-                            rr::RestructureBlas(plan.meshes[plan.small[k]].tri_count,
-                                                static_cast<uint8_t*>(scratch) + plan.work_offset[k],
-                                                results[plan.small[k]], s);
+                        rr::BatchRestructure(table, n_small, chunks, max_roots, scratch,
+                                             static_cast<uint8_t*>(scratch) + roots_offset, s);
                 }
             }
             for (uint32_t i : plan.large) {

@@ -55,4 +55,15 @@ void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, 
 // UpdateBlas mesh by mesh (one launch over the refit chunks of all meshes).
 void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
 
+// Treelet restructuring of every mesh in the table with at least 64
+// triangles, as RestructureBlas mesh by mesh: per round one launch finds the
+// treelet roots of all meshes and one wave per root restructures.  Each
+// mesh's work area holds its arrival counts; `roots` holds max_roots[0]
+// (mesh, node) pairs, max_roots[r] bounding the roots of round r (the sum
+// over meshes of tris / (64 << r)); its counter is scratch word
+// kBatchRootsCounterWord.
+constexpr uint32_t kBatchRootsCounterWord = 1;
+void BatchRestructure(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, const uint32_t max_roots[3],
+                      void* scratch, void* roots, hipStream_t stream);
+
 }  // namespace rr

@@ -17,6 +17,7 @@
 #include <atomic>
 #include <cfloat>
 
+#include "batch.h"
 #include "build.h"
 #include "build_device.h"
 #include "rr_device.h"
@@ -62,7 +63,7 @@ BlasScratchLayout LayoutBlas(uint32_t n) {
 struct RestructureLayout {
     size_t counter, roots, counts, total;
 };
-RestructureLayout LayoutRestructure(uint32_t n) {
+__host__ __device__ RestructureLayout LayoutRestructure(uint32_t n) {
     RestructureLayout l;
     size_t o = 0;
     l.counter = o; o += 256;
@@ -272,35 +273,40 @@ __global__ __launch_bounds__(kBlock) void InitTreeletCountsKernel(uint32_t n, ui
 
 // Climb from each leaf, accumulating primitive counts; the first node whose
 // subtree reaches `min_prims` becomes a treelet root (find_treelet_roots.comp:63-98).
-__global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* __restrict__ nodes, uint32_t n,
-                                                                 uint32_t min_prims, uint32_t* counts, uint32_t* counter,
-                                                                 uint32_t* roots) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+// Root found by the climb from leaf i, or kInvalid.
+__device__ __forceinline__ uint32_t ClimbToTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t i,
+                                                       uint32_t min_prims, uint32_t* counts) {
     uint32_t count = 1;
     uint32_t idx = nodes[n - 1 + i].parent;
     const uint32_t total = 2 * n - 1;
     for (uint32_t level = 0; idx < total && level < total; ++level) {
         const uint32_t old = __hip_atomic_exchange(&counts[idx], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         count += old;
-        if (old == 0) return;
-        if (count >= min_prims) {
-            roots[atomicAdd(counter, 1u)] = idx;
-            return;
-        }
+        if (old == 0) return kInvalid;
+        if (count >= min_prims) return idx;
         idx = nodes[idx].parent;
     }
+    return kInvalid;
+}
+
+__global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* __restrict__ nodes, uint32_t n,
+                                                                 uint32_t min_prims, uint32_t* counts, uint32_t* counter,
+                                                                 uint32_t* roots) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t root = ClimbToTreeletRoot(nodes, n, i, min_prims, counts);
+    if (root != kInvalid) roots[atomicAdd(counter, 1u)] = root;
 }
 
 // One wave per treelet root.  Lane 0 does the pointer-chasing parts (forming
 // the treelet, rewiring it, climbing); the 128 subset areas and the dynamic
 // programme over subsets run across the wave.
+// The body runs for one root of one BVH (nodes, n leaves, arrival counts
+// of its internal nodes); the single-mesh and batched kernels call it.
 template <int LEAVES>
-__global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, uint32_t n, const uint32_t* counter,
-                                                                const uint32_t* __restrict__ roots, uint32_t* counts) {
+__device__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, uint32_t root, uint32_t* counts) {
     constexpr int kTreeletLeaves = LEAVES;
     constexpr uint32_t kTreeletMasks = 1u << LEAVES;
-    if (blockIdx.x >= *counter) return;
     __shared__ uint32_t s_leaf[kTreeletLeaves];
     __shared__ uint32_t s_inner[kTreeletLeaves - 1];
     __shared__ float s_box[kTreeletLeaves][6];
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
     __shared__ uint32_t s_go;
     const uint32_t lane = threadIdx.x;
     const uint32_t n_int = n - 1;
-    if (lane == 0) s_root = roots[blockIdx.x];
+    if (lane == 0) s_root = root;
     __syncthreads();
     // Each pass moves one level up, so a tree of n leaves needs fewer than n passes.
     for (uint32_t pass = 0; pass < n; ++pass) {
@@ -476,6 +482,74 @@ __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, 
         __syncthreads();
         if (!s_go) return;
     }
+}
+
+template <int LEAVES>
+__global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, uint32_t n, const uint32_t* counter,
+                                                                const uint32_t* __restrict__ roots, uint32_t* counts) {
+    if (blockIdx.x >= *counter) return;
+    RestructureFromRoot<LEAVES>(nodes, n, roots[blockIdx.x], counts);
+}
+
+// ---------------------------------------------------------------------------
+// Batched restructuring: each step of each round runs once for all meshes of
+// a batch table (threads over 64-leaf chunks of any mesh, one wave per
+// treelet root of any mesh).  A mesh's arrival counts live in its work area
+// (LayoutRestructure); meshes of fewer than 64 triangles stay as built.
+
+// Mesh and leaf of thread threadIdx.x of block blockIdx.x over the refit
+// chunks of the table; false past the chunks or the mesh's leaves.
+__device__ __forceinline__ bool BatchChunkLeaf(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                               uint32_t total_chunks, uint32_t& mesh, uint32_t& leaf) {
+    const uint32_t c = blockIdx.x * (kBlock / kRefitChunk) + threadIdx.x / kRefitChunk;
+    if (c >= total_chunks) return false;
+    uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (table[mid].first_chunk <= c) lo = mid;
+        else hi = mid;
+    }
+    mesh = lo;
+    leaf = (c - table[lo].first_chunk) * kRefitChunk + threadIdx.x % kRefitChunk;
+    return c >= table[lo].first_chunk && leaf < table[lo].tri_count;
+}
+
+__device__ __forceinline__ uint32_t* BatchCounts(const BatchMesh& bm, uint32_t* arg) {
+    return RebaseOn<uint32_t>(arg, bm.work + LayoutRestructure(bm.tri_count).counts);
+}
+
+__global__ __launch_bounds__(kBlock) void BatchTreeletInitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                                                 uint32_t total_chunks, uint32_t* roots_counter) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *roots_counter = 0;
+    uint32_t mesh, leaf;
+    if (!BatchChunkLeaf(table, meshes, total_chunks, mesh, leaf)) return;
+    const BatchMesh bm = table[mesh];
+    if (bm.tri_count < 64 || leaf + 1 >= bm.tri_count) return;  // counts of internal nodes [0, n - 1)
+    BatchCounts(bm, roots_counter)[leaf] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const BatchMesh* __restrict__ table,
+                                                                      uint32_t meshes, uint32_t total_chunks,
+                                                                      uint32_t min_prims, uint32_t* roots_counter,
+                                                                      uint2* roots) {
+    uint32_t mesh, leaf;
+    if (!BatchChunkLeaf(table, meshes, total_chunks, mesh, leaf)) return;
+    const BatchMesh bm = table[mesh];
+    if (bm.tri_count < 64) return;
+    const BvhNode* nodes = reinterpret_cast<const BvhNode*>(RebaseOn<uint8_t>(roots_counter, bm.result) + sizeof(AccelHeader));
+    const uint32_t root = ClimbToTreeletRoot(nodes, bm.tri_count, leaf, min_prims, BatchCounts(bm, roots_counter));
+    if (root != kInvalid) roots[atomicAdd(roots_counter, 1u)] = make_uint2(mesh, root);
+}
+
+template <int LEAVES>
+__global__ __launch_bounds__(64) void BatchRestructureTreeletsKernel(const BatchMesh* __restrict__ table,
+                                                                     uint32_t* roots_counter,
+                                                                     const uint2* __restrict__ roots) {
+    if (blockIdx.x >= *roots_counter) return;
+    const uint2 r = roots[blockIdx.x];
+    const BatchMesh bm = table[r.x];
+    BvhNode* nodes = reinterpret_cast<BvhNode*>(RebaseOn<uint8_t>(roots_counter, bm.result) + sizeof(AccelHeader));
+    RestructureFromRoot<LEAVES>(nodes, bm.tri_count, r.y, BatchCounts(bm, roots_counter));
 }
 
 
@@ -778,6 +852,26 @@ void RestructureBlas(uint32_t n, void* scratch, void* result, hipStream_t s) {
             case 5: RestructureTreeletsKernel<5><<<grid, 64, 0, s>>>(nodes, n, counter, roots, counts); break;
             case 9: RestructureTreeletsKernel<9><<<grid, 64, 0, s>>>(nodes, n, counter, roots, counts); break;
             default: RestructureTreeletsKernel<7><<<grid, 64, 0, s>>>(nodes, n, counter, roots, counts); break;
+        }
+    }
+}
+
+void BatchRestructure(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, const uint32_t max_roots[3],
+                      void* scratch, void* roots, hipStream_t s) {
+    if (meshes == 0 || total_chunks == 0) return;
+    uint32_t* counter = static_cast<uint32_t*>(scratch) + kBatchRootsCounterWord;
+    uint2* list = static_cast<uint2*>(roots);
+    const uint32_t blocks = (uint32_t)(((uint64_t)total_chunks * kRefitChunk + kBlock - 1) / kBlock);
+    uint32_t min_prims = 64;  // restructure_hlbvh.cpp:34-35,151,276: 64, 128, 256
+    for (int round = 0; round < 3; ++round, min_prims *= 2) {
+        BatchTreeletInitKernel<<<blocks, kBlock, 0, s>>>(table, meshes, total_chunks, counter);
+        BatchFindTreeletRootsKernel<<<blocks, kBlock, 0, s>>>(table, meshes, total_chunks, min_prims, counter, list);
+        if (max_roots[round] == 0) continue;
+        const dim3 grid(max_roots[round]);
+        switch (TreeletLeaves()) {
+            case 5: BatchRestructureTreeletsKernel<5><<<grid, 64, 0, s>>>(table, counter, list); break;
+            case 9: BatchRestructureTreeletsKernel<9><<<grid, 64, 0, s>>>(table, counter, list); break;
+            default: BatchRestructureTreeletsKernel<7><<<grid, 64, 0, s>>>(table, counter, list); break;
         }
     }
 }
