@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 }
                 // Leave for the leaf phase once every descending lane holds a
                 // leaf, or once too few lanes are still descending.
-                if (!__any(leaf == kInvalid) || (uint32_t)__popcll(__ballot(1)) < min_busy) break;
+                if (!__any(leaf == kInvalid) || (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy)) break;
             }
             // Phase B: the postponed leaf, then any leaves that come next in order.
             if (leaf == kInvalid && node != kInvalid && node >= n_int) {
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 leaf = kInvalid;
                 break;
             }
-            if ((uint32_t)__popcll(__ballot(1)) < min_busy) break;
+            if (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy) break;  // refill (nothing left once drained)
         }
     }
 }
@@ -582,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                 }
                 // An index past the level's nodes (corrupt buffer) ends the ray.
                 if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
-                if (!__any(leaf == kInvalid) || (uint32_t)__popcll(__ballot(1)) < min_busy) break;
+                if (!__any(leaf == kInvalid) || (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy)) break;
             }
             // Phase B: the postponed leaf, then the leaves that follow it in order.
             if (leaf == kInvalid && inst != kInvalid && node >= cur_int && node < cur_total) {
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                 leaf = kInvalid;
                 break;
             }
-            if ((uint32_t)__popcll(__ballot(1)) < min_busy) break;
+            if (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy) break;  // refill (nothing left once drained)
         }
     }
 }
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                     leaf = node;
                     node = st.empty() ? kInvalid : st.pop();
                 }
-                if (!__any(leaf == kInvalid) || (uint32_t)__popcll(__ballot(1)) < min_busy) break;
+                if (!__any(leaf == kInvalid) || (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy)) break;
             }
             if (leaf == kInvalid && node != kInvalid && node >= n_int) {
                 leaf = node;
@@ -794,7 +794,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 leaf = kInvalid;
                 break;
             }
-            if ((uint32_t)__popcll(__ballot(1)) < min_busy) break;
+            if (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy) break;  // refill (nothing left once drained)
         }
     }
 }
