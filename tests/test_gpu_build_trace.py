@@ -35,6 +35,31 @@ def test_blas_build_matches_oracle(ctx, dev, n_tris, restructure):
     assert ok, msg
 
 
+def test_small_ray_counts(ctx, dev):
+    """The persistent kernel's queues on tiny launches (fewer rays than one
+    wave, than one fetch, partial last blocks): hits equal the oracle's for
+    every count, query and output."""
+    say("traversal of 1..1000 rays")
+    import torch
+
+    v, i = scenes.triangle_soup(5000, scenes.rng_for(9, 1))
+    accel, buf = _build(ctx, dev, v, i, True)
+    rng = scenes.rng_for(9, 2)
+    o = rng.uniform(-0.2, 1.2, size=(1000, 3)).astype(np.float32)
+    rays = scenes.make_rays(o, scenes.unit_sphere(1000, rng))
+    for count in (1, 2, 31, 32, 63, 64, 65, 127, 129, 255, 257, 1000):
+        r = rays[:count]
+        dr = to_dev(r, dev)
+        for q in ("closest", "any"):
+            ref = oracle.trace(buf, r, any_hit=(q == "any"))
+            hits = ctx.intersect(accel, dr, q, "full_t")
+            ids = ctx.intersect(accel, dr, q, "instance")
+            torch.cuda.synchronize()
+            ok, msg = hits_equal(api.decode_hits(hits, "full_t"), ref)
+            assert ok, f"{count} rays, {q}: {msg}"
+            assert np.array_equal(api.decode_hits(ids, "instance")["id"], ref["prim_id"]), f"{count} rays, {q}"
+
+
 def test_blas_build_duplicate_and_degenerate(ctx, dev):
     say("blas build with duplicate Morton codes and degenerate triangles")
     rng = scenes.rng_for(8)
