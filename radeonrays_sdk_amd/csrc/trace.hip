@@ -538,13 +538,25 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                     }
                 } else if (inst == kInvalid && node < cur_total) {
                     // Instance leaf: enter it (isect_2l.comp:222-237).
+                    // The transform is fetched with the BLAS address, and the BLAS
+                    // header's three fields together: two dependent round trips.
                     const uint32_t id = __float_as_uint(LdG4(&top_nodes[node].a0_max_or_v1).w);
-                    const uint8_t* blas = id < top_prims ? reinterpret_cast<const uint8_t*>(LdG(blas_table + id)) : nullptr;
-                    const AccelHeader* bh = reinterpret_cast<const AccelHeader*>(blas);
-                    const uint32_t bn = blas ? LdG(&bh->node_count) : 0u;
-                    if (bn != 0) {
+                    const uint8_t* blas = nullptr;
+                    float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0;
+                    uint32_t bn = 0, bflags = 0;
+                    uint64_t boff = 0;
+                    if (id < top_prims) {
                         const float4* xp = reinterpret_cast<const float4*>(xforms + 2 * (size_t)id);
-                        const float4 x0 = LdG4(xp), x1 = LdG4(xp + 1), x2 = LdG4(xp + 2);
+                        x0 = LdG4(xp); x1 = LdG4(xp + 1); x2 = LdG4(xp + 2);
+                        blas = reinterpret_cast<const uint8_t*>(LdG(blas_table + id));
+                        if (blas) {
+                            const AccelHeader* bh = reinterpret_cast<const AccelHeader*>(blas);
+                            bn = LdG(&bh->node_count);
+                            boff = LdG(&bh->nodes_offset);
+                            bflags = LdG(&bh->flags);
+                        }
+                    }
+                    if (bn != 0) {
                         const float wox = s_world[0][threadIdx.x], woy = s_world[1][threadIdx.x], woz = s_world[2][threadIdx.x];
                         const float wdx = s_world[3][threadIdx.x], wdy = s_world[4][threadIdx.x], wdz = s_world[5][threadIdx.x];
                         const float ox = ((x0.x * wox + x0.y * woy) + x0.z * woz) + x0.w;
@@ -555,10 +567,10 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                         const float dz = (x2.x * wdx + x2.y * wdy) + x2.z * wdz;
                         SetupRay(r, ox, oy, oz, r.tmin, dx, dy, dz);
                         inst = id;
-                        cur = reinterpret_cast<const BvhNode*>(blas + LdG(&bh->nodes_offset));
+                        cur = reinterpret_cast<const BvhNode*>(blas + boff);
                         cur_total = bn;
                         cur_int = (bn - 1) / 2;
-                        boxes = (LdG(&bh->flags) & kAccelFlagAabbs) != 0;
+                        boxes = (bflags & kAccelFlagAabbs) != 0;
                         st.push(kTopLevelSentinel);
                         node = 0;
                     } else {
