@@ -430,7 +430,10 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                                                                   uint32_t* __restrict__ scratch, uint32_t min_busy,
                                                                   uint32_t fetch) {
     __shared__ uint32_t s_stack[DEPTH][kBlock];
-    __shared__ float s_world[6][kBlock];  // each lane's world ray: origin xyz, direction xyz
+    // Each lane's world ray: origin xyz, direction xyz, and its set-up
+    // (reciprocal direction, shear, axis permutation), restored without
+    // divisions when the lane leaves an instance.
+    __shared__ float s_world[13][kBlock];
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
@@ -494,6 +497,9 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                 s_world[0][threadIdx.x] = ray.a.x; s_world[1][threadIdx.x] = ray.a.y; s_world[2][threadIdx.x] = ray.a.z;
                 s_world[3][threadIdx.x] = ray.b.x; s_world[4][threadIdx.x] = ray.b.y; s_world[5][threadIdx.x] = ray.b.z;
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
+                s_world[6][threadIdx.x] = r.ix; s_world[7][threadIdx.x] = r.iy; s_world[8][threadIdx.x] = r.iz;
+                s_world[9][threadIdx.x] = r.sx; s_world[10][threadIdx.x] = r.sy; s_world[11][threadIdx.x] = r.sz;
+                s_world[12][threadIdx.x] = __uint_as_float(r.k);
                 best_t = ray.b.w;
                 best_prim = kInvalid;
                 best_inst = kInvalid;
@@ -583,8 +589,10 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                     cur_int = top_int;
                     cur_total = top_total;
                     boxes = false;
-                    SetupRay(r, s_world[0][threadIdx.x], s_world[1][threadIdx.x], s_world[2][threadIdx.x], r.tmin,
-                             s_world[3][threadIdx.x], s_world[4][threadIdx.x], s_world[5][threadIdx.x]);
+                    r.ox = s_world[0][threadIdx.x]; r.oy = s_world[1][threadIdx.x]; r.oz = s_world[2][threadIdx.x];
+                    r.ix = s_world[6][threadIdx.x]; r.iy = s_world[7][threadIdx.x]; r.iz = s_world[8][threadIdx.x];
+                    r.sx = s_world[9][threadIdx.x]; r.sy = s_world[10][threadIdx.x]; r.sz = s_world[11][threadIdx.x];
+                    r.k = __float_as_uint(s_world[12][threadIdx.x]);
                     node = st.empty() ? kInvalid : st.pop();
                 } else if (inst != kInvalid && node < cur_total && leaf == kInvalid) {
                     leaf = node;
