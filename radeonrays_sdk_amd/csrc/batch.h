@@ -45,9 +45,10 @@ inline uint32_t RefitChunks(uint32_t tris) { return (tris + kRefitChunk - 1) / k
 
 // LBVH build of every mesh in the table (already in device memory at
 // `table`, inside `scratch`): each workgroup takes the next mesh from a
-// queue and runs bounds, Morton codes, a stable four-pass radix sort and
-// Karras emission for it alone; then one thread per leaf (in chunks of
-// kRefitChunk leaves of any mesh) writes the leaves and climbs.  The results equal BuildBlas with
+// queue and runs bounds, Morton codes and a stable four-pass radix sort for
+// it alone; then one thread per node of any mesh (in chunks of kRefitChunk
+// leaves) emits the hierarchy, and one thread per leaf writes the leaves and
+// climbs.  The results equal BuildBlas with
 // restructure off, mesh by mesh.  total_chunks = sum of RefitChunks.
 void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
 

@@ -1,12 +1,13 @@
 // batch.hip — batched BLAS build and refit for gfx950.
 //
-// Many small meshes (up to kBatchMaxTris triangles each) are built by one
-// persistent launch: each workgroup takes the next mesh from a queue (the
-// host orders the table largest first) and runs the whole LBVH pipeline of
-// the single-mesh builder for it with block-level synchronisation only —
-// bounds (wave64 reductions), Morton codes, a stable LSD radix sort of the
-// (code, index) pairs in four 8-bit passes, Karras emission and the
-// bottom-up refit.  Every step computes exactly what the single-mesh kernels
+// Many small meshes (up to kBatchMaxTris triangles each) are built by three
+// launches.  A persistent one in which each workgroup takes the next mesh
+// from a queue (the host orders the table largest first) and runs the
+// mesh-wide steps of the single-mesh builder with block-level
+// synchronisation only — bounds (wave64 reductions), Morton codes and a
+// stable LSD radix sort of the (code, index) pairs in four 8-bit passes;
+// then Karras emission and the bottom-up refit, one thread per node or leaf
+// of any mesh.  Every step computes exactly what the single-mesh kernels
 // compute (min/max are exact, the sort is stable, emission and refit are the
 // shared device functions), so each result equals BuildBlas of that mesh.
 // Behavioural spec of each step: as build.hip (vlk/hlbvh_builder.cpp:159-351).
@@ -154,7 +155,6 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
         const uint32_t n = bm.tri_count;
         uint8_t* result = RebaseOn<uint8_t>(queue, bm.result);
         AccelHeader* hdr = reinterpret_cast<AccelHeader*>(result);
-        BvhNode* nodes = reinterpret_cast<BvhNode*>(result + sizeof(AccelHeader));
         if (tid == 0) {
             // As InitBuildKernel.
             const uint32_t serial = hdr->magic == kAccelMagic ? hdr->build_serial + 1 : 1;
@@ -222,10 +222,32 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
             BlockSortPass(even ? keys : ktmp, even ? vals : vtmp, even ? ktmp : keys, even ? vtmp : vals, n,
                           (uint32_t)(p * kSortRadixBits), sh);
         }
-        __syncthreads();
-        for (uint32_t i = tid; i < n; i += kBlock) EmitNode(keys, vals, n, i, nodes);
-        __syncthreads();
+        __syncthreads();  // the hierarchy is emitted by BatchEmitKernel, one thread per node of all meshes
     }
+}
+
+// Karras emission of every mesh, one thread per leaf index (node i and
+// leaf i of its mesh), over the 64-leaf chunks of all meshes: each node's
+// binary searches are a chain of dependent loads, so the whole GPU's
+// threads run them rather than one workgroup per mesh.
+__global__ __launch_bounds__(kBlock) void BatchEmitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
+                                                          uint32_t total_chunks, uint32_t* scratch) {
+    const uint32_t c = blockIdx.x * (kBlock / kRefitChunk) + threadIdx.x / kRefitChunk;
+    if (c >= total_chunks) return;
+    uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (table[mid].first_chunk <= c) lo = mid;
+        else hi = mid;
+    }
+    const BatchMesh bm = table[lo];
+    const uint32_t i = (c - bm.first_chunk) * kRefitChunk + threadIdx.x % kRefitChunk;
+    if (c < bm.first_chunk || i >= bm.tri_count) return;
+    const uint32_t n = bm.tri_count;
+    const uint32_t* keys = RebaseOn<const uint32_t>(scratch, bm.work);  // sorted codes after the four passes
+    const uint32_t* vals = keys + AlignUp(n, 64);
+    BvhNode* nodes = reinterpret_cast<BvhNode*>(RebaseOn<uint8_t>(scratch, bm.result) + sizeof(AccelHeader));
+    EmitNode(keys, vals, n, i, nodes);
 }
 
 // Refit of every mesh in the table, one thread per leaf: thread t of block b
@@ -280,6 +302,10 @@ void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, 
     BatchQueueInitKernel<<<1, 64, 0, s>>>(queue);
     const uint32_t blocks = std::min(meshes, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildKernel), 0));
     BatchBuildKernel<<<blocks, kBlock, 0, s>>>(table, meshes, queue);
+    if (total_chunks) {
+        const uint32_t eblocks = (uint32_t)(((uint64_t)total_chunks * kRefitChunk + kBlock - 1) / kBlock);
+        BatchEmitKernel<<<eblocks, kBlock, 0, s>>>(table, meshes, total_chunks, queue);
+    }
     BatchRefit(table, meshes, total_chunks, scratch, s);
 }
 
