@@ -253,24 +253,39 @@ __global__ __launch_bounds__(kBlock) void BatchEmitKernel(const BatchMesh* __res
 // Refit of every mesh in the table, one thread per leaf: thread t of block b
 // takes leaf t % kRefitChunk of refit chunk b * (kBlock / kRefitChunk) +
 // t / kRefitChunk, in the mesh that owns the chunk (binary search over
-// first_chunk).  Chunks of one mesh run on many CUs at once and meet in
-// shared ancestors through ClimbRefit's agent-scope hand-off.
+// first_chunk).  The chunks of one mesh within a block form one refit window
+// (consecutive leaves); chunks of one mesh in different blocks meet in shared
+// ancestors through ClimbRefit's agent-scope hand-off.
 static_assert(kBlock % kRefitChunk == 0, "whole chunks per block");
 __global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
                                                            uint32_t total_chunks, uint32_t* scratch) {
-    const uint32_t c = blockIdx.x * (kBlock / kRefitChunk) + threadIdx.x / kRefitChunk;
-    if (c >= total_chunks) return;
-    uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (table[mid].first_chunk <= c) lo = mid;
-        else hi = mid;
+    __shared__ RefitLds<kBlock> lds;
+    constexpr uint32_t kChunksPerBlock = kBlock / kRefitChunk;
+    const uint32_t cb = blockIdx.x * kChunksPerBlock;
+    const uint32_t c = cb + threadIdx.x / kRefitChunk;
+    bool active = c < total_chunks;
+    uint32_t lo_m = 0, hi_m = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
+    while (active && hi_m - lo_m > 1) {
+        const uint32_t mid = (lo_m + hi_m) / 2;
+        if (table[mid].first_chunk <= c) lo_m = mid;
+        else hi_m = mid;
     }
-    const BatchMesh bm = table[lo];
+    BatchMesh bm{};
+    if (active) bm = table[lo_m];
+    active = active && c >= bm.first_chunk;
     const uint32_t i = (c - bm.first_chunk) * kRefitChunk + threadIdx.x % kRefitChunk;
-    if (c < bm.first_chunk || i >= bm.tri_count) return;
+    active = active && i < bm.tri_count;
+    // This mesh's chunks within the block: [first, last).
+    const uint32_t first = std::max(cb, bm.first_chunk);
+    const uint32_t last = std::min(cb + kChunksPerBlock, bm.first_chunk + (bm.tri_count + kRefitChunk - 1) / kRefitChunk);
     BvhNode* nodes = reinterpret_cast<BvhNode*>(RebaseOn<uint8_t>(scratch, bm.result) + sizeof(AccelHeader));
-    RefitLeaf(ViewOfBatch(bm, scratch), nodes, i);
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t parent = kInvalid;
+    if (active) parent = WriteLeaf(ViewOfBatch(bm, scratch), nodes, i, lo, hi);
+    const uint32_t n = bm.tri_count;
+    WindowRefit<kBlock>(nodes, 2 * n - 1, (first - bm.first_chunk) * kRefitChunk,
+                        active ? (last - first) * kRefitChunk : 0, (first - cb) * kRefitChunk, active,
+                        active && i + 1 < n ? i : kInvalid, n - 1 + i, parent, lo, hi, lds);
 }
 
 // Resident blocks of `kernel` on the current device, cached per device.

@@ -203,9 +203,18 @@ __global__ __launch_bounds__(kBlock) void EmitHierarchyKernel(const uint32_t* __
     if (i < n) EmitNode(codes, prims, n, i, nodes);
 }
 
+// One thread per leaf; block b's leaves (key positions [256 b, 256 b + 256))
+// form one refit window.
 __global__ __launch_bounds__(kBlock) void RefitMeshKernel(MeshView m, BvhNode* __restrict__ nodes) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < m.tri_count) RefitLeaf(m, nodes, i);
+    __shared__ RefitLds<kBlock> lds;
+    const uint32_t n = m.tri_count;
+    const uint32_t base = blockIdx.x * kBlock;
+    const uint32_t i = base + threadIdx.x;
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t parent = kInvalid;
+    if (i < n) parent = WriteLeaf(m, nodes, i, lo, hi);
+    WindowRefit<kBlock>(nodes, 2 * n - 1, base, kBlock, 0, i < n, i + 1 < n ? i : kInvalid, n - 1 + i, parent, lo, hi,
+                        lds);
 }
 
 // Refit of the BLAS in `result` (nodes after the header; topology and leaf
@@ -677,23 +686,31 @@ __global__ __launch_bounds__(kBlock) void TlasMortonKernel(const float* __restri
     MortonAndHistogram(n, head, centroid, keys, vals, passes, sort_hist, sort_status, status_words);
 }
 
+// One thread per instance leaf; block b's leaves form one refit window.
 __global__ __launch_bounds__(kBlock) void RefitSceneKernel(const float* __restrict__ boxes, uint32_t n,
                                                            BvhNode* __restrict__ nodes) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    __shared__ RefitLds<kBlock> lds;
+    const uint32_t base = blockIdx.x * kBlock;
+    const uint32_t i = base + threadIdx.x;
     const uint32_t leaf = n - 1 + i;
-    const uint32_t id = nodes[leaf].child1;
-    const uint32_t parent = nodes[leaf].parent;
-    const float4 blo = reinterpret_cast<const float4*>(boxes)[2 * (size_t)id];
-    const float4 bhi = reinterpret_cast<const float4*>(boxes)[2 * (size_t)id + 1];
-    // A TLAS leaf stores the instance's world box twice (lbvh_fit_aabb_scene.comp:125-128).
-    float4* q = reinterpret_cast<float4*>(nodes + leaf);
-    q[0] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(kInvalid));
-    q[1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(id));
-    q[2] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(parent));
-    q[3] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(0u));
-    float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
-    ClimbRefit(nodes, 2 * n - 1, leaf, parent, lo, hi);
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t parent = kInvalid;
+    if (i < n) {
+        const uint32_t id = nodes[leaf].child1;
+        parent = nodes[leaf].parent;
+        const float4 blo = reinterpret_cast<const float4*>(boxes)[2 * (size_t)id];
+        const float4 bhi = reinterpret_cast<const float4*>(boxes)[2 * (size_t)id + 1];
+        // A TLAS leaf stores the instance's world box twice (lbvh_fit_aabb_scene.comp:125-128).
+        float4* q = reinterpret_cast<float4*>(nodes + leaf);
+        q[0] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(kInvalid));
+        q[1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(id));
+        q[2] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(parent));
+        q[3] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(0u));
+        lo[0] = blo.x; lo[1] = blo.y; lo[2] = blo.z;
+        hi[0] = bhi.x; hi[1] = bhi.y; hi[2] = bhi.z;
+    }
+    WindowRefit<kBlock>(nodes, 2 * n - 1, base, kBlock, 0, i < n, i + 1 < n ? i : kInvalid, leaf, parent, lo, hi,
+                        lds);
 }
 
 }  // namespace

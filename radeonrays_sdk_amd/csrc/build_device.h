@@ -185,8 +185,10 @@ __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t
     }
 }
 
-// Leaf i: write its triangle (and links) into the leaf node, then climb.
-__device__ __forceinline__ void RefitLeaf(const MeshView& m, BvhNode* __restrict__ nodes, uint32_t i) {
+// Leaf i: write its triangle (and links) into the leaf node; returns the
+// leaf's parent and the triangle's box.
+__device__ __forceinline__ uint32_t WriteLeaf(const MeshView& m, BvhNode* __restrict__ nodes, uint32_t i, float lo[3],
+                                              float hi[3]) {
     const uint32_t n = m.tri_count;
     const uint32_t leaf = n - 1 + i;
     const uint32_t prim = nodes[leaf].child1;
@@ -198,12 +200,98 @@ __device__ __forceinline__ void RefitLeaf(const MeshView& m, BvhNode* __restrict
     q[1] = make_float4(v[3], v[4], v[5], __uint_as_float(prim));
     q[2] = make_float4(v[6], v[7], v[8], __uint_as_float(parent));
     q[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-    float lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
         lo[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
         hi[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
     }
-    ClimbRefit(nodes, 2 * n - 1, leaf, parent, lo, hi);
+    return parent;
+}
+
+// Windowed refit.  A workgroup of W threads refits leaves whose key
+// positions form windows of consecutive positions [base, base + width) and
+// keeps in LDS, for every internal node whose index lies in a window, an
+// arrival counter and the node's two child boxes.  In the Karras layout an
+// internal node's index is one end of its key range, so every node whose
+// subtree lies inside a window has its index in that window: those nodes are
+// merged in LDS, with no global atomic and no wait for a global store.  A
+// climb stops at the first node outside its window, or at an in-window node
+// where it arrived first; after the workgroup barrier, a stop whose node saw
+// only that one local arrival (the sibling subtree crosses the window edge)
+// and every out-of-window stop go on with ClimbRefit's global hand-off, which
+// every arrival at a node whose subtree crosses a window edge uses.  Boxes
+// are min/max, so the result does not depend on which thread merges them.
+// The `update` counter advances by two per refit on the nodes that take the
+// global hand-off and stays unchanged on the others (even on all of them).
+// The window's links (child0, parent of each in-window internal node) are
+// read into LDS once, one node per thread, so the climb inside the window
+// makes no global load.
+template <int W>
+struct RefitLds {
+    uint32_t count[W];
+    uint32_t child0[W];
+    uint32_t parent[W];
+    float box[2][6][W];  // [child side][lo xyz, hi xyz][slot]
+};
+
+// Every thread of the workgroup calls this (inactive ones with
+// active = false): it holds a barrier.  Thread t owns LDS slot t, which is
+// the slot of internal node `own` (its leaf's key position; pass kInvalid
+// when the thread has no in-window node); slot0 = LDS slot of the window's
+// first position; `total` = node count of the tree.
+template <int W>
+__device__ __forceinline__ void WindowRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t base, uint32_t width,
+                                            uint32_t slot0, bool active, uint32_t own, uint32_t leaf, uint32_t parent,
+                                            float lo[3], float hi[3], RefitLds<W>& s) {
+    s.count[threadIdx.x] = 0;
+    uint32_t own_c0 = kInvalid, own_up = kInvalid;
+    if (own < total) {
+        own_c0 = nodes[own].child0;
+        own_up = nodes[own].parent;
+    }
+    s.child0[threadIdx.x] = own_c0;
+    s.parent[threadIdx.x] = own_up;
+    __syncthreads();
+    uint32_t child = leaf, p = parent, l = 0;
+    int state = active ? 1 : 0;  // 0 done, 1 climbing in LDS, 2 global climb from (child, p), 3 first arrival at p
+    for (uint32_t level = 0; state == 1; ++level) {
+        if (p - base >= width || p >= total) {
+            state = 2;
+            break;
+        }
+        if (level >= width) {  // more levels than a window's subtree has: corrupt links
+            state = 0;
+            break;
+        }
+        l = slot0 + (p - base);
+        BvhNode* pn = nodes + p;
+        const uint32_t c0 = s.child0[l];
+        const uint32_t up = s.parent[l];
+        const int side = c0 == child ? 0 : 1;
+        StoreBoxAgent(side ? pn->a1_min_or_v2 : pn->a0_min_or_v0, lo[0], lo[1], lo[2]);
+        StoreBoxAgent(side ? pn->a1_max_or_v3 : pn->a0_max_or_v1, hi[0], hi[1], hi[2]);
+        for (int a = 0; a < 3; ++a) {
+            s.box[side][a][l] = lo[a];
+            s.box[side][3 + a][l] = hi[a];
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the boxes are in LDS before the count moves
+        const uint32_t old = __hip_atomic_fetch_add(&s.count[l], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0) {
+            state = 3;
+            break;
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], s.box[side ^ 1][a][l]);
+            hi[a] = fmaxf(hi[a], s.box[side ^ 1][3 + a][l]);
+        }
+        child = p;
+        p = up;
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (state == 3 && s.count[l] == 1) state = 2;
+    if (state == 2) ClimbRefit(nodes, total, child, p, lo, hi);
 }
 
 }  // namespace rr
