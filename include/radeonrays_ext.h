@@ -58,7 +58,8 @@ typedef enum RRxTuning
                                              persistent kernels (default 0, or 1 when the
                                              environment sets RR_TRACE_MODE=plain) */
     RRX_TUNING_SORT_DIGIT_BITS = 4,       /* digit width of rrxCmdSortPairs on >= 2^20 keys:
-                                             8 (default) or 10 (three passes for 30-bit keys) */
+                                             8 (default), 10 (three passes for 30-bit keys)
+                                             or 4 (the reference's radix, eight passes) */
     RRX_TUNING_TREELET_LEAVES = 5         /* leaves per treelet of the restructuring that builds
                                              without RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD run:
                                              5, 7 (default, the reference's) or 9; changes the

@@ -118,7 +118,7 @@ def tune_traversal(ctx, accel, rays, queries=("closest", "any"), reps: int = 5, 
     return {"best": best, "default": default, "table": table}
 
 
-DIGIT_BITS = (8, 10)
+DIGIT_BITS = (4, 8, 10)
 
 
 def tune_sort(ctx, n: int = 16 << 20, reps: int = 5, digit_bits=DIGIT_BITS, seed: int = 0x52520005) -> dict:

@@ -96,7 +96,7 @@ __device__ void BlockSortPass(const uint32_t* __restrict__ ki, const uint32_t* _
             uint32_t m_lo = (uint32_t)valid, m_hi = (uint32_t)(valid >> 32);
 #pragma unroll
             for (int b = 0; b < kSortRadixBits; ++b) {
-                const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)k[j], shift + b, 1);
+                const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);  // bit b of the digit
                 const uint64_t bal = __ballot(neg != 0u);
                 m_lo &= ~((uint32_t)bal ^ neg);
                 m_hi &= ~((uint32_t)(bal >> 32) ^ neg);

@@ -20,24 +20,29 @@ constexpr int kSortMaxPasses = 4;
 // thousands of blocks by that factor; readers sum the copies.
 constexpr int kSortHistCopies = 8;
 constexpr int kSortHistWords = kSortMaxPasses * kSortRadix;  // 8-bit digits: words used in each copy
-// Large sorts may use 10-bit digits (three passes for 30-bit keys); a copy
-// is laid out for the larger of the two.
+// Large sorts may use 10-bit digits (three passes for 30-bit keys, four for
+// 32-bit keys) or the reference's 4-bit digits (eight passes for 32-bit
+// keys); a copy is laid out for the largest (4 << 10 >= 4 << 8 >= 8 << 4).
 constexpr int kSortWideBits = 10;
-constexpr int kSortWidePasses = 3;
-constexpr int kSortHistStride = kSortWidePasses << kSortWideBits;  // words per copy (3072 >= 1024)
+constexpr int kSortWidePasses = 4;
+constexpr int kSortNarrowBits = 4;
+constexpr int kSortNarrowPasses = 8;
+constexpr int kSortHistStride = kSortWidePasses << kSortWideBits;  // words per copy
+static_assert(kSortHistStride >= (kSortNarrowPasses << kSortNarrowBits), "histogram copy too small");
+static_assert(kSortHistStride >= (kSortMaxPasses << kSortRadixBits), "histogram copy too small");
 
 // Tile shape of one pass kernel.  Small inputs use small tiles so that the
 // grid still spreads over all CUs.
 struct SortShape {
     int block;   // threads per tile
     int items;   // keys per thread
-    int bits;    // digit width: 8, or 10 (1024-thread tiles)
+    int bits;    // digit width: 8, or 10 or 4 (1024-thread tiles)
     int tile() const { return block * items; }
 };
 SortShape ChooseSortShape(uint32_t n);
 
 // Digit width of sorts of at least 2^20 keys whose callers did not prepare
-// the histogram (rrxCmdSortPairs): 8 (default) or 10.  Process-wide.
+// the histogram (rrxCmdSortPairs): 8 (default), 10 or 4.  Process-wide.
 int SortDigitBits();
 bool SetSortDigitBits(int bits);
 
@@ -50,7 +55,7 @@ __host__ __device__ inline int SortPasses(int key_bits) { return (key_bits + kSo
 
 struct SortScratch {
     uint32_t* hist;     // [kSortHistCopies][kSortMaxPasses][256] global digit counts
-    uint32_t* tickets;  // [kSortMaxPasses] tile tickets
+    uint32_t* tickets;  // [kSortNarrowPasses] tile tickets, one per pass
     uint32_t* status;   // [kSortMaxPasses][tiles][256] look-back words
     size_t status_words;  // words of pass 0's look-back region, which must be zero before a sort
 };

@@ -66,7 +66,7 @@ __device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* s_w
 __global__ __launch_bounds__(kBlock) void SortClearKernel(uint32_t* hist, uint32_t* tickets, uint32_t words) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < kSortHistCopies * words; i += gridDim.x * kBlock)
         hist[(i / words) * kSortHistStride + i % words] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < kSortMaxPasses) tickets[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kSortNarrowPasses) tickets[threadIdx.x] = 0;
 }
 
 // Digit counts of every pass in one read of the keys: each thread takes 16
@@ -83,7 +83,8 @@ template <int BITS>
 __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __restrict__ keys, uint32_t n, int passes,
                                                               uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
                                                               size_t status_words) {
-    constexpr int kWords = (BITS == 8 ? kSortMaxPasses : kSortWidePasses) << BITS;
+    constexpr int kWords = ((32 + BITS - 1) / BITS) << BITS;  // every pass of 32-bit keys
+    static_assert(kWords <= kSortHistStride, "histogram copy too small");
     __shared__ uint32_t s_h[kWords];
     for (uint32_t i = threadIdx.x; i < (uint32_t)kWords; i += kBlock) s_h[i] = 0;
     __syncthreads();
@@ -124,9 +125,9 @@ __global__ __launch_bounds__(kBlock) void SortHistogramKernel(const uint32_t* __
 // tiles (1024 threads x 16 keys) one tile fills a CU's LDS and at most 128
 // VGPRs keep all 16 waves resident.
 //
-// BITS is the digit width: 8, or 10 with 1024-thread tiles (three passes
+// BITS is the digit width: 8, or with 1024-thread tiles 10 (three passes
 // for 30-bit keys instead of four; digit runs get shorter, so tiles stay
-// large).
+// large) or 4 (the reference's radix: eight passes).
 template <int BLOCK, int ITEMS, int BITS = kSortRadixBits>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
@@ -233,7 +234,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         uint32_t m_lo = ~0u, m_hi = ~0u;
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
-            const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)k[j], shift + b, 1);  // 0 or ~0
+            // Bit b of the digit (not of the key: shift + b may pass bit 31).
+            const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);  // 0 or ~0
             const uint64_t bal = __ballot(neg != 0u);
             m_lo &= ~((uint32_t)bal ^ neg);
             m_hi &= ~((uint32_t)(bal >> 32) ^ neg);
@@ -297,6 +299,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // Tile shapes the pass kernel is instantiated for.
 static bool KnownShape(int block, int items, int bits) {
     if (bits == kSortWideBits) return block == 1024 && (items == 8 || items == 4);
+    if (bits == kSortNarrowBits) return block == 1024 && items == 16;
     return bits == 8 && ((block == 1024 && (items == 16 || items == 8)) || (block == 512 && (items == 16 || items == 8)) ||
                          (block == 256 && (items == 16 || items == 8 || items == 4)));
 }
@@ -304,16 +307,17 @@ static bool KnownShape(int block, int items, int bits) {
 static std::atomic<int> g_digit_bits{8};
 int SortDigitBits() { return g_digit_bits.load(std::memory_order_relaxed); }
 bool SetSortDigitBits(int bits) {
-    if (bits != 8 && bits != kSortWideBits) return false;
+    if (bits != 8 && bits != kSortWideBits && bits != kSortNarrowBits) return false;
     g_digit_bits.store(bits, std::memory_order_relaxed);
     return true;
 }
 
-// Inputs at least this large may take 10-bit digits (8192-key tiles).
+// Inputs at least this large may take 10-bit digits (8192-key tiles) or
+// 4-bit digits (16384-key tiles).
 constexpr uint32_t kWideMinKeys = 1u << 20;
 
-// Shape of a sort of n keys.  wide_ok: 10-bit digits are allowed (not for
-// callers that prepared an 8-bit histogram themselves).
+// Shape of a sort of n keys.  wide_ok: digits other than 8 bits are allowed
+// (not for callers that prepared an 8-bit histogram themselves).
 static SortShape ChooseShape(uint32_t n, bool wide_ok) {
     // RR_SORT_SHAPE=<threads>x<keys per thread>[x<digit bits>] (one of
     // KnownShape) fixes the shape of sorts that allow it (tuning runs).
@@ -325,6 +329,7 @@ static SortShape ChooseShape(uint32_t n, bool wide_ok) {
     }();
     if (forced.block && (wide_ok || forced.bits == 8)) return forced;
     if (wide_ok && n >= kWideMinKeys && SortDigitBits() == kSortWideBits) return {1024, 8, kSortWideBits};
+    if (wide_ok && n >= kWideMinKeys && SortDigitBits() == kSortNarrowBits) return {1024, 16, kSortNarrowBits};
     // Large inputs: 16384-key tiles (fewer tiles, so fewer look-back rounds,
     // and 64-key digit runs per tile on average, i.e. whole cache lines).
     if (n >= (1u << 22)) return {1024, 16, 8};
@@ -346,7 +351,10 @@ size_t SortScratchBytes(uint32_t n) {
     // Enough for any shape a sort of n keys may take, whatever the digit
     // width is set to when it runs.
     size_t status = std::max(StatusWords(n, ChooseShape(n, true)), StatusWords(n, ChooseShape(n, false)));
-    if (n >= kWideMinKeys) status = std::max(status, StatusWords(n, SortShape{1024, 8, kSortWideBits}));
+    if (n >= kWideMinKeys) {
+        status = std::max(status, StatusWords(n, SortShape{1024, 8, kSortWideBits}));
+        status = std::max(status, StatusWords(n, SortShape{1024, 16, kSortNarrowBits}));
+    }
     const size_t words = (size_t)kSortHistCopies * kSortHistStride + 64 + status;
     return AlignUp(words * 4, 256);
 }
@@ -384,6 +392,9 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         const size_t status0 = tiles * radix;
         if (shape.bits == kSortWideBits)
             SortHistogramKernel<kSortWideBits><<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, status0);
+        else if (shape.bits == kSortNarrowBits)
+            SortHistogramKernel<kSortNarrowBits><<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status,
+                                                                               status0);
         else
             SortHistogramKernel<8><<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, status0);
     }
@@ -398,6 +409,8 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         if (shape.bits == kSortWideBits) {
             if (shape.items == 8) RR_SORT_PASS(1024, 8, kSortWideBits);
             else RR_SORT_PASS(1024, 4, kSortWideBits);
+        } else if (shape.bits == kSortNarrowBits) {
+            RR_SORT_PASS(1024, 16, kSortNarrowBits);
         } else if (shape.block == 1024) {
             if (shape.items == 16) RR_SORT_PASS(1024, 16, 8);
             else RR_SORT_PASS(1024, 8, 8);

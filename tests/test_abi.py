@@ -133,7 +133,10 @@ def test_tuning_validation():
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_PLAIN, byref(v)) == rr.RR_SUCCESS and v.value == 1
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, before) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 8
-    assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, 4) == rr.RR_ERROR_INVALID_PARAMETER
+    for bad in (0, 6, 9, 16):
+        assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, bad) == rr.RR_ERROR_INVALID_PARAMETER
+    assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, 4) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 4
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, 10) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 10
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, 8) == rr.RR_SUCCESS

@@ -206,12 +206,15 @@ def test_sort_pairs(ctx, dev, n):
     assert np.array_equal(v.cpu().numpy().view(np.uint32), vals[order])
 
 
-@pytest.mark.parametrize("n", [1 << 20, (1 << 20) + 12345, 5000000])
-def test_sort_pairs_ten_bit_digits(ctx, dev, n):
+@pytest.mark.parametrize("bits,n,key_bits", [(10, 1 << 20, 30), (10, (1 << 20) + 12345, 30), (10, 5000000, 30),
+                                             (10, (1 << 20) + 777, 32), (10, 3000000, 31),
+                                             (4, 1 << 20, 30), (4, (1 << 20) + 12345, 32)])
+def test_sort_pairs_other_digit_widths(ctx, dev, bits, n, key_bits):
     """Sorts of at least 2^20 keys with 10-bit digits (three passes for
-    30-bit keys): the same stable order as numpy, including a partial last
-    tile and many duplicate keys."""
-    say(f"onesweep sort with 10-bit digits, n={n}")
+    30-bit keys, four for 31/32-bit keys) and with 4-bit digits (eight
+    passes): the same stable order as numpy, including a partial last tile
+    and many duplicate keys."""
+    say(f"onesweep sort with {bits}-bit digits, n={n}, {key_bits}-bit keys")
     import torch
 
     from radeonrays_sdk_amd import autotune
@@ -219,14 +222,14 @@ def test_sort_pairs_ten_bit_digits(ctx, dev, n):
     lib = ctx.lib
     start = autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS)
     try:
-        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, 10)
+        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, bits)
         rng = np.random.default_rng(n)
-        keys = rng.integers(0, 1 << 30, size=n, dtype=np.uint32)
+        keys = rng.integers(0, 1 << key_bits, size=n, dtype=np.uint64).astype(np.uint32)
         keys[: n // 4] = keys[7]
         vals = np.arange(n, dtype=np.uint32)
         k = to_dev(keys.view(np.int32), dev)
         v = to_dev(vals.view(np.int32), dev)
-        ctx.sort_pairs(k, v, 30)
+        ctx.sort_pairs(k, v, key_bits)
         torch.cuda.synchronize()
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(k.cpu().numpy().view(np.uint32), keys[order])
