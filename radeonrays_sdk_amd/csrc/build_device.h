@@ -157,13 +157,24 @@ __device__ __forceinline__ void LoadBoxesAgent(const float* a, const float* b, f
     hi[0] = y.x; hi[1] = y.y; hi[2] = y.z;
 }
 
+// A node's links (child0, parent), which no refit changes, loaded together.
+__device__ __forceinline__ void LoadLinks(const BvhNode* nd, uint32_t& c0, uint32_t& up) {
+    c0 = nd->child0;
+    up = nd->parent;
+}
+
+// Three dependent memory round trips per level: the slot stores and their
+// completion, the counter atomic, and the sibling's boxes loaded together
+// with the next node's links.
 __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t child, uint32_t p,
                                            float lo[3], float hi[3]) {
     // Guards: a parent index outside the node array or more levels than any
     // tree of `total` nodes can have stops the climb instead of faulting.
-    for (uint32_t level = 0; p != kInvalid && p < total && level < total; ++level) {
+    if (p == kInvalid || p >= total) return;
+    uint32_t c0, up;
+    LoadLinks(nodes + p, c0, up);
+    for (uint32_t level = 0; level < total; ++level) {
         BvhNode* pn = nodes + p;
-        const uint32_t c0 = pn->child0;
         float* slot_lo = c0 == child ? pn->a0_min_or_v0 : pn->a1_min_or_v2;
         float* slot_hi = c0 == child ? pn->a0_max_or_v1 : pn->a1_max_or_v3;
         const float* other_lo = c0 == child ? pn->a1_min_or_v2 : pn->a0_min_or_v0;
@@ -174,14 +185,20 @@ __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t
         const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((old & 1u) == 0) return;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const bool next = up != kInvalid && up < total;
+        uint32_t nc0 = kInvalid, nup = kInvalid;
+        if (next) LoadLinks(nodes + up, nc0, nup);  // in flight with the box loads below
         float olo[3], ohi[3];
         LoadBoxesAgent(other_lo, other_hi, olo, ohi);
         for (int a = 0; a < 3; ++a) {
             lo[a] = fminf(lo[a], olo[a]);
             hi[a] = fmaxf(hi[a], ohi[a]);
         }
+        if (!next) return;
         child = p;
-        p = pn->parent;
+        p = up;
+        c0 = nc0;
+        up = nup;
     }
 }
 
