@@ -1159,9 +1159,10 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
         const uint32_t n_small = (uint32_t)plan.small.size();
         rr::BatchMesh* host = nullptr;
         if (n_small) host = static_cast<rr::BatchMesh*>(AcquireStaging(c, cs, sizeof(rr::BatchMesh) * n_small));
-        uint32_t chunks = 0;
+        uint32_t chunks = 0, big = 0;
         for (uint32_t k = 0; k < n_small; ++k) {
             const rr::MeshDesc& m = plan.meshes[plan.small[k]];
+            if (m.tri_count > rr::kBatchLdsMaxTris) ++big;  // a prefix: the table is ordered largest first
             rr::BatchMesh b{};
             b.vertices = reinterpret_cast<uint64_t>(m.vertices);
             b.indices = reinterpret_cast<uint64_t>(m.indices);
@@ -1176,6 +1177,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
             chunks += rr::RefitChunks(m.tri_count);
             host[k] = b;
         }
+        const uint32_t big_chunks = big < n_small ? host[big].first_chunk : chunks;  // chunks of the big meshes
         const size_t roots_offset = plan.roots_offset;
         const uint32_t max_roots[3] = {plan.max_roots[0], plan.max_roots[1], plan.max_roots[2]};
         auto cmd = [=](hipStream_t s) {
@@ -1186,7 +1188,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
                 if (update) {
                     rr::BatchRefit(table, n_small, chunks, scratch, s);
                 } else {
-                    rr::BatchBuild(table, n_small, chunks, scratch, s);
+                    rr::BatchBuild(table, n_small, big, big_chunks, chunks, scratch, s);
                     // Treelet restructuring of all meshes at once, each in its own work area.
                     if (restructure)
                         rr::BatchRestructure(table, n_small, chunks, max_roots, scratch,

@@ -43,14 +43,22 @@ inline size_t BatchHeadBytes(uint32_t meshes) {
 constexpr uint32_t kRefitChunk = 64;
 inline uint32_t RefitChunks(uint32_t tris) { return (tris + kRefitChunk - 1) / kRefitChunk; }
 
+// Meshes of at most this many triangles are built with their whole sort in
+// LDS (one 1024-thread workgroup per mesh).
+constexpr uint32_t kBatchLdsMaxTris = 16384;
+
 // LBVH build of every mesh in the table (already in device memory at
-// `table`, inside `scratch`): each workgroup takes the next mesh from a
-// queue and runs bounds, Morton codes and a stable four-pass radix sort for
-// it alone; then one thread per node of any mesh (in chunks of kRefitChunk
-// leaves) emits the hierarchy, and one thread per leaf writes the leaves and
-// climbs.  The results equal BuildBlas with
-// restructure off, mesh by mesh.  total_chunks = sum of RefitChunks.
-void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t stream);
+// `table`, inside `scratch`).  The first `big` meshes (those above
+// kBatchLdsMaxTris triangles; the table is ordered largest first): each
+// workgroup takes the next mesh from a queue and runs bounds, Morton codes
+// and a stable four-pass radix sort through the mesh's work area; then one
+// thread per node of these meshes (in chunks of kRefitChunk leaves, the
+// first big_chunks chunks) emits the hierarchy.  Every other mesh is sorted
+// by one workgroup in LDS, which then emits its hierarchy from the
+// LDS-resident codes.  Last, BatchRefit over all total_chunks chunks.  The
+// results equal BuildBlas with restructure off, mesh by mesh.
+void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t big_chunks, uint32_t total_chunks,
+                void* scratch, hipStream_t stream);
 
 // Refit of every mesh in the table (the topology of each BLAS is kept), as
 // UpdateBlas mesh by mesh (one launch over the refit chunks of all meshes).
@@ -64,6 +72,7 @@ void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, 
 // over meshes of tris / (64 << r)); its counter is scratch word
 // kBatchRootsCounterWord.
 constexpr uint32_t kBatchRootsCounterWord = 1;
+constexpr uint32_t kBatchLdsQueueWord = 2;  // queue of the LDS-sorted meshes
 void BatchRestructure(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, const uint32_t max_roots[3],
                       void* scratch, void* roots, hipStream_t stream);
 

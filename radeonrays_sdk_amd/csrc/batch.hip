@@ -30,6 +30,8 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kChunkItems = 4;  // keys per thread per sort chunk
 constexpr int kChunk = kBlock * kChunkItems;
 
+// Exclusive scan over a workgroup of WAVES waves; contains two barriers.
+template <int WAVES = kWaves>
 __device__ __forceinline__ uint32_t BlockScan(uint32_t v, uint32_t* s_wave_tot) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t x = v;
@@ -42,7 +44,7 @@ __device__ __forceinline__ uint32_t BlockScan(uint32_t v, uint32_t* s_wave_tot) 
     __syncthreads();
     uint32_t add = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) add += (w < (int)wave) ? s_wave_tot[w] : 0u;
+    for (int w = 0; w < WAVES; ++w) add += (w < (int)wave) ? s_wave_tot[w] : 0u;
     __syncthreads();
     return x - v + add;
 }
@@ -137,7 +139,8 @@ __device__ void BlockSortPass(const uint32_t* __restrict__ ki, const uint32_t* _
 }
 
 __global__ void BatchQueueInitKernel(uint32_t* queue) {
-    if (threadIdx.x == 0) *queue = 0;
+    if (threadIdx.x == 0) queue[0] = 0;
+    if (threadIdx.x == kBatchLdsQueueWord) queue[kBatchLdsQueueWord] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
@@ -226,6 +229,183 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
     }
 }
 
+// Meshes of at most kBatchLdsMaxTris triangles: one 1024-thread workgroup
+// per mesh computes bounds and Morton codes, and runs the four 8-bit passes
+// of the stable sort with keys and values resident in LDS (128 KB); only
+// the sorted pairs are written out.  Item j of lane l in wave w is position
+// w * (64 * kLdsItems) + 64 j + l, and ranks follow that order (waves, then
+// items, then lanes), so each pass is stable and the result equals the
+// single-mesh sort of the same (code, index) pairs.  kBatchLdsMaxTris is a
+// multiple of 64, so keys padded to 64 stay within LDS.
+constexpr int kLdsBlock = 1024;
+constexpr int kLdsWaves = kLdsBlock / 64;
+constexpr int kLdsItems = 16;
+static_assert(kLdsBlock * kLdsItems == (int)kBatchLdsMaxTris, "one item per triangle");
+
+struct BatchLdsShared {
+    uint32_t keys[kBatchLdsMaxTris];
+    uint32_t vals[kBatchLdsMaxTris];
+    uint32_t wave_hist[kLdsWaves][kSortRadix];
+    uint32_t digit_start[kSortRadix];
+    uint32_t bounds[kLdsWaves][6];
+    uint32_t wave_tot[kLdsWaves];
+    uint32_t mesh;
+};
+
+__global__ __launch_bounds__(kLdsBlock) void BatchBuildLdsKernel(const BatchMesh* __restrict__ table, uint32_t first,
+                                                                 uint32_t meshes, uint32_t* queue) {
+    __shared__ BatchLdsShared sh;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t wbase = wave * (64 * kLdsItems);  // position of this wave's item 0, lane 0
+    for (;;) {
+        if (tid == 0) sh.mesh = first + atomicAdd(queue + kBatchLdsQueueWord, 1u);
+        __syncthreads();
+        const uint32_t mi = sh.mesh;
+        __syncthreads();
+        if (mi >= meshes) break;
+        const BatchMesh bm = table[mi];
+        const MeshView m = ViewOfBatch(bm, queue);
+        uint8_t* result = RebaseOn<uint8_t>(queue, bm.result);
+        AccelHeader* hdr = reinterpret_cast<AccelHeader*>(result);
+        if (tid == 0) {
+            // As InitBuildKernel.
+            const uint32_t serial = hdr->magic == kAccelMagic ? hdr->build_serial + 1 : 1;
+            AccelHeader h = {};
+            h.magic = kAccelMagic;
+            h.kind = kAccelBlas;
+            h.prim_count = bm.tri_count;
+            h.node_count = NodeCount(bm.tri_count);
+            h.nodes_offset = sizeof(AccelHeader);
+            h.flags = bm.flags;
+            h.build_serial = serial;
+            *hdr = h;
+        }
+        // The host routes larger meshes elsewhere; the clamp keeps a bad
+        // table within LDS.
+        const uint32_t n = min(bm.tri_count, kBatchLdsMaxTris);
+        if (n == 0) continue;
+
+        // Bounds of all vertices (ordered-uint min/max: exact).
+        uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+        for (uint32_t t = tid; t < n; t += kLdsBlock) {
+            float v[9];
+            LoadTriangle(m, t, v);
+            for (int c = 0; c < 3; ++c)
+                for (int a = 0; a < 3; ++a) {
+                    const uint32_t o = FloatToOrdered(v[3 * c + a]);
+                    mn[a] = min(mn[a], o);
+                    mx[a] = max(mx[a], o);
+                }
+        }
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = WaveMin(mn[a]);
+            mx[a] = WaveMax(mx[a]);
+        }
+        if (lane == 0)
+            for (int a = 0; a < 3; ++a) {
+                sh.bounds[wave][a] = mn[a];
+                sh.bounds[wave][3 + a] = mx[a];
+            }
+        __syncthreads();
+        float lo[3], ext[3];
+        for (int a = 0; a < 3; ++a) {
+            uint32_t l = sh.bounds[0][a], h = sh.bounds[0][3 + a];
+            for (int w = 1; w < kLdsWaves; ++w) {
+                l = min(l, sh.bounds[w][a]);
+                h = max(h, sh.bounds[w][3 + a]);
+            }
+            lo[a] = OrderedToFloat(l);
+            ext[a] = OrderedToFloat(h) - lo[a];
+        }
+        // Morton codes of the triangle-box centroids into LDS, values =
+        // triangle index (the mesh's second read of its triangles).  Keys are
+        // padded to a multiple of 64 with all-ones keys: above every 30-bit
+        // code, in the last digit of every pass, so a stable pass keeps them
+        // after every real key, at positions [n, n64).
+        const uint32_t n64 = (n + 63u) & ~63u;
+        for (uint32_t t = tid; t < n64; t += kLdsBlock) {
+            uint32_t key = 0xFFFFFFFFu;
+            if (t < n) {
+                float c[3];
+                TriangleCentroid(m, t, c);
+                key = Morton30(UnitCoord(c[0], lo[0], ext[0]), UnitCoord(c[1], lo[1], ext[1]),
+                               UnitCoord(c[2], lo[2], ext[2]));
+            }
+            sh.keys[t] = key;
+            sh.vals[t] = t;
+        }
+
+        for (int p = 0; p < SortPasses(30); ++p) {
+            const uint32_t shift = (uint32_t)(p * kSortRadixBits);
+            __syncthreads();  // the previous step's LDS writes are done
+            // This wave's items are positions wbase + 64 j + lane for
+            // j < items (whole 64-key chunks below n64).
+            const uint32_t items = wbase < n64 ? min((n64 - wbase) / 64u, (uint32_t)kLdsItems) : 0u;
+            uint32_t k[kLdsItems], v[kLdsItems], rank[kLdsItems];
+#pragma unroll
+            for (int j = 0; j < kLdsItems; ++j) {
+                k[j] = (uint32_t)j < items ? sh.keys[wbase + 64u * j + lane] : 0xFFFFFFFFu;
+                rank[j] = 0;
+            }
+            for (uint32_t i = tid; i < kLdsWaves * kSortRadix; i += kLdsBlock) (&sh.wave_hist[0][0])[i] = 0;
+            __syncthreads();
+            // Rank within the wave: lanes with the same digit are the AND
+            // over the digit's bits of the bit's ballot or its complement.
+#pragma unroll
+            for (int j = 0; j < kLdsItems; ++j) {
+                if ((uint32_t)j >= items) continue;  // uniform in the wave
+                const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
+                uint32_t m_lo = ~0u, m_hi = ~0u;
+#pragma unroll
+                for (int b = 0; b < kSortRadixBits; ++b) {
+                    const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);  // bit b of the digit
+                    const uint64_t bal = __ballot(neg != 0u);
+                    m_lo &= ~((uint32_t)bal ^ neg);
+                    m_hi &= ~((uint32_t)(bal >> 32) ^ neg);
+                }
+                const uint32_t before = __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
+                const uint32_t pre = sh.wave_hist[wave][d];
+                rank[j] = pre + before;
+                __builtin_amdgcn_wave_barrier();
+                if (before == 0) sh.wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
+                __builtin_amdgcn_wave_barrier();
+            }
+            // Values only now, so they hold no registers through the ranking
+            // (LDS is rewritten only after the barriers below).
+#pragma unroll
+            for (int j = 0; j < kLdsItems; ++j) v[j] = (uint32_t)j < items ? sh.vals[wbase + 64u * j + lane] : 0u;
+            __syncthreads();
+            // Per digit: exclusive offsets of the waves, then the digits' starts.
+            uint32_t count = 0;
+            if (tid < (uint32_t)kSortRadix) {
+#pragma unroll
+                for (int w = 0; w < kLdsWaves; ++w) {
+                    const uint32_t c = sh.wave_hist[w][tid];
+                    sh.wave_hist[w][tid] = count;
+                    count += c;
+                }
+            }
+            const uint32_t dstart = BlockScan<kLdsWaves>(count, sh.wave_tot);  // contains barriers
+            if (tid < (uint32_t)kSortRadix) sh.digit_start[tid] = dstart;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kLdsItems; ++j) {
+                if ((uint32_t)j >= items) continue;
+                const uint32_t d = (k[j] >> shift) & (kSortRadix - 1);
+                const uint32_t pos = sh.digit_start[d] + sh.wave_hist[wave][d] + rank[j];
+                sh.keys[pos] = k[j];
+                sh.vals[pos] = v[j];
+            }
+            __syncthreads();
+        }
+        // The hierarchy from the LDS-resident sorted codes (EmitNode, as
+        // EmitHierarchyKernel); BatchRefitKernel refits it.
+        BvhNode* nodes = reinterpret_cast<BvhNode*>(result + sizeof(AccelHeader));
+        for (uint32_t i = tid; i < n; i += kLdsBlock) EmitNode(sh.keys, sh.vals, n, i, nodes);
+        // The next mesh's first barrier orders these LDS reads before reuse.
+    }
+}
+
 // Karras emission of every mesh, one thread per leaf index (node i and
 // leaf i of its mesh), over the 64-leaf chunks of all meshes: each node's
 // binary searches are a chain of dependent loads, so the whole GPU's
@@ -288,16 +468,17 @@ __global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __re
                         active && i + 1 < n ? i : kInvalid, n - 1 + i, parent, lo, hi, lds);
 }
 
-// Resident blocks of `kernel` on the current device, cached per device.
-uint32_t ResidentBlocks(const void* kernel, int which) {
-    static std::atomic<uint32_t> cache[1][64];
+// Resident blocks of `kernel` (workgroups of `block` threads) on the current
+// device, cached per device; `which` names the kernel's cache row.
+uint32_t ResidentBlocks(const void* kernel, int block, int which) {
+    static std::atomic<uint32_t> cache[2][64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) dev = 0;
     uint32_t b = cache[which][dev].load(std::memory_order_relaxed);
     if (!b) {
         int per_cu = 0, cus = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         b = (uint32_t)std::max(1, per_cu * std::max(1, cus));
         cache[which][dev].store(b, std::memory_order_relaxed);
@@ -311,15 +492,25 @@ size_t BatchWorkBytes(uint32_t tris) {
     return std::max(AlignUp(size_t(AlignUp(tris, 64)) * 16, 256), RestructureScratchBytes(tris));
 }
 
-void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, void* scratch, hipStream_t s) {
+void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t big_chunks, uint32_t total_chunks,
+                void* scratch, hipStream_t s) {
     if (meshes == 0) return;
+    big = std::min(big, meshes);
     uint32_t* queue = static_cast<uint32_t*>(scratch);
     BatchQueueInitKernel<<<1, 64, 0, s>>>(queue);
-    const uint32_t blocks = std::min(meshes, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildKernel), 0));
-    BatchBuildKernel<<<blocks, kBlock, 0, s>>>(table, meshes, queue);
-    if (total_chunks) {
-        const uint32_t eblocks = (uint32_t)(((uint64_t)total_chunks * kRefitChunk + kBlock - 1) / kBlock);
-        BatchEmitKernel<<<eblocks, kBlock, 0, s>>>(table, meshes, total_chunks, queue);
+    if (big) {
+        const uint32_t blocks =
+            std::min(big, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildKernel), kBlock, 0));
+        BatchBuildKernel<<<blocks, kBlock, 0, s>>>(table, big, queue);
+        if (big_chunks) {
+            const uint32_t eblocks = (uint32_t)(((uint64_t)big_chunks * kRefitChunk + kBlock - 1) / kBlock);
+            BatchEmitKernel<<<eblocks, kBlock, 0, s>>>(table, big, big_chunks, queue);
+        }
+    }
+    if (meshes > big) {
+        const uint32_t blocks = std::min(
+            meshes - big, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildLdsKernel), kLdsBlock, 1));
+        BatchBuildLdsKernel<<<blocks, kLdsBlock, 0, s>>>(table, big, meshes, queue);
     }
     BatchRefit(table, meshes, total_chunks, scratch, s);
 }

@@ -227,7 +227,7 @@ __device__ __forceinline__ uint32_t WriteLeaf(const MeshView& m, BvhNode* __rest
 // Windowed refit.  A workgroup of W threads refits leaves whose key
 // positions form windows of consecutive positions [base, base + width) and
 // keeps in LDS, for every internal node whose index lies in a window, an
-// arrival counter and the node's two child boxes.  In the Karras layout an
+// arrival counter, the node's links and its two child boxes.  In the Karras layout an
 // internal node's index is one end of its key range, so every node whose
 // subtree lies inside a window has its index in that window: those nodes are
 // merged in LDS, with no global atomic and no wait for a global store.  A
@@ -239,13 +239,17 @@ __device__ __forceinline__ uint32_t WriteLeaf(const MeshView& m, BvhNode* __rest
 // are min/max, so the result does not depend on which thread merges them.
 // The `update` counter advances by two per refit on the nodes that take the
 // global hand-off and stays unchanged on the others (even on all of them).
-// The window's links (child0, parent of each in-window internal node) are
-// read into LDS once, one node per thread, so the climb inside the window
-// makes no global load.
+// The window's links (child0, child1, parent of each in-window internal
+// node) are read into LDS once, one node per thread, so the climb inside
+// the window makes no global load, and the second arrival at an in-window
+// node writes the whole node (both boxes and its links, 60 of its 64 bytes;
+// `update` is left alone) with plain full-quad stores — the first arrival
+// stores nothing unless the node turns out to cross the window edge.
 template <int W>
 struct RefitLds {
     uint32_t count[W];
     uint32_t child0[W];
+    uint32_t child1[W];
     uint32_t parent[W];
     float box[2][6][W];  // [child side][lo xyz, hi xyz][slot]
 };
@@ -260,12 +264,14 @@ __device__ __forceinline__ void WindowRefit(BvhNode* __restrict__ nodes, uint32_
                                             uint32_t slot0, bool active, uint32_t own, uint32_t leaf, uint32_t parent,
                                             float lo[3], float hi[3], RefitLds<W>& s) {
     s.count[threadIdx.x] = 0;
-    uint32_t own_c0 = kInvalid, own_up = kInvalid;
+    uint32_t own_c0 = kInvalid, own_c1 = kInvalid, own_up = kInvalid;
     if (own < total) {
         own_c0 = nodes[own].child0;
+        own_c1 = nodes[own].child1;
         own_up = nodes[own].parent;
     }
     s.child0[threadIdx.x] = own_c0;
+    s.child1[threadIdx.x] = own_c1;
     s.parent[threadIdx.x] = own_up;
     __syncthreads();
     uint32_t child = leaf, p = parent, l = 0;
@@ -280,12 +286,9 @@ __device__ __forceinline__ void WindowRefit(BvhNode* __restrict__ nodes, uint32_
             break;
         }
         l = slot0 + (p - base);
-        BvhNode* pn = nodes + p;
         const uint32_t c0 = s.child0[l];
         const uint32_t up = s.parent[l];
         const int side = c0 == child ? 0 : 1;
-        StoreBoxAgent(side ? pn->a1_min_or_v2 : pn->a0_min_or_v0, lo[0], lo[1], lo[2]);
-        StoreBoxAgent(side ? pn->a1_max_or_v3 : pn->a0_max_or_v1, hi[0], hi[1], hi[2]);
         for (int a = 0; a < 3; ++a) {
             s.box[side][a][l] = lo[a];
             s.box[side][3 + a][l] = hi[a];
@@ -297,9 +300,26 @@ __device__ __forceinline__ void WindowRefit(BvhNode* __restrict__ nodes, uint32_
             break;
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        float olo[3], ohi[3];
         for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], s.box[side ^ 1][a][l]);
-            hi[a] = fmaxf(hi[a], s.box[side ^ 1][3 + a][l]);
+            olo[a] = s.box[side ^ 1][a][l];
+            ohi[a] = s.box[side ^ 1][3 + a][l];
+        }
+        // Both children are in: the whole node, child 0's box first.
+        const float* b0lo = side ? olo : lo;
+        const float* b0hi = side ? ohi : hi;
+        const float* b1lo = side ? lo : olo;
+        const float* b1hi = side ? hi : ohi;
+        float4* q = reinterpret_cast<float4*>(nodes + p);
+        q[0] = make_float4(b0lo[0], b0lo[1], b0lo[2], __uint_as_float(c0));
+        q[1] = make_float4(b0hi[0], b0hi[1], b0hi[2], __uint_as_float(s.child1[l]));
+        q[2] = make_float4(b1lo[0], b1lo[1], b1lo[2], __uint_as_float(up));
+        RrBox3 last;
+        last.x = b1hi[0]; last.y = b1hi[1]; last.z = b1hi[2];
+        *reinterpret_cast<RrBox3*>(&q[3]) = last;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], olo[a]);
+            hi[a] = fmaxf(hi[a], ohi[a]);
         }
         child = p;
         p = up;
