@@ -244,14 +244,23 @@ __device__ __forceinline__ uint32_t WriteLeaf(const MeshView& m, BvhNode* __rest
 // the window makes no global load, and the second arrival at an in-window
 // node writes the whole node (both boxes and its links, 60 of its 64 bytes;
 // `update` is left alone) with plain full-quad stores — the first arrival
-// stores nothing unless the node turns out to cross the window edge.
+// stores nothing unless the node turns out to cross the window edge.  No
+// other thread writes an in-window node that both children reach inside
+// the window.
+//
+// After the barrier the climbs that continue with the global hand-off (a
+// few per workgroup) are handed to the first wave through LDS, in the box
+// slots, which nobody reads any more: the other waves end at once instead
+// of staying resident behind one lane's chain of memory round trips.
+constexpr int kPendingWords = 11;  // child, p, lo[3], hi[3], node count, nodes (2 words)
 template <int W>
 struct RefitLds {
     uint32_t count[W];
     uint32_t child0[W];
     uint32_t child1[W];
     uint32_t parent[W];
-    float box[2][6][W];  // [child side][lo xyz, hi xyz][slot]
+    float box[2][6][W];  // [child side][lo xyz, hi xyz][slot]; then the pending climbs
+    uint32_t pending;
 };
 
 // Every thread of the workgroup calls this (inactive ones with
@@ -263,7 +272,9 @@ template <int W>
 __device__ __forceinline__ void WindowRefit(BvhNode* __restrict__ nodes, uint32_t total, uint32_t base, uint32_t width,
                                             uint32_t slot0, bool active, uint32_t own, uint32_t leaf, uint32_t parent,
                                             float lo[3], float hi[3], RefitLds<W>& s) {
+    static_assert(kPendingWords <= 12, "a pending climb per thread fits in the box slots");
     s.count[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s.pending = 0;
     uint32_t own_c0 = kInvalid, own_c1 = kInvalid, own_up = kInvalid;
     if (own < total) {
         own_c0 = nodes[own].child0;
@@ -328,7 +339,38 @@ __device__ __forceinline__ void WindowRefit(BvhNode* __restrict__ nodes, uint32_
     __builtin_amdgcn_s_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (state == 3 && s.count[l] == 1) state = 2;
-    if (state == 2) ClimbRefit(nodes, total, child, p, lo, hi);
+    uint32_t* list = reinterpret_cast<uint32_t*>(&s.box[0][0][0]);
+    if (state == 2) {
+        uint32_t* q = list + kPendingWords * atomicAdd(&s.pending, 1u);
+        const uint64_t base = reinterpret_cast<uint64_t>(nodes);
+        q[0] = child;
+        q[1] = p;
+        for (int a = 0; a < 3; ++a) {
+            q[2 + a] = __float_as_uint(lo[a]);
+            q[5 + a] = __float_as_uint(hi[a]);
+        }
+        q[8] = total;
+        q[9] = (uint32_t)base;
+        q[10] = (uint32_t)(base >> 32);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (threadIdx.x >= 64) return;
+    const uint32_t pending = s.pending;
+    for (uint32_t e = threadIdx.x; e < pending; e += 64) {
+        const uint32_t* q = list + kPendingWords * e;
+        float plo[3], phi[3];
+        for (int a = 0; a < 3; ++a) {
+            plo[a] = __uint_as_float(q[2 + a]);
+            phi[a] = __uint_as_float(q[5 + a]);
+        }
+        // The climb's own tree (workgroups of the batched refit hold several
+        // meshes), addressed from this thread's `nodes` so that it stays a
+        // global-memory pointer.
+        BvhNode* tree = RebaseOn<BvhNode>(nodes, (uint64_t)q[9] | ((uint64_t)q[10] << 32));
+        ClimbRefit(tree, q[8], q[0], q[1], plo, phi);
+    }
 }
 
 }  // namespace rr
