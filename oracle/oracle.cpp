@@ -227,7 +227,7 @@ void Refit(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t n, N
             box[x].lo[a] = Min(b0.lo[a], b1.lo[a]);
             box[x].hi[a] = Max(b0.hi[a], b1.hi[a]);
         }
-        nd.update += 2;  // two refit arrivals, as the GPU counter
+        nd.update = 0;  // the GPU refit's second arrival clears the arrival counter
     }
 }
 
@@ -937,7 +937,7 @@ void orc_build_tlas(const float* xforms, const uint64_t* blas_addr, const void* 
             box[x].lo[k] = Min(box[nd.c0].lo[k], box[nd.c1].lo[k]);
             box[x].hi[k] = Max(box[nd.c0].hi[k], box[nd.c1].hi[k]);
         }
-        nd.update += 2;
+        nd.update = 0;
     }
 }
 

@@ -112,7 +112,8 @@ __device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, con
 // (lbvh_fit_aabb_mesh.comp:122-205, lbvh_fit_aabb_scene.comp:98-167).  Each
 // finished node stores its box into its parent's child slot, then bumps the
 // parent's arrival counter; an odd previous value means the sibling already
-// stored its box, so this thread merges both and climbs on.
+// stored its box, so this thread clears the counter, merges both boxes and
+// climbs on.
 //
 // Cross-workgroup hand-off without an L2 write-back per step: the slot is
 // written with agent-scope (write-through) stores, the wave waits for them to
@@ -185,6 +186,10 @@ __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t
         const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((old & 1u) == 0) return;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        // Both children are in and no one else touches this counter in this
+        // refit: clear it, so that `update` is 0 between refits whichever
+        // nodes took this hand-off.
+        __hip_atomic_store(&pn->update, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const bool next = up != kInvalid && up < total;
         uint32_t nc0 = kInvalid, nup = kInvalid;
         if (next) LoadLinks(nodes + up, nc0, nup);  // in flight with the box loads below
@@ -237,8 +242,8 @@ __device__ __forceinline__ uint32_t WriteLeaf(const MeshView& m, BvhNode* __rest
 // and every out-of-window stop go on with ClimbRefit's global hand-off, which
 // every arrival at a node whose subtree crosses a window edge uses.  Boxes
 // are min/max, so the result does not depend on which thread merges them.
-// The `update` counter advances by two per refit on the nodes that take the
-// global hand-off and stays unchanged on the others (even on all of them).
+// The `update` counter is 0 on every node between refits: nodes merged in
+// LDS never touch it, and the global hand-off clears it.
 // The window's links (child0, child1, parent of each in-window internal
 // node) are read into LDS once, one node per thread, so the climb inside
 // the window makes no global load, and the second arrival at an in-window

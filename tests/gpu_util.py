@@ -18,8 +18,8 @@ def to_dev(a, dev):
 
 def nodes_equal(gpu_buf: np.ndarray, ref_buf: np.ndarray) -> tuple[bool, str]:
     """Compare two BLAS/TLAS buffers node by node.  Float fields compare with
-    == (so +0 and -0 are equal); the `update` counter of internal nodes is
-    skipped (it counts refits)."""
+    == (so +0 and -0 are equal); the integer fields (links and the `update`
+    counter, 0 between refits) must be equal."""
     import oracle
 
     g, r = oracle.nodes_of(gpu_buf), oracle.nodes_of(ref_buf)
@@ -27,7 +27,7 @@ def nodes_equal(gpu_buf: np.ndarray, ref_buf: np.ndarray) -> tuple[bool, str]:
         return False, f"node count {g.shape} vs {r.shape}"
     if g.size == 0:
         return True, ""
-    ints = [3, 7, 11]
+    ints = [3, 7, 11, 15]
     floats = [0, 1, 2, 4, 5, 6, 8, 9, 10, 12, 13, 14]
     bad_i = np.nonzero((g[:, ints] != r[:, ints]).any(axis=1))[0]
     if bad_i.size:
