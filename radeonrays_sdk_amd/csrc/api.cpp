@@ -1159,10 +1159,12 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
         const uint32_t n_small = (uint32_t)plan.small.size();
         rr::BatchMesh* host = nullptr;
         if (n_small) host = static_cast<rr::BatchMesh*>(AcquireStaging(c, cs, sizeof(rr::BatchMesh) * n_small));
-        uint32_t chunks = 0, big = 0;
+        uint32_t chunks = 0, big = 0, small = 0;
         for (uint32_t k = 0; k < n_small; ++k) {
             const rr::MeshDesc& m = plan.meshes[plan.small[k]];
-            if (m.tri_count > rr::kBatchLdsMaxTris) ++big;  // a prefix: the table is ordered largest first
+            // Prefixes: the table is ordered largest first.
+            if (m.tri_count > rr::kBatchLdsMaxTris) ++big;
+            if (m.tri_count > rr::kBatchLdsSmallMaxTris) ++small;
             rr::BatchMesh b{};
             b.vertices = reinterpret_cast<uint64_t>(m.vertices);
             b.indices = reinterpret_cast<uint64_t>(m.indices);
@@ -1188,7 +1190,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
                 if (update) {
                     rr::BatchRefit(table, n_small, chunks, scratch, s);
                 } else {
-                    rr::BatchBuild(table, n_small, big, big_chunks, chunks, scratch, s);
+                    rr::BatchBuild(table, n_small, big, small, big_chunks, chunks, scratch, s);
                     // Treelet restructuring of all meshes at once, each in its own work area.
                     if (restructure)
                         rr::BatchRestructure(table, n_small, chunks, max_roots, scratch,

@@ -44,8 +44,10 @@ constexpr uint32_t kRefitChunk = 64;
 inline uint32_t RefitChunks(uint32_t tris) { return (tris + kRefitChunk - 1) / kRefitChunk; }
 
 // Meshes of at most this many triangles are built with their whole sort in
-// LDS (one 1024-thread workgroup per mesh).
+// LDS (one 1024-thread workgroup per mesh; 256 threads for meshes of at
+// most kBatchLdsSmallMaxTris).
 constexpr uint32_t kBatchLdsMaxTris = 16384;
+constexpr uint32_t kBatchLdsSmallMaxTris = 4096;
 
 // LBVH build of every mesh in the table (already in device memory at
 // `table`, inside `scratch`).  The first `big` meshes (those above
@@ -55,10 +57,12 @@ constexpr uint32_t kBatchLdsMaxTris = 16384;
 // thread per node of these meshes (in chunks of kRefitChunk leaves, the
 // first big_chunks chunks) emits the hierarchy.  Every other mesh is sorted
 // by one workgroup in LDS, which then emits its hierarchy from the
-// LDS-resident codes.  Last, BatchRefit over all total_chunks chunks.  The
-// results equal BuildBlas with restructure off, mesh by mesh.
-void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t big_chunks, uint32_t total_chunks,
-                void* scratch, hipStream_t stream);
+// LDS-resident codes; meshes from index `small` on (at most
+// kBatchLdsSmallMaxTris triangles) by 256-thread workgroups.  Last,
+// BatchRefit over all total_chunks chunks.  The results equal BuildBlas
+// with restructure off, mesh by mesh.
+void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t small, uint32_t big_chunks,
+                uint32_t total_chunks, void* scratch, hipStream_t stream);
 
 // Refit of every mesh in the table (the topology of each BLAS is kept), as
 // UpdateBlas mesh by mesh (one launch over the refit chunks of all meshes).
@@ -72,7 +76,8 @@ void BatchRefit(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, 
 // over meshes of tris / (64 << r)); its counter is scratch word
 // kBatchRootsCounterWord.
 constexpr uint32_t kBatchRootsCounterWord = 1;
-constexpr uint32_t kBatchLdsQueueWord = 2;  // queue of the LDS-sorted meshes
+constexpr uint32_t kBatchLdsQueueWord = 2;       // queue of the LDS-sorted meshes (1024-thread workgroups)
+constexpr uint32_t kBatchLdsSmallQueueWord = 3;  // and of those for 256-thread workgroups
 void BatchRestructure(const BatchMesh* table, uint32_t meshes, uint32_t total_chunks, const uint32_t max_roots[3],
                       void* scratch, void* roots, hipStream_t stream);
 

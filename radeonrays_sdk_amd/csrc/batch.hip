@@ -141,6 +141,7 @@ __device__ void BlockSortPass(const uint32_t* __restrict__ ki, const uint32_t* _
 __global__ void BatchQueueInitKernel(uint32_t* queue) {
     if (threadIdx.x == 0) queue[0] = 0;
     if (threadIdx.x == kBatchLdsQueueWord) queue[kBatchLdsQueueWord] = 0;
+    if (threadIdx.x == kBatchLdsSmallQueueWord) queue[kBatchLdsSmallQueueWord] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
@@ -229,36 +230,43 @@ __global__ __launch_bounds__(kBlock) void BatchBuildKernel(const BatchMesh* __re
     }
 }
 
-// Meshes of at most kBatchLdsMaxTris triangles: one 1024-thread workgroup
-// per mesh computes bounds and Morton codes, and runs the four 8-bit passes
-// of the stable sort with keys and values resident in LDS (128 KB); only
-// the sorted pairs are written out.  Item j of lane l in wave w is position
+// Meshes of at most kBatchLdsMaxTris triangles: one workgroup per mesh
+// computes bounds and Morton codes, runs the four 8-bit passes of the
+// stable sort with keys and values resident in LDS (128 KB for 16,384
+// pairs), and emits the hierarchy from the LDS-resident codes.  Item j of lane l in wave w is position
 // w * (64 * kLdsItems) + 64 j + l, and ranks follow that order (waves, then
 // items, then lanes), so each pass is stable and the result equals the
 // single-mesh sort of the same (code, index) pairs.  kBatchLdsMaxTris is a
 // multiple of 64, so keys padded to 64 stay within LDS.
-constexpr int kLdsBlock = 1024;
-constexpr int kLdsWaves = kLdsBlock / 64;
-constexpr int kLdsItems = 16;
-static_assert(kLdsBlock * kLdsItems == (int)kBatchLdsMaxTris, "one item per triangle");
-
+// BLOCK threads x ITEMS keys: 1024 x 16 for meshes of up to 16,384
+// triangles (one workgroup per CU), 256 x 16 for meshes of up to 4,096
+// (four per CU, so that one mesh's memory round trips overlap another's
+// ranking).
+template <int BLOCK, int ITEMS>
 struct BatchLdsShared {
-    uint32_t keys[kBatchLdsMaxTris];
-    uint32_t vals[kBatchLdsMaxTris];
-    uint32_t wave_hist[kLdsWaves][kSortRadix];
+    static constexpr int kWavesPer = BLOCK / 64;
+    uint32_t keys[BLOCK * ITEMS];
+    uint32_t vals[BLOCK * ITEMS];
+    uint32_t wave_hist[kWavesPer][kSortRadix];
     uint32_t digit_start[kSortRadix];
-    uint32_t bounds[kLdsWaves][6];
-    uint32_t wave_tot[kLdsWaves];
+    uint32_t bounds[kWavesPer][6];
+    uint32_t wave_tot[kWavesPer];
     uint32_t mesh;
 };
 
-__global__ __launch_bounds__(kLdsBlock) void BatchBuildLdsKernel(const BatchMesh* __restrict__ table, uint32_t first,
-                                                                 uint32_t meshes, uint32_t* queue) {
-    __shared__ BatchLdsShared sh;
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void BatchBuildLdsKernel(const BatchMesh* __restrict__ table, uint32_t first,
+                                                             uint32_t meshes, uint32_t* queue, uint32_t queue_word) {
+    constexpr int kLdsBlock = BLOCK;
+    constexpr int kLdsWaves = BLOCK / 64;
+    constexpr int kLdsItems = ITEMS;
+    constexpr uint32_t kMaxTris = (uint32_t)(BLOCK * ITEMS);
+    static_assert(BLOCK >= kSortRadix, "one thread per digit");
+    __shared__ BatchLdsShared<BLOCK, ITEMS> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t wbase = wave * (64 * kLdsItems);  // position of this wave's item 0, lane 0
     for (;;) {
-        if (tid == 0) sh.mesh = first + atomicAdd(queue + kBatchLdsQueueWord, 1u);
+        if (tid == 0) sh.mesh = first + atomicAdd(queue + queue_word, 1u);
         __syncthreads();
         const uint32_t mi = sh.mesh;
         __syncthreads();
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(kLdsBlock) void BatchBuildLdsKernel(const BatchMesh
         }
         // The host routes larger meshes elsewhere; the clamp keeps a bad
         // table within LDS.
-        const uint32_t n = min(bm.tri_count, kBatchLdsMaxTris);
+        const uint32_t n = min(bm.tri_count, kMaxTris);
         if (n == 0) continue;
 
         // Bounds of all vertices (ordered-uint min/max: exact).
@@ -471,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __re
 // Resident blocks of `kernel` (workgroups of `block` threads) on the current
 // device, cached per device; `which` names the kernel's cache row.
 uint32_t ResidentBlocks(const void* kernel, int block, int which) {
-    static std::atomic<uint32_t> cache[2][64];
+    static std::atomic<uint32_t> cache[3][64];
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) dev = 0;
@@ -492,8 +500,8 @@ size_t BatchWorkBytes(uint32_t tris) {
     return std::max(AlignUp(size_t(AlignUp(tris, 64)) * 16, 256), RestructureScratchBytes(tris));
 }
 
-void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t big_chunks, uint32_t total_chunks,
-                void* scratch, hipStream_t s) {
+void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t small, uint32_t big_chunks,
+                uint32_t total_chunks, void* scratch, hipStream_t s) {
     if (meshes == 0) return;
     big = std::min(big, meshes);
     uint32_t* queue = static_cast<uint32_t*>(scratch);
@@ -507,10 +515,18 @@ void BatchBuild(const BatchMesh* table, uint32_t meshes, uint32_t big, uint32_t 
             BatchEmitKernel<<<eblocks, kBlock, 0, s>>>(table, big, big_chunks, queue);
         }
     }
-    if (meshes > big) {
-        const uint32_t blocks = std::min(
-            meshes - big, ResidentBlocks(reinterpret_cast<const void*>(&BatchBuildLdsKernel), kLdsBlock, 1));
-        BatchBuildLdsKernel<<<blocks, kLdsBlock, 0, s>>>(table, big, meshes, queue);
+    small = std::min(std::max(small, big), meshes);
+    if (small > big) {
+        const auto kernel = &BatchBuildLdsKernel<1024, 16>;
+        const uint32_t blocks =
+            std::min(small - big, ResidentBlocks(reinterpret_cast<const void*>(kernel), 1024, 1));
+        kernel<<<blocks, 1024, 0, s>>>(table, big, small, queue, kBatchLdsQueueWord);
+    }
+    if (meshes > small) {
+        const auto kernel = &BatchBuildLdsKernel<256, 16>;
+        const uint32_t blocks =
+            std::min(meshes - small, ResidentBlocks(reinterpret_cast<const void*>(kernel), 256, 2));
+        kernel<<<blocks, 256, 0, s>>>(table, small, meshes, queue, kBatchLdsSmallQueueWord);
     }
     BatchRefit(table, meshes, total_chunks, scratch, s);
 }

@@ -111,9 +111,9 @@ __device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, con
 // Bottom-up refit shared by BLAS build/update and TLAS build
 // (lbvh_fit_aabb_mesh.comp:122-205, lbvh_fit_aabb_scene.comp:98-167).  Each
 // finished node stores its box into its parent's child slot, then bumps the
-// parent's arrival counter; an odd previous value means the sibling already
-// stored its box, so this thread clears the counter, merges both boxes and
-// climbs on.
+// parent's arrival counter (wrapping from 1 back to 0); a previous value of
+// 1 means the sibling already stored its box, so this thread merges both
+// boxes and climbs on.
 //
 // Cross-workgroup hand-off without an L2 write-back per step: the slot is
 // written with agent-scope (write-through) stores, the wave waits for them to
@@ -183,13 +183,12 @@ __device__ __forceinline__ void ClimbRefit(BvhNode* __restrict__ nodes, uint32_t
         StoreBoxAgent(slot_lo, lo[0], lo[1], lo[2]);
         StoreBoxAgent(slot_hi, hi[0], hi[1], hi[2]);
         WaitStores();
-        const uint32_t old = __hip_atomic_fetch_add(&pn->update, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((old & 1u) == 0) return;
+        // Wrapping increment (old >= 1 ? 0 : old + 1): the first arrival
+        // reads 0 and leaves 1, the second reads 1 and leaves 0, so `update`
+        // is 0 between refits whichever nodes took this hand-off.
+        const uint32_t old = __builtin_amdgcn_atomic_inc32(&pn->update, 1u, __ATOMIC_RELAXED, "agent");
+        if (old == 0) return;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        // Both children are in and no one else touches this counter in this
-        // refit: clear it, so that `update` is 0 between refits whichever
-        // nodes took this hand-off.
-        __hip_atomic_store(&pn->update, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const bool next = up != kInvalid && up < total;
         uint32_t nc0 = kInvalid, nup = kInvalid;
         if (next) LoadLinks(nodes + up, nc0, nup);  // in flight with the box loads below

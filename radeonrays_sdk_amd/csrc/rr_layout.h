@@ -18,9 +18,9 @@ constexpr uint32_t kTopLevelSentinel = 0xFFFFFFFEu; // isect_2l.comp:28
 //  internal node: aabb0 = box of child0, aabb1 = box of child1;
 //  leaf:          child0 = kInvalid, child1 = primitive id, v0/v1/v2 = the
 //                 triangle's vertices (vlk/kernels/bvh2.h:22-23).
-//  `update` of an internal node counts refit arrivals; an odd value tells
-//  the second child that the first one is done, and the second clears it
-//  (it is 0 between refits).
+//  `update` of an internal node counts refit arrivals modulo 2: 1 tells the
+//  second child that the first one is done, and the second's increment
+//  wraps it back to 0 (it is 0 between refits).
 // Internal nodes occupy [0, N-1), leaves [N-1, 2N-1); the root is node 0
 // (lbvh_emit_hierarchy_mesh.comp:31-32).  With N == 1 the root is a leaf.
 struct BvhNode {

@@ -16,7 +16,7 @@ from tests.gpu_util import hits_equal, nodes_equal, say, to_dev
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [0, 1, 2, 3, 7, 63, 64, 65, 127, 128, 255, 256, 1000, 1024, 1025, 4096, 5000, 16383, 16384, 16385, 20000, 65536, 70000]
+SIZES = [0, 1, 2, 3, 7, 63, 64, 65, 127, 128, 255, 256, 1000, 1024, 1025, 4095, 4096, 4097, 5000, 16383, 16384, 16385, 20000, 65536, 70000]
 
 
 def _meshes(sizes, seed):
