@@ -252,6 +252,40 @@ def instanced_bench(ctx, dev, log, reps=5):
             "tlas_build_ms_wall": round(tlas_ms, 3)}
 
 
+def soup_bench(ctx, dev, log, reps=10):
+    """The small workload: a 65,536-triangle soup (centroids uniform in the
+    unit cube, vertices at N(0, 0.01^2) around them), one restructured BLAS
+    build, 1024x1024 primary rays from (0.5, 0.5, -1.5), closest hit with
+    full-hit output and t.  Returns the GPU numbers and, for the CPU leg, the
+    host copies of the BVH and rays."""
+    import torch
+
+    from radeonrays_sdk_amd import scenes
+
+    log("soup: 65,536 triangles, 1024x1024 camera rays")
+    w = scenes.soup_camera()
+    v = torch.from_numpy(w.vertices).to(dev)
+    idx = torch.from_numpy(w.indices).to(dev)
+    rays = torch.from_numpy(w.rays).to(dev)
+    req = ctx.geometry_requirements(idx.shape[0], v.shape[0], True)
+    scratch = ctx.empty(req.temporary_build_buffer_size)
+    result = ctx.empty(req.result_buffer_size)
+    accel = ctx.build_geometry(v, idx, restructure=True, scratch=scratch, result=result)
+    build_ms = timed(lambda: ctx.build_geometry(v, idx, True, scratch=scratch, result=result), reps)
+    n = w.rays.shape[0]
+    hits = ctx.empty(n * 32)
+    tscratch = ctx.trace_scratch(n)
+    trace_ms = timed(lambda: ctx.intersect(accel, rays, "closest", "full_t", hits=hits, scratch=tscratch), reps)
+    torch.cuda.synchronize()
+    from radeonrays_sdk_amd import api
+
+    h = api.decode_hits(hits, "full_t")
+    out = {"workload": f"triangle soup ({idx.shape[0]} tris), {n} camera rays, closest, FULL_HIT_T",
+           "build_ms": round(build_ms, 4), "closest_mrays_s": round(n / trace_ms / 1e3, 2),
+           "closest_ms": round(trace_ms, 4), "hit_fraction": round(float((h["prim_id"] != 0xFFFFFFFF).mean()), 4)}
+    return out, accel, rays, w.rays
+
+
 def batched_bench(ctx, dev, log, frames=16):
     """Batched build / refit: 8,192 soups (log-uniform 64-16,384 triangles)
     built by one rrxCmdBuildGeometryBatch (against one rrCmdBuildGeometry per
@@ -367,6 +401,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream comparison")
     ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene traversal")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched build / graph refit workload")
+    ap.add_argument("--no-soup", action="store_true", help="skip the 65,536-triangle soup camera workload")
     ap.add_argument("--no-autotune", action="store_true", help="skip the traversal autotuner sweep")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -509,6 +544,9 @@ def main():
         out["sort16m_gbs"] = round(sort_gbs, 1)
         out["sort16m_hbm_frac"] = round(sort_gbs / HBM_PEAK_GBS, 4)
 
+    soup = None
+    if not args.no_soup:
+        out["soup"], *soup = soup_bench(ctx, dev, log)
     if not args.no_instanced:
         out["instanced"] = instanced_bench(ctx, dev, log)
     if not args.no_batched:
@@ -532,6 +570,22 @@ def main():
         ref = cpu_baseline_reference(ctx, accel, rays, threads)
         if ref is not None:
             out["cpu_baseline_reference"] = ref
+        if soup:
+            # The small workload's CPU traversal (the oracle's port, closest
+            # hit) and the reference's bvh_analyzer on the same BVH and rays.
+            import oracle
+
+            s_accel, s_rays_dev, s_rays = soup
+            s_buf = s_accel.buffer.cpu().numpy()
+            oracle.trace(s_buf, s_rays[: s_rays.shape[0] // 8], threads=threads)
+            t0 = time.perf_counter()
+            oracle.trace(s_buf, s_rays, threads=threads)
+            port = s_rays.shape[0] / (time.perf_counter() - t0) / 1e6
+            out["soup"]["cpu_port_mrays_s"] = round(port, 2)
+            sref = cpu_baseline_reference(ctx, s_accel, s_rays_dev, threads)
+            if sref is not None:
+                out["soup"]["cpu_reference_mrays_s"] = sref["value"]
+            out["soup"]["cpu_cores"] = threads
     ctx.close()
     print(json.dumps(out), flush=True)
 
