@@ -282,28 +282,92 @@ __global__ __launch_bounds__(kBlock) void InitTreeletCountsKernel(uint32_t n, ui
 
 // Climb from each leaf, accumulating primitive counts; the first node whose
 // subtree reaches `min_prims` becomes a treelet root (find_treelet_roots.comp:63-98).
-// Root found by the climb from leaf i, or kInvalid.
-__device__ __forceinline__ uint32_t ClimbToTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t i,
-                                                       uint32_t min_prims, uint32_t* counts) {
-    uint32_t count = 1;
-    uint32_t idx = nodes[n - 1 + i].parent;
+// This is the climb from node p on, `count` primitives arriving: an
+// exchange on each node's count, the first arrival stops.  Root found, or
+// kInvalid.  Afterwards counts[p] is non-zero exactly on the nodes some
+// climb reached, which the restructuring's hand-off relies on.
+__device__ __forceinline__ uint32_t ClimbCountsFrom(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t p,
+                                                    uint32_t count, uint32_t min_prims, uint32_t* counts) {
     const uint32_t total = 2 * n - 1;
-    for (uint32_t level = 0; idx < total && level < total; ++level) {
-        const uint32_t old = __hip_atomic_exchange(&counts[idx], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t level = 0; p < total && level < total; ++level) {
+        const uint32_t old = __hip_atomic_exchange(&counts[p], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         count += old;
         if (old == 0) return kInvalid;
-        if (count >= min_prims) return idx;
-        idx = nodes[idx].parent;
+        if (count >= min_prims) return p;
+        p = nodes[p].parent;
     }
     return kInvalid;
+}
+
+__device__ __forceinline__ uint32_t ClimbToTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t i,
+                                                       uint32_t min_prims, uint32_t* counts) {
+    return ClimbCountsFrom(nodes, n, nodes[n - 1 + i].parent, 1, min_prims, counts);
+}
+
+// The same climbs over a window of leaves, as the windowed refit
+// (WindowRefit): a node whose subtree lies inside the workgroup's window
+// counts its two arrivals in LDS — one atomic add of 4 * count + 1, so the
+// second arrival reads the first one's count with it — and the second
+// arrival marks the node in memory with a plain store of its count (no one
+// else touches it).  A climb that leaves the window, or that arrived first
+// at a node whose other child crosses the window edge (one arrival when
+// the workgroup is done), continues with the exchange climb.  The roots
+// found and the nodes marked are those of the climb from every leaf with
+// exchanges.  Every thread of the workgroup calls this (it holds barriers).
+template <int W>
+struct RootsLds {
+    uint32_t acc[W];
+    uint32_t parent[W];
+};
+template <int W>
+__device__ __forceinline__ uint32_t WindowTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t base,
+                                                      uint32_t width, uint32_t slot0, bool active, uint32_t own,
+                                                      uint32_t leaf, uint32_t min_prims, uint32_t* counts,
+                                                      RootsLds<W>& s) {
+    s.acc[threadIdx.x] = 0;
+    s.parent[threadIdx.x] = (own != kInvalid && own + 1 < n) ? nodes[own].parent : kInvalid;
+    __syncthreads();
+    uint32_t p = active ? nodes[leaf].parent : kInvalid;
+    uint32_t count = 1, l = 0, root = kInvalid;
+    int state = active ? 1 : 0;  // 0 done, 1 climbing in LDS, 2 exchange climb from p, 3 first arrival at p
+    for (uint32_t level = 0; state == 1; ++level) {
+        if (p - base >= width || p >= n - 1) {  // outside the window, or not an internal node
+            state = 2;
+            break;
+        }
+        if (level >= width) {  // more levels than a window's subtree has: corrupt links
+            state = 0;
+            break;
+        }
+        l = slot0 + (p - base);
+        const uint32_t old = atomicAdd(&s.acc[l], 4u * count + 1u);
+        if (old == 0) {
+            state = 3;
+            break;
+        }
+        count += old >> 2;
+        counts[p] = count;  // both children arrived here; no exchange will touch this node
+        if (count >= min_prims) {
+            root = p;
+            state = 0;
+            break;
+        }
+        p = s.parent[l];
+    }
+    __syncthreads();
+    if (state == 3 && (s.acc[l] & 3u) == 1u) state = 2;
+    if (state == 2) root = ClimbCountsFrom(nodes, n, p, count, min_prims, counts);
+    return root;
 }
 
 __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* __restrict__ nodes, uint32_t n,
                                                                  uint32_t min_prims, uint32_t* counts, uint32_t* counter,
                                                                  uint32_t* roots) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t root = ClimbToTreeletRoot(nodes, n, i, min_prims, counts);
+    __shared__ RootsLds<kBlock> lds;
+    const uint32_t base = blockIdx.x * kBlock;
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t root = WindowTreeletRoot<kBlock>(nodes, n, base, kBlock, 0, i < n, i + 1 < n ? i : kInvalid,
+                                                    n - 1 + i, min_prims, counts, lds);
     if (root != kInvalid) roots[atomicAdd(counter, 1u)] = root;
 }
 
@@ -537,17 +601,37 @@ __global__ __launch_bounds__(kBlock) void BatchTreeletInitKernel(const BatchMesh
     BatchCounts(bm, roots_counter)[leaf] = 0;
 }
 
+// Windows as in BatchRefitKernel: the chunks of one mesh within a block.
 __global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const BatchMesh* __restrict__ table,
                                                                       uint32_t meshes, uint32_t total_chunks,
                                                                       uint32_t min_prims, uint32_t* roots_counter,
                                                                       uint2* roots) {
-    uint32_t mesh, leaf;
-    if (!BatchChunkLeaf(table, meshes, total_chunks, mesh, leaf)) return;
-    const BatchMesh bm = table[mesh];
-    if (bm.tri_count < 64) return;
-    const BvhNode* nodes = reinterpret_cast<const BvhNode*>(RebaseOn<uint8_t>(roots_counter, bm.result) + sizeof(AccelHeader));
-    const uint32_t root = ClimbToTreeletRoot(nodes, bm.tri_count, leaf, min_prims, BatchCounts(bm, roots_counter));
-    if (root != kInvalid) roots[atomicAdd(roots_counter, 1u)] = make_uint2(mesh, root);
+    __shared__ RootsLds<kBlock> lds;
+    constexpr uint32_t kChunksPerBlock = kBlock / kRefitChunk;
+    const uint32_t cb = blockIdx.x * kChunksPerBlock;
+    const uint32_t c = cb + threadIdx.x / kRefitChunk;
+    bool active = c < total_chunks;
+    uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
+    while (active && hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (table[mid].first_chunk <= c) lo = mid;
+        else hi = mid;
+    }
+    BatchMesh bm{};
+    if (active) bm = table[lo];
+    active = active && c >= bm.first_chunk && bm.tri_count >= 64;
+    const uint32_t i = (c - bm.first_chunk) * kRefitChunk + threadIdx.x % kRefitChunk;
+    active = active && i < bm.tri_count;
+    const uint32_t first = std::max(cb, bm.first_chunk);
+    const uint32_t last = std::min(cb + kChunksPerBlock, bm.first_chunk + (bm.tri_count + kRefitChunk - 1) / kRefitChunk);
+    const uint32_t n = bm.tri_count;
+    const BvhNode* nodes =
+        reinterpret_cast<const BvhNode*>(RebaseOn<uint8_t>(roots_counter, bm.result) + sizeof(AccelHeader));
+    uint32_t* counts = active ? BatchCounts(bm, roots_counter) : nullptr;
+    const uint32_t root = WindowTreeletRoot<kBlock>(
+        nodes, n, (first - bm.first_chunk) * kRefitChunk, active ? (last - first) * kRefitChunk : 0,
+        (first - cb) * kRefitChunk, active, active && i + 1 < n ? i : kInvalid, n - 1 + i, min_prims, counts, lds);
+    if (root != kInvalid) roots[atomicAdd(roots_counter, 1u)] = make_uint2(lo, root);
 }
 
 template <int LEAVES>
