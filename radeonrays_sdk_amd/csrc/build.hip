@@ -634,8 +634,13 @@ __global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const Batc
     if (root != kInvalid) roots[atomicAdd(roots_counter, 1u)] = make_uint2(lo, root);
 }
 
+// Eight waves per SIMD (64 VGPRs, a few spilled by lane 0's rewiring): the
+// batched rounds run hundreds of thousands of short treelet chains at once
+// and are bound by how many are in flight (18.7 against 23.5 ms for the
+// 8,192-mesh batch); the single-mesh kernel, bound by one long chain, keeps
+// its registers.
 template <int LEAVES>
-__global__ __launch_bounds__(64) void BatchRestructureTreeletsKernel(const BatchMesh* __restrict__ table,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void BatchRestructureTreeletsKernel(const BatchMesh* __restrict__ table,
                                                                      uint32_t* roots_counter,
                                                                      const uint2* __restrict__ roots) {
     if (blockIdx.x >= *roots_counter) return;
