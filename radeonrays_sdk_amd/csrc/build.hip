@@ -196,11 +196,35 @@ __global__ __launch_bounds__(kBlock) void MeshMortonKernel(MeshView m, const Bui
     MortonAndHistogram(m.tri_count, head, centroid, keys, vals, passes, sort_hist, sort_status, status_words);
 }
 
+// Block b emits nodes [256 b, 256 b + 256).  Its codes and kEmitMargin
+// codes on either side are staged in LDS: most nodes have short key
+// ranges, so most probes of their searches read LDS; the others read
+// memory.  Same codes either way, so the same tree.
+constexpr int kEmitMargin = 256;
+constexpr int kEmitWindow = kBlock + 2 * kEmitMargin;
 __global__ __launch_bounds__(kBlock) void EmitHierarchyKernel(const uint32_t* __restrict__ codes,
                                                               const uint32_t* __restrict__ prims, uint32_t n,
                                                               BvhNode* __restrict__ nodes) {
+    __shared__ uint32_t s_codes[kEmitWindow];
+    const int lo = (int)(blockIdx.x * kBlock) - kEmitMargin;
+    for (int k = threadIdx.x; k < kEmitWindow; k += kBlock) {
+        const int g = lo + k;
+        s_codes[k] = (g >= 0 && g < (int)n) ? codes[g] : 0u;
+    }
+    __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) EmitNode(codes, prims, n, i, nodes);
+    if (i >= n) return;
+    // An LDS read always (clamped index) and a memory read only outside the
+    // window, kept as two loads: one select between the two addresses would
+    // become a flat load.
+    auto code = [&](int k) -> uint32_t {
+        const uint32_t o = (uint32_t)(k - lo);
+        const bool in = o < (uint32_t)kEmitWindow;
+        uint32_t v = s_codes[in ? o : 0u];
+        if (!in) v = LdG(codes + k);
+        return v;
+    };
+    EmitNodeWith(code, prims, n, i, nodes);
 }
 
 // One thread per leaf; block b's leaves (key positions [256 b, 256 b + 256))
@@ -377,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* 
 // The body runs for one root of one BVH (nodes, n leaves, arrival counts
 // of its internal nodes); the single-mesh and batched kernels call it.
 template <int LEAVES>
-__device__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, uint32_t root, uint32_t* counts) {
+__device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, uint32_t root, uint32_t* counts) {
     constexpr int kTreeletLeaves = LEAVES;
     constexpr uint32_t kTreeletMasks = 1u << LEAVES;
     __shared__ uint32_t s_leaf[kTreeletLeaves];
@@ -634,11 +658,12 @@ __global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const Batc
     if (root != kInvalid) roots[atomicAdd(roots_counter, 1u)] = make_uint2(lo, root);
 }
 
-// Eight waves per SIMD (64 VGPRs, a few spilled by lane 0's rewiring): the
-// batched rounds run hundreds of thousands of short treelet chains at once
-// and are bound by how many are in flight (18.7 against 23.5 ms for the
-// 8,192-mesh batch); the single-mesh kernel, bound by one long chain, keeps
-// its registers.
+// Eight waves per SIMD (64 VGPRs): the batched rounds run hundreds of
+// thousands of short treelet chains at once and are bound by how many are
+// in flight (18.8 against 23.5 ms for the 8,192-mesh batch).  The body is
+// inlined into each kernel, so this bound applies to this kernel alone (a
+// shared out-of-line body takes the largest register budget of its callers,
+// and the call itself cost the single-mesh kernel 36 VGPRs and a spill).
 template <int LEAVES>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void BatchRestructureTreeletsKernel(const BatchMesh* __restrict__ table,
                                                                      uint32_t* roots_counter,

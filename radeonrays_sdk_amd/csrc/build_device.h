@@ -59,16 +59,19 @@ __device__ __forceinline__ void TriangleCentroid(const MeshView& m, uint32_t t, 
 // Karras 2012 binary radix tree over sorted codes
 // (lbvh_emit_hierarchy_mesh.comp:85-217).  delta(i, j) is the common prefix
 // length of codes i and j, with 32 + clz(i ^ j) for equal codes and -1 for
-// j outside [0, n).
-__device__ __forceinline__ int Delta(const uint32_t* __restrict__ codes, int n, int i, int j) {
+// j outside [0, n).  `code(k)` returns sorted code k (from memory, or from a
+// copy in LDS); ci = code(i).
+template <typename Code>
+__device__ __forceinline__ int DeltaOf(const Code& code, int n, int i, uint32_t ci, int j) {
     if (j < 0 || j >= n) return -1;
-    const uint32_t a = codes[i], b = codes[j];
-    return a != b ? __clz(a ^ b) : 32 + __clz((uint32_t)(i ^ j));
+    const uint32_t b = code(j);
+    return ci != b ? __clz(ci ^ b) : 32 + __clz((uint32_t)(i ^ j));
 }
 
 // Node i's topology (and leaf i's links) from the sorted codes: Karras 2012.
-__device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ prims,
-                                         uint32_t n, uint32_t i, BvhNode* __restrict__ nodes) {
+template <typename Code>
+__device__ __forceinline__ void EmitNodeWith(const Code& code, const uint32_t* __restrict__ prims, uint32_t n,
+                                             uint32_t i, BvhNode* __restrict__ nodes) {
     const uint32_t leaf = n - 1 + i;
     nodes[leaf].child0 = kInvalid;
     nodes[leaf].child1 = prims[i];
@@ -79,22 +82,24 @@ __device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, con
     }
     if (i >= n - 1) return;
     const int N = (int)n, I = (int)i;
+    const uint32_t ci = code(I);
     // Direction and far end of node i's key range.
-    const int d = (Delta(codes, N, I, I + 1) - Delta(codes, N, I, I - 1)) > 0 ? 1 : -1;
-    const int dmin = Delta(codes, N, I, I - d);
+    const int d = (DeltaOf(code, N, I, ci, I + 1) - DeltaOf(code, N, I, ci, I - 1)) > 0 ? 1 : -1;
+    const int dmin = DeltaOf(code, N, I, ci, I - d);
     int lmax = 2;
-    while (Delta(codes, N, I, I + lmax * d) > dmin) lmax *= 2;
+    while (DeltaOf(code, N, I, ci, I + lmax * d) > dmin) lmax *= 2;
     int l = 0;
     for (int t = lmax / 2; t >= 1; t /= 2)
-        if (Delta(codes, N, I, I + (l + t) * d) > dmin) l += t;
+        if (DeltaOf(code, N, I, ci, I + (l + t) * d) > dmin) l += t;
     const int j = I + l * d;
     const int first = min(I, j), last = max(I, j);
     // Split: last position that shares more than the range's common prefix with `first`.
-    const int common = Delta(codes, N, first, last);
+    const uint32_t cf = code(first);
+    const int common = DeltaOf(code, N, first, cf, last);
     int lo = first, hi = last;
     while (hi > lo + 1) {
         const int mid = (lo + hi) / 2;
-        if (Delta(codes, N, first, mid) > common) lo = mid;
+        if (DeltaOf(code, N, first, cf, mid) > common) lo = mid;
         else hi = mid;
     }
     const int split = lo;
@@ -106,6 +111,11 @@ __device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, con
     nodes[left].parent = i;
     nodes[right].parent = i;
     if (i == 0) nodes[0].parent = kInvalid;
+}
+
+__device__ __forceinline__ void EmitNode(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ prims,
+                                         uint32_t n, uint32_t i, BvhNode* __restrict__ nodes) {
+    EmitNodeWith([codes](int k) { return codes[k]; }, prims, n, i, nodes);
 }
 
 // Bottom-up refit shared by BLAS build/update and TLAS build
