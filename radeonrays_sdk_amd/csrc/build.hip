@@ -409,6 +409,11 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
     __shared__ float s_box[kTreeletLeaves][6];
     __shared__ float s_leaf_area[kTreeletLeaves];
     __shared__ float s_area[kTreeletMasks];
+    // Box of each leaf subset (lo xyz, hi xyz), kept from step 2 for the
+    // rewiring; not for 9-leaf treelets, whose 512 boxes would take 12 KB
+    // of LDS per wave.
+    constexpr bool kSubsetBoxes = LEAVES <= 7;
+    __shared__ float s_sbox[kSubsetBoxes ? kTreeletMasks : 1][6];
     __shared__ float s_cost[kTreeletMasks];
     __shared__ uint32_t s_part[kTreeletMasks];
     __shared__ uint32_t s_root;
@@ -474,6 +479,11 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
                             hi[a] = fmaxf(hi[a], s_box[k][3 + a]);
                         }
                 s_area[m] = m ? SurfaceArea(lo, hi) : 0.0f;
+                if constexpr (kSubsetBoxes)
+                    for (int a = 0; a < 3; ++a) {
+                        s_sbox[m][a] = lo[a];
+                        s_sbox[m][3 + a] = hi[a];
+                    }
                 s_cost[m] = __popc(m) == 1 ? kCostInternal * s_area[m] : 0.0f;
                 s_part[m] = 0;
             }
@@ -527,15 +537,21 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
                             child[c] = s_leaf[__ffs(side[c]) - 1];
                         }
                     }
+                    // Each side's box: its leaf subset's, from step 2, or
+                    // the same union in the same leaf order.
                     float box[2][6];
                     for (int c = 0; c < 2; ++c) {
-                        for (int a = 0; a < 3; ++a) { box[c][a] = FLT_MAX; box[c][3 + a] = -FLT_MAX; }
-                        for (int k = 0; k < kTreeletLeaves; ++k)
-                            if (side[c] & (1u << k))
-                                for (int a = 0; a < 3; ++a) {
-                                    box[c][a] = fminf(box[c][a], s_box[k][a]);
-                                    box[c][3 + a] = fmaxf(box[c][3 + a], s_box[k][3 + a]);
-                                }
+                        if constexpr (kSubsetBoxes) {
+                            for (int a = 0; a < 6; ++a) box[c][a] = s_sbox[side[c]][a];
+                        } else {
+                            for (int a = 0; a < 3; ++a) { box[c][a] = FLT_MAX; box[c][3 + a] = -FLT_MAX; }
+                            for (int k = 0; k < kTreeletLeaves; ++k)
+                                if (side[c] & (1u << k))
+                                    for (int a = 0; a < 3; ++a) {
+                                        box[c][a] = fminf(box[c][a], s_box[k][a]);
+                                        box[c][3 + a] = fmaxf(box[c][3 + a], s_box[k][3 + a]);
+                                    }
+                        }
                     }
                     BvhNode& nd = nodes[node];
                     StoreBoxAgent(nd.a0_min_or_v0, box[0][0], box[0][1], box[0][2]);
