@@ -493,7 +493,41 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
             //    without its lowest leaf, in increasing order; the first
             //    cheapest wins.  Orientation as restructure_bvh.comp:395-397
             //    (sizes 2-5 flipped for even sizes, 6 and 7 unflipped).
+            //    The full set, last, has the most candidates (63 for 7
+            //    leaves): they are (x + 1) << 1 in increasing order, split
+            //    over the lanes and reduced to the cheapest with the lowest
+            //    x, the sequential loop's choice.
             for (int bits = 2; bits <= kTreeletLeaves; ++bits) {
+                if (bits == kTreeletLeaves) {
+                    constexpr uint32_t s = kTreeletMasks - 1;
+                    constexpr uint32_t candidates = kTreeletMasks / 2 - 1;
+                    float best_cost = FLT_MAX;
+                    uint32_t best_x = 0xFFFFFFFFu;
+                    for (uint32_t x = lane; x < candidates; x += 64) {
+                        const uint32_t p = (x + 1) << 1;
+                        const float c = s_cost[p] + s_cost[s ^ p];
+                        if (best_x == 0xFFFFFFFFu || c < best_cost) {
+                            best_cost = c;
+                            best_x = x;
+                        }
+                    }
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        const float oc = __shfl_xor(best_cost, off, 64);
+                        const uint32_t ox = __shfl_xor(best_x, off, 64);
+                        if (ox != 0xFFFFFFFFu &&
+                            (best_x == 0xFFFFFFFFu || oc < best_cost || (oc == best_cost && ox < best_x))) {
+                            best_cost = oc;
+                            best_x = ox;
+                        }
+                    }
+                    if (lane == 0) {
+                        const uint32_t best = (best_x + 1) << 1;
+                        s_cost[s] = kCostInternal * s_area[s] + best_cost;
+                        s_part[s] = (bits <= 5 && (bits & 1) == 0) ? (s ^ best) : best;
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 for (uint32_t s = lane; s < kTreeletMasks; s += 64) {
                     if (__popc(s) != bits) continue;
                     const uint32_t delta = (s - 1) & s;
