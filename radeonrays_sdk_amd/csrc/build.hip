@@ -497,7 +497,52 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
             //    leaves): they are (x + 1) << 1 in increasing order, split
             //    over the lanes and reduced to the cheapest with the lowest
             //    x, the sequential loop's choice.
+            //    For 7 leaves the six-leaf subsets (127 without one leaf g)
+            //    get eight lanes each: candidate x + 1 is x + 1's bits
+            //    deposited into the subset's bits above its lowest leaf,
+            //    and the group reduces to its first cheapest.
             for (int bits = 2; bits <= kTreeletLeaves; ++bits) {
+                if (kTreeletLeaves == 7 && bits == 6) {
+                    const uint32_t g = lane >> 3, j = lane & 7u;
+                    const uint32_t s = (kTreeletMasks - 1) ^ (1u << (g < 7 ? g : 0u));
+                    const uint32_t delta = (s - 1) & s;
+                    float best_cost = FLT_MAX;
+                    uint32_t best_x = 0xFFFFFFFFu, best_p = 0;
+                    if (g < 7) {
+                        for (uint32_t x = j; x < 31; x += 8) {
+                            uint32_t p = 0, m = delta, v = x + 1;
+                            for (int k = 0; k < 6; ++k) {
+                                const uint32_t low = m & (0u - m);
+                                if (v & 1u) p |= low;
+                                v >>= 1;
+                                m ^= low;
+                            }
+                            const float c = s_cost[p] + s_cost[s ^ p];
+                            if (best_x == 0xFFFFFFFFu || c < best_cost) {
+                                best_cost = c;
+                                best_x = x;
+                                best_p = p;
+                            }
+                        }
+                    }
+                    for (int off = 4; off >= 1; off >>= 1) {  // within the group of eight
+                        const float oc = __shfl_xor(best_cost, off, 64);
+                        const uint32_t ox = __shfl_xor(best_x, off, 64);
+                        const uint32_t op = __shfl_xor(best_p, off, 64);
+                        if (ox != 0xFFFFFFFFu &&
+                            (best_x == 0xFFFFFFFFu || oc < best_cost || (oc == best_cost && ox < best_x))) {
+                            best_cost = oc;
+                            best_x = ox;
+                            best_p = op;
+                        }
+                    }
+                    if (g < 7 && j == 0) {
+                        s_cost[s] = kCostInternal * s_area[s] + best_cost;
+                        s_part[s] = best_p;  // six leaves: not flipped
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 if (bits == kTreeletLeaves) {
                     constexpr uint32_t s = kTreeletMasks - 1;
                     constexpr uint32_t candidates = kTreeletMasks / 2 - 1;
