@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* 
 // programme over subsets run across the wave.
 // The body runs for one root of one BVH (nodes, n leaves, arrival counts
 // of its internal nodes); the single-mesh and batched kernels call it.
-template <int LEAVES>
+template <int LEAVES, bool kFiveLeafGroups>
 __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, uint32_t root, uint32_t* counts) {
     constexpr int kTreeletLeaves = LEAVES;
     constexpr uint32_t kTreeletMasks = 1u << LEAVES;
@@ -501,7 +501,64 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
             //    get eight lanes each: candidate x + 1 is x + 1's bits
             //    deposited into the subset's bits above its lowest leaf,
             //    and the group reduces to its first cheapest.
+            //    And, when kFiveLeafGroups (the single-mesh kernel: the
+            //    batched one, held to 64 VGPRs, would spill), the five-leaf
+            //    subsets (127 without leaves a < b, 21 of them in (a, b)
+            //    order) get three lanes each.
             for (int bits = 2; bits <= kTreeletLeaves; ++bits) {
+                if (kFiveLeafGroups && kTreeletLeaves == 7 && bits == 5) {
+                    const uint32_t g = lane / 3, j = lane - 3 * g;
+                    uint32_t a = 0, b = 1;
+                    for (uint32_t r = g < 21 ? g : 0, first = 0; first < 6; ++first) {
+                        if (r < 6 - first) {
+                            a = first;
+                            b = first + 1 + r;
+                            break;
+                        }
+                        r -= 6 - first;
+                    }
+                    const uint32_t s = (kTreeletMasks - 1) ^ (1u << a) ^ (1u << b);
+                    const uint32_t delta = (s - 1) & s;
+                    float best_cost = FLT_MAX;
+                    uint32_t best_x = 0xFFFFFFFFu, best_p = 0;
+                    if (g < 21) {
+                        for (uint32_t x = j; x < 15; x += 3) {
+                            uint32_t p = 0, m = delta, v = x + 1;
+                            for (int k = 0; k < 5; ++k) {
+                                const uint32_t low = m & (0u - m);
+                                if (v & 1u) p |= low;
+                                v >>= 1;
+                                m ^= low;
+                            }
+                            const float c = s_cost[p] + s_cost[s ^ p];
+                            if (best_x == 0xFFFFFFFFu || c < best_cost) {
+                                best_cost = c;
+                                best_x = x;
+                                best_p = p;
+                            }
+                        }
+                    }
+                    // The group's first cheapest, in candidate order.
+                    const uint32_t base = 3 * (g < 21 ? g : 0);
+                    float gc = FLT_MAX;
+                    uint32_t gx = 0xFFFFFFFFu, gp = 0;
+                    for (uint32_t t = 0; t < 3; ++t) {
+                        const float oc = __shfl(best_cost, (int)(base + t), 64);
+                        const uint32_t ox = __shfl(best_x, (int)(base + t), 64);
+                        const uint32_t op = __shfl(best_p, (int)(base + t), 64);
+                        if (ox != 0xFFFFFFFFu && (gx == 0xFFFFFFFFu || oc < gc || (oc == gc && ox < gx))) {
+                            gc = oc;
+                            gx = ox;
+                            gp = op;
+                        }
+                    }
+                    if (g < 21 && j == 0) {
+                        s_cost[s] = kCostInternal * s_area[s] + gc;
+                        s_part[s] = gp;  // five leaves: odd, not flipped
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 if (kTreeletLeaves == 7 && bits == 6) {
                     const uint32_t g = lane >> 3, j = lane & 7u;
                     const uint32_t s = (kTreeletMasks - 1) ^ (1u << (g < 7 ? g : 0u));
@@ -680,7 +737,7 @@ template <int LEAVES>
 __global__ __launch_bounds__(64) void RestructureTreeletsKernel(BvhNode* nodes, uint32_t n, const uint32_t* counter,
                                                                 const uint32_t* __restrict__ roots, uint32_t* counts) {
     if (blockIdx.x >= *counter) return;
-    RestructureFromRoot<LEAVES>(nodes, n, roots[blockIdx.x], counts);
+    RestructureFromRoot<LEAVES, true>(nodes, n, roots[blockIdx.x], counts);
 }
 
 // ---------------------------------------------------------------------------
@@ -767,7 +824,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void Ba
     const uint2 r = roots[blockIdx.x];
     const BatchMesh bm = table[r.x];
     BvhNode* nodes = reinterpret_cast<BvhNode*>(RebaseOn<uint8_t>(roots_counter, bm.result) + sizeof(AccelHeader));
-    RestructureFromRoot<LEAVES>(nodes, bm.tri_count, r.y, BatchCounts(bm, roots_counter));
+    RestructureFromRoot<LEAVES, false>(nodes, bm.tri_count, r.y, BatchCounts(bm, roots_counter));
 }
 
 
