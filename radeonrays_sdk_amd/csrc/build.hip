@@ -338,10 +338,16 @@ __device__ __forceinline__ uint32_t ClimbToTreeletRoot(const BvhNode* __restrict
 // the workgroup is done), continues with the exchange climb.  The roots
 // found and the nodes marked are those of the climb from every leaf with
 // exchanges.  Every thread of the workgroup calls this (it holds barriers).
+//
+// Leaf parents come from the window too: the thread that loads internal
+// node i's parent reads i's child links from the same line and records i
+// as the parent of each child leaf inside the window, so a leaf reads its
+// node (a whole line for one link) only when its parent is outside.
 template <int W>
 struct RootsLds {
     uint32_t acc[W];
     uint32_t parent[W];
+    uint32_t leaf_parent[W];
 };
 template <int W>
 __device__ __forceinline__ uint32_t WindowTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t base,
@@ -349,9 +355,25 @@ __device__ __forceinline__ uint32_t WindowTreeletRoot(const BvhNode* __restrict_
                                                       uint32_t leaf, uint32_t min_prims, uint32_t* counts,
                                                       RootsLds<W>& s) {
     s.acc[threadIdx.x] = 0;
-    s.parent[threadIdx.x] = (own != kInvalid && own + 1 < n) ? nodes[own].parent : kInvalid;
+    s.leaf_parent[threadIdx.x] = kInvalid;
     __syncthreads();
-    uint32_t p = active ? nodes[leaf].parent : kInvalid;
+    uint32_t own_parent = kInvalid;
+    if (own != kInvalid && own + 1 < n) {
+        const BvhNode& nd = nodes[own];
+        own_parent = nd.parent;
+        const uint32_t c[2] = {nd.child0, nd.child1};
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t j = c[k] - (n - 1);  // leaf position, if c[k] is a leaf
+            if (c[k] >= n - 1 && c[k] < 2 * n - 1 && j - base < width) s.leaf_parent[slot0 + (j - base)] = own;
+        }
+    }
+    s.parent[threadIdx.x] = own_parent;
+    __syncthreads();
+    uint32_t p = kInvalid;
+    if (active) {
+        p = s.leaf_parent[slot0 + (leaf - (n - 1) - base)];
+        if (p == kInvalid) p = nodes[leaf].parent;
+    }
     uint32_t count = 1, l = 0, root = kInvalid;
     int state = active ? 1 : 0;  // 0 done, 1 climbing in LDS, 2 exchange climb from p, 3 first arrival at p
     for (uint32_t level = 0; state == 1; ++level) {
