@@ -343,18 +343,31 @@ __device__ __forceinline__ uint32_t ClimbToTreeletRoot(const BvhNode* __restrict
 // node i's parent reads i's child links from the same line and records i
 // as the parent of each child leaf inside the window, so a leaf reads its
 // node (a whole line for one link) only when its parent is outside.
+//
+// After the barrier the climbs that continue with exchanges (the few at
+// window edges) are handed to the first wave through LDS, as in
+// WindowRefit, so the workgroup's other waves end at once; a pending climb
+// carries its tree (nodes, leaf count, counts array) and its owner tag.
+constexpr int kRootPendingWords = 8;  // p, count, n, tag, nodes (2), counts (2)
 template <int W>
 struct RootsLds {
     uint32_t acc[W];
     uint32_t parent[W];
     uint32_t leaf_parent[W];
+    uint32_t pending[W * kRootPendingWords];
+    uint32_t npending;
 };
-template <int W>
-__device__ __forceinline__ uint32_t WindowTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t base,
-                                                      uint32_t width, uint32_t slot0, bool active, uint32_t own,
-                                                      uint32_t leaf, uint32_t min_prims, uint32_t* counts,
-                                                      RootsLds<W>& s) {
+// emit(tag, root) records a root found (tag: the caller's owner tag, e.g.
+// the mesh); counts_arg is any global-memory pointer of the launch, from
+// which handed-over counts arrays are addressed.
+template <int W, typename Emit>
+__device__ __forceinline__ void WindowTreeletRoot(const BvhNode* __restrict__ nodes, uint32_t n, uint32_t base,
+                                                  uint32_t width, uint32_t slot0, bool active, uint32_t own,
+                                                  uint32_t leaf, uint32_t min_prims, uint32_t* counts,
+                                                  uint32_t* counts_arg, uint32_t tag, RootsLds<W>& s,
+                                                  const Emit& emit) {
     s.acc[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s.npending = 0;
     s.leaf_parent[threadIdx.x] = kInvalid;
     __syncthreads();
     uint32_t own_parent = kInvalid;
@@ -402,8 +415,30 @@ __device__ __forceinline__ uint32_t WindowTreeletRoot(const BvhNode* __restrict_
     }
     __syncthreads();
     if (state == 3 && (s.acc[l] & 3u) == 1u) state = 2;
-    if (state == 2) root = ClimbCountsFrom(nodes, n, p, count, min_prims, counts);
-    return root;
+    if (root != kInvalid) emit(tag, root);
+    if (state == 2) {
+        uint32_t* q = s.pending + kRootPendingWords * atomicAdd(&s.npending, 1u);
+        const uint64_t nb = reinterpret_cast<uint64_t>(nodes), cb = reinterpret_cast<uint64_t>(counts);
+        q[0] = p;
+        q[1] = count;
+        q[2] = n;
+        q[3] = tag;
+        q[4] = (uint32_t)nb;
+        q[5] = (uint32_t)(nb >> 32);
+        q[6] = (uint32_t)cb;
+        q[7] = (uint32_t)(cb >> 32);
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    const uint32_t npending = s.npending;
+    for (uint32_t e = threadIdx.x; e < npending; e += 64) {
+        const uint32_t* q = s.pending + kRootPendingWords * e;
+        // Addressed from this thread's own pointers so that they stay global.
+        const BvhNode* tree = RebaseOn<const BvhNode>(nodes, (uint64_t)q[4] | ((uint64_t)q[5] << 32));
+        uint32_t* tcounts = RebaseOn<uint32_t>(counts_arg, (uint64_t)q[6] | ((uint64_t)q[7] << 32));
+        const uint32_t r = ClimbCountsFrom(tree, q[2], q[0], q[1], min_prims, tcounts);
+        if (r != kInvalid) emit(q[3], r);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* __restrict__ nodes, uint32_t n,
@@ -412,9 +447,9 @@ __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* 
     __shared__ RootsLds<kBlock> lds;
     const uint32_t base = blockIdx.x * kBlock;
     const uint32_t i = base + threadIdx.x;
-    const uint32_t root = WindowTreeletRoot<kBlock>(nodes, n, base, kBlock, 0, i < n, i + 1 < n ? i : kInvalid,
-                                                    n - 1 + i, min_prims, counts, lds);
-    if (root != kInvalid) roots[atomicAdd(counter, 1u)] = root;
+    WindowTreeletRoot<kBlock>(nodes, n, base, kBlock, 0, i < n, i + 1 < n ? i : kInvalid, n - 1 + i, min_prims,
+                              counts, counts, 0u, lds,
+                              [&](uint32_t, uint32_t root) { roots[atomicAdd(counter, 1u)] = root; });
 }
 
 // One wave per treelet root.  Lane 0 does the pointer-chasing parts (forming
@@ -826,10 +861,11 @@ __global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const Batc
     const BvhNode* nodes =
         reinterpret_cast<const BvhNode*>(RebaseOn<uint8_t>(roots_counter, bm.result) + sizeof(AccelHeader));
     uint32_t* counts = active ? BatchCounts(bm, roots_counter) : nullptr;
-    const uint32_t root = WindowTreeletRoot<kBlock>(
+    WindowTreeletRoot<kBlock>(
         nodes, n, (first - bm.first_chunk) * kRefitChunk, active ? (last - first) * kRefitChunk : 0,
-        (first - cb) * kRefitChunk, active, active && i + 1 < n ? i : kInvalid, n - 1 + i, min_prims, counts, lds);
-    if (root != kInvalid) roots[atomicAdd(roots_counter, 1u)] = make_uint2(lo, root);
+        (first - cb) * kRefitChunk, active, active && i + 1 < n ? i : kInvalid, n - 1 + i, min_prims, counts,
+        roots_counter, lo, lds,
+        [&](uint32_t mesh, uint32_t root) { roots[atomicAdd(roots_counter, 1u)] = make_uint2(mesh, root); });
 }
 
 // Eight waves per SIMD (64 VGPRs): the batched rounds run hundreds of
