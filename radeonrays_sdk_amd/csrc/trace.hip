@@ -430,10 +430,12 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                                                                   uint32_t* __restrict__ scratch, uint32_t min_busy,
                                                                   uint32_t fetch) {
     __shared__ uint32_t s_stack[DEPTH][kBlock];
-    // Each lane's world ray: origin xyz, direction xyz, and its set-up
-    // (reciprocal direction, shear, axis permutation), restored without
-    // divisions when the lane leaves an instance.
-    __shared__ float s_world[13][kBlock];
+    // Each lane's world ray: origin xyz, direction xyz and reciprocal
+    // direction, restored without divisions when the lane leaves an
+    // instance.  The TLAS level only tests boxes (instance leaves are entered
+    // with a fresh set-up), so the watertight shear and axis permutation of
+    // the world ray are not kept.
+    __shared__ float s_world[9][kBlock];
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
@@ -498,8 +500,6 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                 s_world[3][threadIdx.x] = ray.b.x; s_world[4][threadIdx.x] = ray.b.y; s_world[5][threadIdx.x] = ray.b.z;
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
                 s_world[6][threadIdx.x] = r.ix; s_world[7][threadIdx.x] = r.iy; s_world[8][threadIdx.x] = r.iz;
-                s_world[9][threadIdx.x] = r.sx; s_world[10][threadIdx.x] = r.sy; s_world[11][threadIdx.x] = r.sz;
-                s_world[12][threadIdx.x] = __uint_as_float(r.k);
                 best_t = ray.b.w;
                 best_prim = kInvalid;
                 best_inst = kInvalid;
@@ -591,8 +591,6 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                     boxes = false;
                     r.ox = s_world[0][threadIdx.x]; r.oy = s_world[1][threadIdx.x]; r.oz = s_world[2][threadIdx.x];
                     r.ix = s_world[6][threadIdx.x]; r.iy = s_world[7][threadIdx.x]; r.iz = s_world[8][threadIdx.x];
-                    r.sx = s_world[9][threadIdx.x]; r.sy = s_world[10][threadIdx.x]; r.sz = s_world[11][threadIdx.x];
-                    r.k = __float_as_uint(s_world[12][threadIdx.x]);
                     node = st.empty() ? kInvalid : st.pop();
                 } else if (inst != kInvalid && node < cur_total && leaf == kInvalid) {
                     leaf = node;
