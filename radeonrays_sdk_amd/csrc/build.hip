@@ -261,6 +261,24 @@ void LaunchMeshRefit(const MeshView& m, void* result, hipStream_t s) {
 static std::atomic<int> g_treelet_leaves{7};
 constexpr float kCostInternal = 1.2f;  // C_INT, restructure_bvh.comp:38
 
+// The subsets of 7 leaves with `bits` leaves, in increasing order
+// (C(7, 3) = C(7, 4) = 35 of them), for the DP's one-subset-per-lane levels.
+struct SubsetTable {
+    uint8_t mask[35];
+};
+constexpr SubsetTable SubsetsOf7(int bits) {
+    SubsetTable t{};
+    int k = 0;
+    for (int m = 0; m < 128; ++m) {
+        int c = 0;
+        for (int b = 0; b < 7; ++b) c += (m >> b) & 1;
+        if (c == bits && k < 35) t.mask[k++] = (uint8_t)m;
+    }
+    return t;
+}
+__constant__ constexpr SubsetTable kSubsets7Of3 = SubsetsOf7(3);
+__constant__ constexpr SubsetTable kSubsets7Of4 = SubsetsOf7(4);
+
 __device__ __forceinline__ float SurfaceArea(const float lo[3], const float hi[3]) {
     const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     float s = ex * ez;
@@ -563,6 +581,28 @@ __device__ __forceinline__ void RestructureFromRoot(BvhNode* nodes, uint32_t n, 
             //    subsets (127 without leaves a < b, 21 of them in (a, b)
             //    order) get three lanes each.
             for (int bits = 2; bits <= kTreeletLeaves; ++bits) {
+                if (kFiveLeafGroups && kTreeletLeaves == 7 && (bits == 3 || bits == 4)) {
+                    // One subset per lane (35 subsets, lanes 0-34).
+                    if (lane < 35) {
+                        const uint32_t s = bits == 3 ? kSubsets7Of3.mask[lane] : kSubsets7Of4.mask[lane];
+                        const uint32_t delta = (s - 1) & s;
+                        uint32_t p = (0u - delta) & s;
+                        float best_cost = FLT_MAX;
+                        uint32_t best = 0;
+                        do {
+                            const float c = s_cost[p] + s_cost[s ^ p];
+                            if (best == 0 || c < best_cost) {
+                                best_cost = c;
+                                best = p;
+                            }
+                            p = (p - delta) & s;
+                        } while (p != 0);
+                        s_cost[s] = kCostInternal * s_area[s] + best_cost;
+                        s_part[s] = (bits <= 5 && (bits & 1) == 0) ? (s ^ best) : best;
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 if (kFiveLeafGroups && kTreeletLeaves == 7 && bits == 5) {
                     const uint32_t g = lane / 3, j = lane - 3 * g;
                     uint32_t a = 0, b = 1;
