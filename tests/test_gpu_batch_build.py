@@ -62,6 +62,39 @@ def test_batch_build_equals_single_builds(ctx, dev, restructure):
         assert ok, f"oracle, mesh {k}: {msg}"
 
 
+def test_batch_build_uint16_indices_and_vertex_stride(ctx, dev):
+    """Indexed meshes with uint16 indices, half of them with a 16-byte
+    vertex stride (x, y, z, pad), on both sides of the 4,096- and
+    16,384-triangle limits of the batched kernels: the batch equals per-mesh
+    builds node for node, and the oracle's build of the same triangles."""
+    import torch
+
+    sizes = [2, 500, 4000, 4200, 16000, 16900, 30000]
+    meshes, dm = [], []
+    for k, n in enumerate(sizes):
+        res = max(2, int(round(np.sqrt(n / 2))) + 1)
+        v, i = scenes.heightfield_mesh(res, seed=scenes.SEED_BASE + 90 + k)
+        i = i.reshape(-1, 3)
+        assert v.shape[0] <= 65535
+        if k % 2:
+            v_dev = to_dev(np.concatenate([v, np.full((v.shape[0], 1), 7.0, np.float32)], axis=1), dev)
+        else:
+            v_dev = to_dev(v, dev)
+        meshes.append((v, i))
+        dm.append((v_dev, to_dev(i.astype(np.uint16).view(np.int16), dev)))
+    say(f"batched build, uint16 indices, strides 12/16, sizes {sizes}")
+    accels = ctx.build_geometry_batch(dm, restructure=True)
+    singles = [ctx.build_geometry(vd, idx, restructure=True) for vd, idx in dm]
+    torch.cuda.synchronize()
+    for k, (a, b) in enumerate(zip(accels, singles)):
+        ok, msg = nodes_equal(a.buffer.cpu().numpy(), b.buffer.cpu().numpy())
+        assert ok, f"mesh {k} (n={meshes[k][1].shape[0]}): {msg}"
+    for k in (1, 3, 5, 6):
+        v, i = meshes[k]
+        ok, msg = nodes_equal(accels[k].buffer.cpu().numpy(), oracle.build_blas(v, i, True))
+        assert ok, f"oracle, mesh {k}: {msg}"
+
+
 def test_batch_refit_equals_single_refits(ctx, dev):
     import torch
 
