@@ -66,7 +66,8 @@ def test_batch_build_uint16_indices_and_vertex_stride(ctx, dev):
     """Indexed meshes with uint16 indices, half of them with a 16-byte
     vertex stride (x, y, z, pad), on both sides of the 4,096- and
     16,384-triangle limits of the batched kernels: the batch equals per-mesh
-    builds node for node, and the oracle's build of the same triangles."""
+    builds node for node, and the oracle's build of the same triangles; a
+    batched refit of moved vertices equals per-mesh refits."""
     import torch
 
     sizes = [2, 500, 4000, 4200, 16000, 16900, 30000]
@@ -93,6 +94,20 @@ def test_batch_build_uint16_indices_and_vertex_stride(ctx, dev):
         v, i = meshes[k]
         ok, msg = nodes_equal(accels[k].buffer.cpu().numpy(), oracle.build_blas(v, i, True))
         assert ok, f"oracle, mesh {k}: {msg}"
+    # Refit with moved vertices, same formats: batched equals per-mesh refits.
+    moved = []
+    for k, (vd, idx) in enumerate(dm):
+        m = vd.clone()
+        m[:, 1] += 0.01 * torch.sin(torch.arange(m.shape[0], device=m.device, dtype=torch.float32) * (k + 1))
+        moved.append((m, idx))
+    ctx.update_geometry_batch(accels, moved)
+    for b, (m, idx) in zip(singles, moved):
+        ctx.update_geometry(b, m, idx)
+    torch.cuda.synchronize()
+    for k, (a, b) in enumerate(zip(accels, singles)):
+        ok, msg = nodes_equal(a.buffer.cpu().numpy(), b.buffer.cpu().numpy())
+        assert ok, f"refit, mesh {k}: {msg}"
+        assert oracle.check_consistency(a.buffer.cpu().numpy()), f"refit, mesh {k}"
 
 
 def test_batch_refit_equals_single_refits(ctx, dev):
