@@ -68,14 +68,15 @@ __device__ __forceinline__ int DeltaOf(const Code& code, int n, int i, uint32_t 
     return ci != b ? __clz(ci ^ b) : 32 + __clz((uint32_t)(i ^ j));
 }
 
-// Node i's topology (and leaf i's links) from the sorted codes: Karras 2012.
+// Node i's topology (and leaf i's primitive) from the sorted codes: Karras
+// 2012.  A leaf's other words (its vertices or box, child0 = kInvalid,
+// update = 0) are written by the refit that follows every emission
+// (WriteLeaf, RefitSceneKernel), which reads child1 and parent from here.
 template <typename Code>
 __device__ __forceinline__ void EmitNodeWith(const Code& code, const uint32_t* __restrict__ prims, uint32_t n,
                                              uint32_t i, BvhNode* __restrict__ nodes) {
     const uint32_t leaf = n - 1 + i;
-    nodes[leaf].child0 = kInvalid;
     nodes[leaf].child1 = prims[i];
-    nodes[leaf].update = 0;
     if (n == 1) {
         nodes[0].parent = kInvalid;
         return;
