@@ -60,15 +60,32 @@ typedef enum RRxTuning
     RRX_TUNING_SORT_DIGIT_BITS = 4,       /* digit width of rrxCmdSortPairs on >= 2^20 keys:
                                              8 (default), 10 (three passes for 30-bit keys)
                                              or 4 (the reference's radix, eight passes) */
-    RRX_TUNING_TREELET_LEAVES = 5         /* leaves per treelet of the restructuring that builds
+    RRX_TUNING_TREELET_LEAVES = 5,        /* leaves per treelet of the restructuring that builds
                                              without RR_BUILD_FLAG_BITS_PREFER_FAST_BUILD run:
                                              5, 7 (default, the reference's) or 9; changes the
                                              tree (so any-hit may report another hit), not the
                                              closest hits */
+    RRX_TUNING_SORT_SPIN_LIMIT = 6        /* test hook, not a speed setting: bound of a sort
+                                             tile's wait for its predecessors, in polls that find
+                                             one unpublished (default 2^22; 0 gives up at the
+                                             first).  A tile that gives up sorts wrongly and the
+                                             failure is reported (rrxGetDeviceErrors) */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);
 RR_API RRError rrxGetTuning(RRxTuning key, uint32_t* value);
+
+/* Failures the device reported since they were last read (bit k set for
+ * kind k), for work enqueued on client streams, where no rrWaitEvent call
+ * sees them; rrWaitEvent reports the same failures as RR_ERROR_INTERNAL.
+ * Waits for the device, then clears what it returns.  The reference has no
+ * counterpart (its only failure mode on the device is a wait that never
+ * ends, vlk/device.cpp:249-262). */
+#define RRX_DEVICE_ERROR_STACK_OVERFLOW   (1u << 0) /* a ray needed more than 80 stack entries; it ended early */
+#define RRX_DEVICE_ERROR_SORT_LOOKBACK    (1u << 1) /* a sort tile gave up waiting; that sort's order is wrong */
+#define RRX_DEVICE_ERROR_INVALID_BUFFER   (1u << 2) /* a query named a buffer without a valid header; its hits
+                                                       were not written */
+RR_API RRError rrxGetDeviceErrors(RRContext context, uint32_t* errors);
 
 /* Scratch bytes for rrxCmdBuildGeometryBatch over `count` build inputs
  * (each one triangle mesh, as for rrGetGeometryBuildMemoryRequirements).

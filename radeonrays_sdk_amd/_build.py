@@ -22,11 +22,15 @@ OBJ_DIR = PKG_DIR / "_obj"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
+# Only the rr*/rrx* entry points (declared RR_API, i.e. default visibility,
+# in include/*.h) are exported; everything else stays inside the library.
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
     "-fPIC",
+    "-fvisibility=hidden",
+    "-fvisibility-inlines-hidden",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -41,8 +45,9 @@ def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ_DIR / (src.stem + ".o")
     cmd = [HIPCC, *DEVICE_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-Wno-unused-result", f"-I{INCLUDE}",
-               f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden",
+               "-fvisibility-inlines-hidden", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o",
+               str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     errs = "\n".join(l for l in proc.stderr.splitlines() if "packed-fp32-ops" not in l)
     if proc.returncode != 0:
@@ -60,7 +65,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", f"-Wl,--version-script={CSRC / 'exports.map'}",
+           *map(str, objs), "-o", str(tmp)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")

@@ -11,6 +11,13 @@
 // context's HIP stream and returns an event (the reference records into a
 // Vulkan command buffer and submits it, vlk/device.cpp:221-240).  A stream
 // wrapped from a client hipStream_t enqueues immediately on that stream.
+//
+// Handles: every context, device pointer, command stream and event the
+// library hands out is kept in a registry until it is released, and each
+// entry point checks the handles it is given against it, so a stale or
+// foreign handle gives RR_ERROR_INVALID_PARAMETER instead of being
+// dereferenced.  No entry point that records work reads device memory: the
+// level of a buffer the host has not seen built is decided on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +32,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/radeonrays.h"
@@ -33,6 +41,7 @@
 #include "api_internal.h"
 #include "batch.h"
 #include "build.h"
+#include "device_errors.h"
 #include "rr_layout.h"
 #include "sort.h"
 #include "trace.h"
@@ -71,6 +80,11 @@ Logger& Log() {
 }
 #define RR_LOG(lvl, ...) Log().Log(lvl, __VA_ARGS__)
 
+// A handle that is not live (never handed out, or already released).
+struct BadHandle : std::runtime_error {
+    BadHandle() : std::runtime_error("handle is not live") {}
+};
+
 struct HipError : std::runtime_error {
     hipError_t code;
     HipError(hipError_t c, const char* what) : std::runtime_error(what), code(c) {}
@@ -81,6 +95,37 @@ void Check(hipError_t e, const char* what) {
 
 // ---------------------------------------------------------------------------
 // Handles.
+
+// Live handles of one kind.
+class HandleSet {
+public:
+    void Add(const void* p) {
+        std::lock_guard<std::mutex> g(mu_);
+        set_.insert(p);
+    }
+    bool Remove(const void* p) {
+        std::lock_guard<std::mutex> g(mu_);
+        return set_.erase(p) != 0;
+    }
+    bool Has(const void* p) const {
+        if (!p) return false;
+        std::lock_guard<std::mutex> g(mu_);
+        return set_.count(p) != 0;
+    }
+
+private:
+    mutable std::mutex mu_;
+    std::unordered_set<const void*> set_;
+};
+HandleSet& LiveContexts() { static HandleSet s; return s; }
+HandleSet& LivePtrs() { static HandleSet s; return s; }
+HandleSet& LiveStreams() { static HandleSet s; return s; }
+HandleSet& LiveEvents() { static HandleSet s; return s; }
+bool LiveCtx(RRContext c) { return LiveContexts().Has(c); }
+bool LivePtr(RRDevicePtr p) { return LivePtrs().Has(p); }
+bool OptPtr(RRDevicePtr p) { return !p || LivePtrs().Has(p); }  // null allowed
+bool LiveStream(RRCommandStream s) { return LiveStreams().Has(s); }
+bool LiveEvent(RREvent e) { return LiveEvents().Has(e); }
 
 struct DevicePtr {
     void*  base = nullptr;
@@ -95,6 +140,7 @@ using Command = std::function<void(hipStream_t)>;
 
 struct Event {
     hipEvent_t ev = nullptr;
+    bool pooled = false;  // released by the client, kept for reuse
 };
 
 struct Context;
@@ -112,12 +158,18 @@ struct PendingFree {
     void* host;
 };
 
+// What the context knows of a result buffer it recorded a build into.
+struct AccelInfo {
+    rr::AccelKind kind;
+    uint32_t prims;  // triangles or instances
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true;
     std::mutex mu;
-    std::unordered_map<uintptr_t, rr::AccelKind> kinds;  // result buffer -> BLAS/TLAS
+    std::unordered_map<uintptr_t, AccelInfo> kinds;  // result buffer -> kind and size
     std::vector<PendingFree> pending;                      // pinned staging to free once done
     std::vector<Event*> event_pool;
     // Pinned staging blocks kept for reuse (hipHostMalloc costs far more than
@@ -206,7 +258,18 @@ Context* AsCtx(RRContext c) { return reinterpret_cast<Context*>(c); }
 DevicePtr* AsPtr(RRDevicePtr p) { return reinterpret_cast<DevicePtr*>(p); }
 CommandStream* AsStream(RRCommandStream s) { return reinterpret_cast<CommandStream*>(s); }
 Event* AsEvent(RREvent e) { return reinterpret_cast<Event*>(e); }
-void* Addr(RRDevicePtr p) { return p ? AsPtr(p)->addr() : nullptr; }
+// Device address behind a handle (null stays null); throws BadHandle for a
+// handle that is not live, which Guarded turns into RR_ERROR_INVALID_PARAMETER.
+void* Addr(RRDevicePtr p) {
+    if (!p) return nullptr;
+    if (!LivePtr(p)) throw BadHandle();
+    return AsPtr(p)->addr();
+}
+
+void RecordAccel(Context* c, void* result, rr::AccelKind kind, uint32_t prims) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->kinds[reinterpret_cast<uintptr_t>(result)] = AccelInfo{kind, prims};
+}
 
 struct DeviceGuard {
     int prev = -1;
@@ -274,6 +337,9 @@ template <typename F>
 RRError Guarded(const char* name, F&& f) {
     try {
         return f();
+    } catch (const BadHandle&) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "%s: invalid parameter (a device pointer handle is not live)", name);
+        return RR_ERROR_INVALID_PARAMETER;
     } catch (const HipError& e) {
         RR_LOG(RR_LOG_LEVEL_ERROR, "%s: %s", name, e.what());
         return (e.code == hipErrorOutOfMemory) ? RR_ERROR_OUT_OF_DEVICE_MEMORY : RR_ERROR_INTERNAL;
@@ -402,12 +468,60 @@ RRError CreateContext(int device, hipStream_t stream, RRContext* out) {
         } else {
             Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         }
+        rr::InitDeviceErrors();
+        LiveContexts().Add(c);
         *out = reinterpret_cast<RRContext>(c);
         return RR_SUCCESS;
     });
 }
 
+// Report device-side failures seen since the last report (device_errors.h).
+RRError DeviceErrorsAsResult(const char* name) {
+    const uint32_t e = rr::TakeDeviceErrors();
+    if (!e) return RR_SUCCESS;
+    RR_LOG(RR_LOG_LEVEL_ERROR, "%s: device reported%s%s%s", name, (e >> rr::kErrStackOverflow) & 1 ? " traversal stack overflow" : "",
+           (e >> rr::kErrLookbackTimeout) & 1 ? " sort look-back timeout" : "",
+           (e >> rr::kErrBadAccel) & 1 ? " a query on a buffer without a valid header" : "");
+    return RR_ERROR_INTERNAL;
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Device error words (device_errors.h): pinned, coherent, mapped host memory.
+
+namespace {
+uint32_t* g_err_host = nullptr;    // host view
+uint32_t* g_err_device = nullptr;  // the address kernels store to
+std::once_flag g_err_once;
+}  // namespace
+
+void rr::InitDeviceErrors() {
+    std::call_once(g_err_once, [] {
+        void* p = nullptr;
+        Check(hipHostMalloc(&p, 256, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
+        std::memset(p, 0, 256);
+        void* d = nullptr;
+        Check(hipHostGetDevicePointer(&d, p, 0), "hipHostGetDevicePointer");
+        g_err_host = static_cast<uint32_t*>(p);
+        g_err_device = static_cast<uint32_t*>(d);
+    });
+}
+
+uint32_t* rr::DeviceErrorWords() { return g_err_device; }
+
+uint32_t rr::TakeDeviceErrors() {
+    if (!g_err_host) return 0;
+    uint32_t bits = 0;
+    for (uint32_t k = 0; k < rr::kErrKinds; ++k) {
+        volatile uint32_t* w = g_err_host + k;
+        if (*w) {
+            bits |= 1u << k;
+            *w = 0;
+        }
+    }
+    return bits;
+}
 
 extern "C" {
 
@@ -434,14 +548,16 @@ RRError rrCreateContextHip(uint32_t api_version, int device, void* hip_stream, R
 
 RRError rrDestroyContext(RRContext context) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrDestroyContext");
-    RR_REQUIRE(context, "rrDestroyContext");
+    RR_REQUIRE(LiveCtx(context), "rrDestroyContext");
     return Guarded("rrDestroyContext", [&]() -> RRError {
         Context* c = AsCtx(context);
+        LiveContexts().Remove(c);
         DeviceGuard g(c->device);
         (void)hipStreamSynchronize(c->stream);
         c->Reap(true);
         c->FreePool();
         for (Event* e : c->event_pool) {
+            LiveEvents().Remove(e);
             (void)hipEventDestroy(e->ev);
             delete e;
         }
@@ -472,7 +588,9 @@ RRError rrSetLogFile(char const* filename) {
 }
 
 // ---------------------------------------------------------------------------
-// Memory requirements
+// Memory requirements.  Sizes are functions of the build input alone: these
+// queries read nothing from the context (as long as it is not null), so
+// they also answer where no device is present.
 
 RRError rrGetGeometryBuildMemoryRequirements(RRContext context, const RRGeometryBuildInput* build_input,
                                              const RRBuildOptions* build_options,
@@ -516,7 +634,9 @@ RRError rrGetSceneBuildMemoryRequirements(RRContext context, const RRSceneBuildI
     RR_REQUIRE(build_input->instance_count < (1u << 30), "rrGetSceneBuildMemoryRequirements");
     memory_requirements->result_buffer_size = rr::TlasResultSize(build_input->instance_count);
     memory_requirements->temporary_build_buffer_size = rr::TlasBuildScratchBytes(build_input->instance_count);
-    memory_requirements->temporary_update_buffer_size = 0;
+    // A scene refit (rrxCmdUpdateScene) stages the instances and their world
+    // boxes in scratch.
+    memory_requirements->temporary_update_buffer_size = rr::TlasUpdateScratchBytes(build_input->instance_count);
     return RR_SUCCESS;
 }
 
@@ -535,49 +655,40 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
                            const RRBuildOptions* build_options, RRDevicePtr temporary_buffer,
                            RRDevicePtr geometry_buffer, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdBuildGeometry");
-    RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildGeometry");
+    RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && build_input, "rrCmdBuildGeometry");
     RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
                "rrCmdBuildGeometry");
-    if (IsAabbList(build_input)) {
-        rr::AabbDesc a;
-        const RRError e = AabbsFromInput(build_input, a, "rrCmdBuildGeometry", true);
-        if (e != RR_SUCCESS) return e;
-        RR_REQUIRE(geometry_buffer && temporary_buffer, "rrCmdBuildGeometry");
-        return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
-            Context* c = AsCtx(context);
+    RR_REQUIRE(OptPtr(temporary_buffer) && OptPtr(geometry_buffer), "rrCmdBuildGeometry");
+    return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
+        Context* c = AsCtx(context);
+        CommandStream* cs = AsStream(command_stream);
+        const bool restructure = WantsRestructure(build_options);
+        const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
+        if (IsAabbList(build_input)) {
+            rr::AabbDesc a;
+            const RRError e = AabbsFromInput(build_input, a, "rrCmdBuildGeometry", true);
+            if (e != RR_SUCCESS) return e;
+            RR_REQUIRE(geometry_buffer && temporary_buffer, "rrCmdBuildGeometry");
             DeviceGuard g(c->device);
             void* result = Addr(geometry_buffer);
             void* scratch = Addr(temporary_buffer);
-            const bool restructure = WantsRestructure(build_options);
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelBlas;
-            }
-            if (build_operation == RR_BUILD_OPERATION_BUILD)
-                Record(AsStream(command_stream), [=](hipStream_t s) { rr::BuildAabbs(a, restructure, scratch, result, s); });
+            RecordAccel(c, result, rr::kAccelBlas, a.count);
+            if (!update)
+                Record(cs, [=](hipStream_t s) { rr::BuildAabbs(a, restructure, scratch, result, s); });
             else
-                Record(AsStream(command_stream), [=](hipStream_t s) { rr::UpdateAabbs(a, scratch, result, s); });
+                Record(cs, [=](hipStream_t s) { rr::UpdateAabbs(a, scratch, result, s); });
             return RR_SUCCESS;
-        });
-    }
-    if (IsMultiMesh(build_input)) {
-        std::vector<rr::SubMesh> subs;
-        uint32_t tris = 0;
-        const RRError e = SubMeshesFromInput(build_input, subs, tris, "rrCmdBuildGeometry", true);
-        if (e != RR_SUCCESS) return e;
-        RR_REQUIRE(geometry_buffer && temporary_buffer, "rrCmdBuildGeometry");
-        return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
-            Context* c = AsCtx(context);
-            CommandStream* cs = AsStream(command_stream);
+        }
+        if (IsMultiMesh(build_input)) {
+            std::vector<rr::SubMesh> subs;
+            uint32_t tris = 0;
+            const RRError e = SubMeshesFromInput(build_input, subs, tris, "rrCmdBuildGeometry", true);
+            if (e != RR_SUCCESS) return e;
+            RR_REQUIRE(geometry_buffer && temporary_buffer, "rrCmdBuildGeometry");
             DeviceGuard g(c->device);
             void* result = Addr(geometry_buffer);
             void* scratch = Addr(temporary_buffer);
-            const bool restructure = WantsRestructure(build_options);
-            const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelBlas;
-            }
+            RecordAccel(c, result, rr::kAccelBlas, tris);
             const uint32_t count = (uint32_t)subs.size();
             const size_t bytes = sizeof(rr::SubMesh) * count;
             void* host = AcquireStaging(c, cs, bytes);
@@ -590,27 +701,20 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
             });
             KeepStaging(c, cs, host);
             return RR_SUCCESS;
-        });
-    }
-    rr::MeshDesc m;
-    const RRError e = MeshFromInput(build_input, m, "rrCmdBuildGeometry", true);
-    if (e != RR_SUCCESS) return e;
-    RR_REQUIRE(geometry_buffer, "rrCmdBuildGeometry");
-    if (build_operation == RR_BUILD_OPERATION_BUILD) RR_REQUIRE(temporary_buffer, "rrCmdBuildGeometry");
-    return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
-        Context* c = AsCtx(context);
+        }
+        rr::MeshDesc m;
+        const RRError e = MeshFromInput(build_input, m, "rrCmdBuildGeometry", true);
+        if (e != RR_SUCCESS) return e;
+        RR_REQUIRE(geometry_buffer, "rrCmdBuildGeometry");
+        if (!update) RR_REQUIRE(temporary_buffer, "rrCmdBuildGeometry");
         DeviceGuard g(c->device);
         void* result = Addr(geometry_buffer);
         void* scratch = Addr(temporary_buffer);
-        const bool restructure = WantsRestructure(build_options);
-        {
-            std::lock_guard<std::mutex> lk(c->mu);
-            c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelBlas;
-        }
-        if (build_operation == RR_BUILD_OPERATION_BUILD)
-            Record(AsStream(command_stream), [=](hipStream_t s) { rr::BuildBlas(m, restructure, scratch, result, s); });
+        RecordAccel(c, result, rr::kAccelBlas, m.tri_count);
+        if (!update)
+            Record(cs, [=](hipStream_t s) { rr::BuildBlas(m, restructure, scratch, result, s); });
         else
-            Record(AsStream(command_stream), [=](hipStream_t s) { rr::UpdateBlas(m, result, s); });
+            Record(cs, [=](hipStream_t s) { rr::UpdateBlas(m, result, s); });
         return RR_SUCCESS;
     });
 }
@@ -618,12 +722,12 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
 // Build (or refit) a scene over the instances, as rrCmdBuildScene.
 static RRError SceneCommand(bool update, RRContext context, const RRSceneBuildInput* build_input,
                             RRDevicePtr temporary_buffer, RRDevicePtr scene_buffer, RRCommandStream command_stream) {
-    RR_REQUIRE(context && command_stream && build_input, "rrCmdBuildScene");
-    RR_REQUIRE(scene_buffer && temporary_buffer, "rrCmdBuildScene");
+    RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && build_input, "rrCmdBuildScene");
+    RR_REQUIRE(LivePtr(scene_buffer) && LivePtr(temporary_buffer), "rrCmdBuildScene");
     RR_REQUIRE(build_input->instance_count == 0 || build_input->instances, "rrCmdBuildScene");
     RR_REQUIRE(build_input->instance_count < (1u << 30), "rrCmdBuildScene");
     for (uint32_t i = 0; i < build_input->instance_count; ++i)
-        RR_REQUIRE(build_input->instances[i].geometry, "rrCmdBuildScene");
+        RR_REQUIRE(LivePtr(build_input->instances[i].geometry), "rrCmdBuildScene");
     return Guarded("rrCmdBuildScene", [&]() -> RRError {
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
@@ -631,6 +735,17 @@ static RRError SceneCommand(bool update, RRContext context, const RRSceneBuildIn
         const uint32_t count = build_input->instance_count;
         void* result = Addr(scene_buffer);
         void* scratch = Addr(temporary_buffer);
+        if (update) {
+            // A refit keeps the scene's layout, which depends on the instance
+            // count: refuse another count for a scene this context built.
+            std::lock_guard<std::mutex> lk(c->mu);
+            auto it = c->kinds.find(reinterpret_cast<uintptr_t>(result));
+            if (it != c->kinds.end() && (it->second.kind != rr::kAccelTlas || it->second.prims != count)) {
+                RR_LOG(RR_LOG_LEVEL_ERROR, "rrxCmdUpdateScene: invalid parameter (instance count %u, scene built with %u)",
+                       count, it->second.prims);
+                return RR_ERROR_INVALID_PARAMETER;
+            }
+        }
         // Pack the instances now (the caller's array may go away after this
         // call, as in intersector.cpp:218-247) into pinned memory that the
         // recorded upload reads at execution time.
@@ -643,10 +758,7 @@ static RRError SceneCommand(bool update, RRContext context, const RRSceneBuildIn
                 host[i].pad = 0;
             }
         }
-        {
-            std::lock_guard<std::mutex> lk(c->mu);
-            c->kinds[reinterpret_cast<uintptr_t>(result)] = rr::kAccelTlas;
-        }
+        RecordAccel(c, result, rr::kAccelTlas, count);
         auto cmd = [=](hipStream_t s) {
             auto* dev_inst = reinterpret_cast<rr::InstanceUpload*>(static_cast<uint8_t*>(scratch) + rr::TlasUploadOffset());
             if (count)
@@ -673,29 +785,24 @@ RRError rrxCmdUpdateScene(RRContext context, const RRSceneBuildInput* build_inpu
     return SceneCommand(true, context, build_input, temporary_buffer, scene_buffer, command_stream);
 }
 
-static RRError KindOf(Context* c, void* accel, rr::AccelKind& kind) {
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        auto it = c->kinds.find(reinterpret_cast<uintptr_t>(accel));
-        if (it != c->kinds.end()) {
-            kind = it->second;
-            return RR_SUCCESS;
-        }
-    }
-    // Built elsewhere: read the buffer's own header.
-    rr::AccelHeader h;
-    Check(hipMemcpy(&h, accel, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (h.magic != rr::kAccelMagic || (h.kind != rr::kAccelBlas && h.kind != rr::kAccelTlas))
-        return RR_ERROR_INVALID_PARAMETER;
-    kind = static_cast<rr::AccelKind>(h.kind);
-    return RR_SUCCESS;
+// Kind of a result buffer as this context recorded it, or kAccelUnknown for
+// a buffer built elsewhere (another context, a copy): traversal then reads
+// the kind from the buffer's header on the device, so recording never waits
+// for the device (the reference decides through a host map,
+// vlk/intersector.cpp:289-291).
+static rr::AccelKind KindOf(Context* c, void* accel) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->kinds.find(reinterpret_cast<uintptr_t>(accel));
+    return it != c->kinds.end() ? it->second.kind : rr::kAccelUnknown;
 }
 
 RRError rrCmdIntersect(RRContext context, RRDevicePtr scene_buffer, RRIntersectQuery query, RRDevicePtr rays,
                        uint32_t ray_count, RRDevicePtr indirect_ray_count, RRIntersectQueryOutput query_output,
                        RRDevicePtr hits, RRDevicePtr scratch, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdIntersect");
-    RR_REQUIRE(context && scene_buffer && rays && hits && scratch && command_stream, "rrCmdIntersect");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(scene_buffer) && LivePtr(rays) && LivePtr(hits) && LivePtr(scratch) &&
+                   LiveStream(command_stream) && OptPtr(indirect_ray_count),
+               "rrCmdIntersect");
     RR_REQUIRE(query == RR_INTERSECT_QUERY_CLOSEST || query == RR_INTERSECT_QUERY_ANY, "rrCmdIntersect");
     RR_REQUIRE(query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT || query_output == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
                    query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
@@ -704,9 +811,7 @@ RRError rrCmdIntersect(RRContext context, RRDevicePtr scene_buffer, RRIntersectQ
         Context* c = AsCtx(context);
         DeviceGuard g(c->device);
         void* accel = Addr(scene_buffer);
-        rr::AccelKind kind;
-        const RRError e = KindOf(c, accel, kind);
-        if (e != RR_SUCCESS) return e;
+        const rr::AccelKind kind = KindOf(c, accel);
         const void* r = Addr(rays);
         void* h = Addr(hits);
         void* sc = Addr(scratch);
@@ -723,10 +828,11 @@ RRError rrCmdIntersect(RRContext context, RRDevicePtr scene_buffer, RRIntersectQ
 
 RRError rrAllocateCommandStream(RRContext context, RRCommandStream* command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrAllocateCommandStream");
-    RR_REQUIRE(context && command_stream, "rrAllocateCommandStream");
+    RR_REQUIRE(LiveCtx(context) && command_stream, "rrAllocateCommandStream");
     return Guarded("rrAllocateCommandStream", [&]() -> RRError {
         auto* cs = new CommandStream;
         cs->ctx = AsCtx(context);
+        LiveStreams().Add(cs);
         *command_stream = reinterpret_cast<RRCommandStream>(cs);
         return RR_SUCCESS;
     });
@@ -745,10 +851,12 @@ static void ReleaseStaging(Context* c, CommandStream* cs) {
 
 RRError rrReleaseCommandStream(RRContext context, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseCommandStream");
-    RR_REQUIRE(context && command_stream, "rrReleaseCommandStream");
+    RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && !AsStream(command_stream)->external,
+               "rrReleaseCommandStream");
     return Guarded("rrReleaseCommandStream", [&]() -> RRError {
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
+        LiveStreams().Remove(cs);
         DeviceGuard g(c->device);
         ReleaseStaging(c, cs);
         delete cs;
@@ -758,7 +866,8 @@ RRError rrReleaseCommandStream(RRContext context, RRCommandStream command_stream
 
 RRError rrSumbitCommandStream(RRContext context, RRCommandStream command_stream, RREvent wait_event, RREvent* out_event) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrSumbitCommandStream");
-    RR_REQUIRE(context && command_stream && out_event, "rrSumbitCommandStream");
+    RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && out_event, "rrSumbitCommandStream");
+    RR_REQUIRE(!wait_event || LiveEvent(wait_event), "rrSumbitCommandStream");
     return Guarded("rrSumbitCommandStream", [&]() -> RRError {
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
@@ -782,7 +891,9 @@ RRError rrSumbitCommandStream(RRContext context, RRCommandStream command_stream,
         if (!ev) {
             ev = new Event;
             Check(hipEventCreateWithFlags(&ev->ev, hipEventDisableTiming), "hipEventCreate");
+            LiveEvents().Add(ev);
         }
+        ev->pooled = false;
         Check(hipEventRecord(ev->ev, s), "hipEventRecord");
         *out_event = reinterpret_cast<RREvent>(ev);
         return RR_SUCCESS;
@@ -791,42 +902,50 @@ RRError rrSumbitCommandStream(RRContext context, RRCommandStream command_stream,
 
 RRError rrReleaseEvent(RRContext context, RREvent event) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseEvent");
-    RR_REQUIRE(context && event, "rrReleaseEvent");
+    RR_REQUIRE(LiveCtx(context) && LiveEvent(event), "rrReleaseEvent");
     Context* c = AsCtx(context);
     std::lock_guard<std::mutex> lk(c->mu);
-    c->event_pool.push_back(AsEvent(event));
+    Event* e = AsEvent(event);
+    if (e->pooled) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "rrReleaseEvent: invalid parameter (event already released)");
+        return RR_ERROR_INVALID_PARAMETER;
+    }
+    e->pooled = true;
+    c->event_pool.push_back(e);
     return RR_SUCCESS;
 }
 
 RRError rrWaitEvent(RRContext context, RREvent event) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrWaitEvent");
-    RR_REQUIRE(context && event, "rrWaitEvent");
+    RR_REQUIRE(LiveCtx(context) && LiveEvent(event) && !AsEvent(event)->pooled, "rrWaitEvent");
     return Guarded("rrWaitEvent", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
         // Poll first (short queries finish in tens of microseconds, less than
-        // a blocking wait takes to wake up), then block.
+        // a blocking wait takes to wake up), then block.  Failures the
+        // device reported while the work ran make the wait fail.
         const hipEvent_t ev = AsEvent(event)->ev;
         const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(WaitSpinMicros());
         for (;;) {
             const hipError_t q = hipEventQuery(ev);
-            if (q == hipSuccess) return RR_SUCCESS;
+            if (q == hipSuccess) return DeviceErrorsAsResult("rrWaitEvent");
             if (q != hipErrorNotReady) Check(q, "hipEventQuery");
             if (std::chrono::steady_clock::now() >= until) break;
             std::this_thread::yield();
         }
         Check(hipEventSynchronize(ev), "hipEventSynchronize");
-        return RR_SUCCESS;
+        return DeviceErrorsAsResult("rrWaitEvent");
     });
 }
 
 RRError rrGetCommandStreamFromHipStream(RRContext context, void* hip_stream, RRCommandStream* command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrGetCommandStreamFromHipStream");
-    RR_REQUIRE(context && command_stream, "rrGetCommandStreamFromHipStream");
+    RR_REQUIRE(LiveCtx(context) && command_stream, "rrGetCommandStreamFromHipStream");
     return Guarded("rrGetCommandStreamFromHipStream", [&]() -> RRError {
         auto* cs = new CommandStream;
         cs->ctx = AsCtx(context);
         cs->external = true;
         cs->stream = static_cast<hipStream_t>(hip_stream);
+        LiveStreams().Add(cs);
         *command_stream = reinterpret_cast<RRCommandStream>(cs);
         return RR_SUCCESS;
     });
@@ -834,13 +953,15 @@ RRError rrGetCommandStreamFromHipStream(RRContext context, void* hip_stream, RRC
 
 RRError rrReleaseExternalCommandStream(RRContext context, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseExternalCommandStream");
-    RR_REQUIRE(context && command_stream, "rrReleaseExternalCommandStream");
+    RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && AsStream(command_stream)->external,
+               "rrReleaseExternalCommandStream");
+    LiveStreams().Remove(command_stream);
     delete AsStream(command_stream);
     return RR_SUCCESS;
 }
 
 void* rrGetHipStream(RRCommandStream command_stream) {
-    if (!command_stream) return nullptr;
+    if (!LiveStream(command_stream)) return nullptr;
     CommandStream* cs = AsStream(command_stream);
     return cs->external ? cs->stream : cs->ctx->stream;
 }
@@ -850,11 +971,12 @@ void* rrGetHipStream(RRCommandStream command_stream) {
 
 RRError rrGetDevicePtrFromHipPtr(RRContext context, void* device_address, size_t offset, RRDevicePtr* device_ptr) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrGetDevicePtrFromHipPtr");
-    RR_REQUIRE(context && device_address && device_ptr, "rrGetDevicePtrFromHipPtr");
+    RR_REQUIRE(LiveCtx(context) && device_address && device_ptr, "rrGetDevicePtrFromHipPtr");
     return Guarded("rrGetDevicePtrFromHipPtr", [&]() -> RRError {
         auto* p = new DevicePtr;
         p->base = device_address;
         p->offset = offset;
+        LivePtrs().Add(p);
         *device_ptr = reinterpret_cast<RRDevicePtr>(p);
         return RR_SUCCESS;
     });
@@ -862,9 +984,10 @@ RRError rrGetDevicePtrFromHipPtr(RRContext context, void* device_address, size_t
 
 RRError rrReleaseDevicePtr(RRContext context, RRDevicePtr ptr) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrReleaseDevicePtr");
-    RR_REQUIRE(context && ptr, "rrReleaseDevicePtr");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(ptr), "rrReleaseDevicePtr");
     return Guarded("rrReleaseDevicePtr", [&]() -> RRError {
         DevicePtr* p = AsPtr(ptr);
+        LivePtrs().Remove(p);
         if (p->owned) {
             DeviceGuard g(AsCtx(context)->device);
             (void)hipStreamSynchronize(AsCtx(context)->stream);
@@ -878,7 +1001,7 @@ RRError rrReleaseDevicePtr(RRContext context, RRDevicePtr ptr) {
 
 RRError rrAllocateDeviceBuffer(RRContext context, size_t size, RRDevicePtr* device_ptr) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrAllocateDeviceBuffer(%zu)", size);
-    RR_REQUIRE(context && size && device_ptr, "rrAllocateDeviceBuffer");
+    RR_REQUIRE(LiveCtx(context) && size && device_ptr, "rrAllocateDeviceBuffer");
     return Guarded("rrAllocateDeviceBuffer", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
         void* mem = nullptr;
@@ -887,6 +1010,7 @@ RRError rrAllocateDeviceBuffer(RRContext context, size_t size, RRDevicePtr* devi
         p->base = mem;
         p->owned = true;
         p->size = size;
+        LivePtrs().Add(p);
         *device_ptr = reinterpret_cast<RRDevicePtr>(p);
         return RR_SUCCESS;
     });
@@ -894,7 +1018,7 @@ RRError rrAllocateDeviceBuffer(RRContext context, size_t size, RRDevicePtr* devi
 
 RRError rrMapDevicePtr(RRContext context, RRDevicePtr device_ptr, void** mapping_ptr) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrMapDevicePtr");
-    RR_REQUIRE(context && device_ptr && mapping_ptr, "rrMapDevicePtr");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(device_ptr) && mapping_ptr, "rrMapDevicePtr");
     DevicePtr* p = AsPtr(device_ptr);
     RR_REQUIRE(p->owned, "rrMapDevicePtr");
     return Guarded("rrMapDevicePtr", [&]() -> RRError {
@@ -913,7 +1037,7 @@ RRError rrMapDevicePtr(RRContext context, RRDevicePtr device_ptr, void** mapping
 
 RRError rrUnmapDevicePtr(RRContext context, RRDevicePtr device_ptr, void** mapping_ptr) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrUnmapDevicePtr");
-    RR_REQUIRE(context && device_ptr && mapping_ptr, "rrUnmapDevicePtr");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(device_ptr) && mapping_ptr, "rrUnmapDevicePtr");
     DevicePtr* p = AsPtr(device_ptr);
     RR_REQUIRE(p->owned && p->mapped, "rrUnmapDevicePtr");
     return Guarded("rrUnmapDevicePtr", [&]() -> RRError {
@@ -926,7 +1050,7 @@ RRError rrUnmapDevicePtr(RRContext context, RRDevicePtr device_ptr, void** mappi
     });
 }
 
-void* rrGetHipAddress(RRDevicePtr device_ptr) { return Addr(device_ptr); }
+void* rrGetHipAddress(RRDevicePtr device_ptr) { return LivePtr(device_ptr) ? AsPtr(device_ptr)->addr() : nullptr; }
 
 // ---------------------------------------------------------------------------
 // Extensions
@@ -941,7 +1065,9 @@ RRError rrxGetSortMemoryRequirements(RRContext context, uint32_t n, size_t* scra
 RRError rrxCmdSortPairs(RRContext context, RRDevicePtr keys, RRDevicePtr values, RRDevicePtr keys_tmp,
                         RRDevicePtr values_tmp, uint32_t n, uint32_t key_bits, RRDevicePtr scratch,
                         RRCommandStream command_stream) {
-    RR_REQUIRE(context && keys && values && keys_tmp && values_tmp && scratch && command_stream, "rrxCmdSortPairs");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(keys) && LivePtr(values) && LivePtr(keys_tmp) && LivePtr(values_tmp) &&
+                   LivePtr(scratch) && LiveStream(command_stream),
+               "rrxCmdSortPairs");
     RR_REQUIRE(n < (1u << 30) && key_bits >= 1 && key_bits <= 32, "rrxCmdSortPairs");
     return Guarded("rrxCmdSortPairs", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
@@ -965,14 +1091,14 @@ RRError rrxCmdSortPairs(RRContext context, RRDevicePtr keys, RRDevicePtr values,
 RRError rrxCmdIntersectWithStats(RRContext context, RRDevicePtr scene_buffer, RRIntersectQuery query, RRDevicePtr rays,
                                  uint32_t ray_count, RRDevicePtr hits, RRDevicePtr scratch, RRDevicePtr stats,
                                  RRCommandStream command_stream) {
-    RR_REQUIRE(context && scene_buffer && rays && hits && scratch && stats && command_stream, "rrxCmdIntersectWithStats");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(scene_buffer) && LivePtr(rays) && LivePtr(hits) && LivePtr(scratch) &&
+                   LivePtr(stats) && LiveStream(command_stream),
+               "rrxCmdIntersectWithStats");
     return Guarded("rrxCmdIntersectWithStats", [&]() -> RRError {
         Context* c = AsCtx(context);
         DeviceGuard g(c->device);
         void* accel = Addr(scene_buffer);
-        rr::AccelKind kind;
-        const RRError e = KindOf(c, accel, kind);
-        if (e != RR_SUCCESS) return e;
+        const rr::AccelKind kind = KindOf(c, accel);
         const void* r = Addr(rays);
         void* h = Addr(hits);
         void* sc = Addr(scratch);
@@ -985,7 +1111,7 @@ RRError rrxCmdIntersectWithStats(RRContext context, RRDevicePtr scene_buffer, RR
 
 RRError rrxGetIntersectBatchMemoryRequirements(RRContext context, uint32_t request_count, uint64_t total_ray_count,
                                                uint32_t max_request_rays, size_t* scratch_size) {
-    RR_REQUIRE(context && scratch_size, "rrxGetIntersectBatchMemoryRequirements");
+    RR_REQUIRE(LiveCtx(context) && scratch_size, "rrxGetIntersectBatchMemoryRequirements");
     RR_REQUIRE(total_ray_count < (uint64_t(1) << 37), "rrxGetIntersectBatchMemoryRequirements");
     return Guarded("rrxGetIntersectBatchMemoryRequirements", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
@@ -1000,11 +1126,12 @@ RRError rrxGetIntersectBatchMemoryRequirements(RRContext context, uint32_t reque
 RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* requests, uint32_t request_count,
                              RRDevicePtr scratch, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrxCmdIntersectBatch");
-    RR_REQUIRE(context && scratch && command_stream, "rrxCmdIntersectBatch");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(scratch) && LiveStream(command_stream), "rrxCmdIntersectBatch");
     RR_REQUIRE(request_count == 0 || requests, "rrxCmdIntersectBatch");
     for (uint32_t i = 0; i < request_count; ++i) {
         const RRxIntersectRequest& q = requests[i];
-        RR_REQUIRE(q.scene_buffer && (q.ray_count == 0 || (q.rays && q.hits)), "rrxCmdIntersectBatch");
+        RR_REQUIRE(LivePtr(q.scene_buffer) && (q.ray_count == 0 || (LivePtr(q.rays) && LivePtr(q.hits))),
+                   "rrxCmdIntersectBatch");
         RR_REQUIRE(q.query == RR_INTERSECT_QUERY_CLOSEST || q.query == RR_INTERSECT_QUERY_ANY, "rrxCmdIntersectBatch");
         RR_REQUIRE(q.query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT ||
                        q.query_output == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
@@ -1019,6 +1146,7 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
         // run one launch each after it.
         struct Single {
             void* accel;
+            rr::AccelKind kind;
             rr::QueryKind q;
             rr::OutputKind o;
             const void* rays;
@@ -1032,13 +1160,11 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
             const RRxIntersectRequest& q = requests[i];
             if (q.ray_count == 0) continue;
             void* accel = Addr(q.scene_buffer);
-            rr::AccelKind kind;
-            const RRError e = KindOf(c, accel, kind);
-            if (e != RR_SUCCESS) return e;
+            const rr::AccelKind kind = KindOf(c, accel);
             const auto qk = q.query == RR_INTERSECT_QUERY_ANY ? rr::kQueryAny : rr::kQueryClosest;
             const auto ok = static_cast<rr::OutputKind>(q.query_output);
-            if (kind == rr::kAccelTlas) {
-                singles.push_back({accel, qk, ok, Addr(q.rays), q.ray_count, Addr(q.hits)});
+            if (kind != rr::kAccelBlas) {
+                singles.push_back({accel, kind, qk, ok, Addr(q.rays), q.ray_count, Addr(q.hits)});
                 continue;
             }
             rr::BatchRequest b{};
@@ -1067,7 +1193,7 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
                 rr::TraceBatch(dev_table, n_req, n_chunks, sc, s);
             }
             for (const Single& x : singles)
-                rr::Trace(x.accel, rr::kAccelTlas, x.q, x.o, x.rays, x.n, nullptr, x.hits, sc, s);
+                rr::Trace(x.accel, x.kind, x.q, x.o, x.rays, x.n, nullptr, x.hits, sc, s);
         };
         Record(cs, cmd);
         KeepStaging(c, cs, host);
@@ -1119,6 +1245,7 @@ static RRError PlanBatch(const RRGeometryBuildInput* inputs, uint32_t count, boo
 RRError rrxGetGeometryBatchBuildMemoryRequirements(RRContext context, const RRGeometryBuildInput* build_inputs,
                                                    uint32_t count, const RRBuildOptions* build_options,
                                                    size_t* scratch_size) {
+    // A size query: reads nothing from the context.
     RR_REQUIRE(context && scratch_size && (count == 0 || build_inputs), "rrxGetGeometryBatchBuildMemoryRequirements");
     BatchPlan plan;
     const RRError e = PlanBatch(build_inputs, count, WantsRestructure(build_options), false, plan,
@@ -1133,11 +1260,11 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
                                  const RRBuildOptions* build_options, RRDevicePtr temporary_buffer,
                                  const RRDevicePtr* geometry_buffers, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrxCmdBuildGeometryBatch");
-    RR_REQUIRE(context && command_stream && temporary_buffer, "rrxCmdBuildGeometryBatch");
+    RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && LivePtr(temporary_buffer), "rrxCmdBuildGeometryBatch");
     RR_REQUIRE(count == 0 || (build_inputs && geometry_buffers), "rrxCmdBuildGeometryBatch");
     RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
                "rrxCmdBuildGeometryBatch");
-    for (uint32_t i = 0; i < count; ++i) RR_REQUIRE(geometry_buffers[i], "rrxCmdBuildGeometryBatch");
+    for (uint32_t i = 0; i < count; ++i) RR_REQUIRE(LivePtr(geometry_buffers[i]), "rrxCmdBuildGeometryBatch");
     return Guarded("rrxCmdBuildGeometryBatch", [&]() -> RRError {
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
@@ -1150,10 +1277,8 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
         std::vector<void*> results(count);
         for (uint32_t i = 0; i < count; ++i) results[i] = Addr(geometry_buffers[i]);
         const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
-        if (!update) {
-            std::lock_guard<std::mutex> lk(c->mu);
-            for (void* r : results) c->kinds[reinterpret_cast<uintptr_t>(r)] = rr::kAccelBlas;
-        }
+        if (!update)
+            for (uint32_t i = 0; i < count; ++i) RecordAccel(c, results[i], rr::kAccelBlas, plan.meshes[i].tri_count);
         // The mesh table, packed now into pinned memory that the recorded
         // upload reads at execution time.
         const uint32_t n_small = (uint32_t)plan.small.size();
@@ -1226,6 +1351,9 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             return rr::SetSortDigitBits((int)value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
         case RRX_TUNING_TREELET_LEAVES:
             return rr::SetTreeletLeaves((int)value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_SORT_SPIN_LIMIT:
+            rr::SetSortSpinLimit(value);
+            return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1239,6 +1367,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_TRACE_PLAIN: *value = rr::TraceMode() == rr::kTraceModePlain ? 1u : 0u; return RR_SUCCESS;
         case RRX_TUNING_SORT_DIGIT_BITS: *value = (uint32_t)rr::SortDigitBits(); return RR_SUCCESS;
         case RRX_TUNING_TREELET_LEAVES: *value = (uint32_t)rr::TreeletLeaves(); return RR_SUCCESS;
+        case RRX_TUNING_SORT_SPIN_LIMIT: *value = rr::SortSpinLimit(); return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1246,7 +1375,8 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
 RRError rrxWriteAnalyzerDump(RRContext context, RRDevicePtr geometry, RRDevicePtr rays, uint32_t width, uint32_t height,
                              const char* bvh_path, const char* rays_path, const char* config_path) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrxWriteAnalyzerDump");
-    RR_REQUIRE(context && geometry && bvh_path && rays_path && config_path, "rrxWriteAnalyzerDump");
+    RR_REQUIRE(LiveCtx(context) && LivePtr(geometry) && OptPtr(rays) && bvh_path && rays_path && config_path,
+               "rrxWriteAnalyzerDump");
     RR_REQUIRE(!rays || (uint64_t)width * height > 0, "rrxWriteAnalyzerDump");
     return Guarded("rrxWriteAnalyzerDump", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
@@ -1283,6 +1413,17 @@ RRError rrxWriteAnalyzerDump(RRContext context, RRDevicePtr geometry, RRDevicePt
     });
 }
 
+RRError rrxGetDeviceErrors(RRContext context, uint32_t* errors) {
+    RR_REQUIRE(LiveCtx(context) && errors, "rrxGetDeviceErrors");
+    return Guarded("rrxGetDeviceErrors", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        *errors = rr::TakeDeviceErrors();
+        return RR_SUCCESS;
+    });
+}
+
 }  // extern "C"
 
 int rr::ContextDevice(RRContext context) { return AsCtx(context)->device; }
+bool rr::IsLiveContext(RRContext context) { return LiveCtx(context); }

@@ -8,4 +8,7 @@ namespace rr {
 // HIP device ordinal the context was created on.
 int ContextDevice(RRContext context);
 
+// True while `context` is a context the library created and has not destroyed.
+bool IsLiveContext(RRContext context);
+
 }  // namespace rr

@@ -39,7 +39,9 @@ static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 bytes");
 // from scene buffer to BLAS list (vlk/intersector.cpp:263,289-291); here the
 // buffer describes itself, so traversal needs no host state and a TLAS
 // carries its own table of BLAS addresses (no 2048-instance cap).
-enum AccelKind : uint32_t { kAccelBlas = 1, kAccelTlas = 2 };
+// kAccelUnknown is never stored: it asks traversal to read the kind from the
+// header on the device (a buffer the host has not seen built).
+enum AccelKind : uint32_t { kAccelUnknown = 0, kAccelBlas = 1, kAccelTlas = 2 };
 constexpr uint32_t kAccelMagic = 0x53415252u;  // "RRAS"
 
 struct AccelHeader {

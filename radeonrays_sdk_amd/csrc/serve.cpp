@@ -28,6 +28,7 @@
 #include <mutex>
 #include <queue>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/radeonrays_ext.h"
@@ -265,6 +266,15 @@ struct Server {
 
 Server* AsServer(RRxServer s) { return reinterpret_cast<Server*>(s); }
 
+// Live servers: a handle that was never created or is already destroyed is
+// rejected with RR_ERROR_INVALID_PARAMETER rather than dereferenced.
+std::mutex g_servers_mu;
+std::unordered_set<const void*> g_servers;
+bool LiveServer(RRxServer s) {
+    std::lock_guard<std::mutex> lk(g_servers_mu);
+    return s && g_servers.count(s) != 0;
+}
+
 void DestroyResources(Server* sv) {
     (void)hipSetDevice(sv->device);
     for (Slot& s : sv->slots) {
@@ -284,7 +294,7 @@ void DestroyResources(Server* sv) {
 extern "C" {
 
 RRError rrxCreateServer(RRContext context, const RRxServerOptions* options, RRxServer* server) {
-    if (!context || !options || !server) return RR_ERROR_INVALID_PARAMETER;
+    if (!rr::IsLiveContext(context) || !options || !server) return RR_ERROR_INVALID_PARAMETER;
     const RRxServerOptions& o = *options;
     if (o.mode != RRX_SERVE_BATCHED && o.mode != RRX_SERVE_PLAIN) return RR_ERROR_INVALID_PARAMETER;
     if (o.max_batch_rays == 0 || o.memory_budget == 0) return RR_ERROR_INVALID_PARAMETER;
@@ -327,12 +337,19 @@ RRError rrxCreateServer(RRContext context, const RRxServerOptions* options, RRxS
         return e;
     }
     sv->worker = std::thread([sv] { sv->Run(); });
+    {
+        std::lock_guard<std::mutex> lk(g_servers_mu);
+        g_servers.insert(sv);
+    }
     *server = reinterpret_cast<RRxServer>(sv);
     return RR_SUCCESS;
 }
 
 RRError rrxDestroyServer(RRxServer server) {
-    if (!server) return RR_ERROR_INVALID_PARAMETER;
+    {
+        std::lock_guard<std::mutex> lk(g_servers_mu);
+        if (!server || g_servers.erase(server) == 0) return RR_ERROR_INVALID_PARAMETER;
+    }
     Server* sv = AsServer(server);
     {
         std::lock_guard<std::mutex> lk(sv->mu);
@@ -345,11 +362,11 @@ RRError rrxDestroyServer(RRxServer server) {
     return RR_SUCCESS;
 }
 
-uint64_t rrxServerNow(RRxServer server) { return server ? AsServer(server)->Now() : 0; }
+uint64_t rrxServerNow(RRxServer server) { return LiveServer(server) ? AsServer(server)->Now() : 0; }
 
 RRError rrxServerSubmit(RRxServer server, const RRxIntersectRequest* requests, const uint64_t* release_ns, uint32_t count,
                         uint64_t* first_ticket) {
-    if (!server || !first_ticket || (count && !requests)) return RR_ERROR_INVALID_PARAMETER;
+    if (!LiveServer(server) || !first_ticket || (count && !requests)) return RR_ERROR_INVALID_PARAMETER;
     Server* sv = AsServer(server);
     for (uint32_t i = 0; i < count; ++i) {
         const RRxIntersectRequest& q = requests[i];
@@ -385,7 +402,7 @@ RRError rrxServerSubmit(RRxServer server, const RRxIntersectRequest* requests, c
 }
 
 RRError rrxServerDrain(RRxServer server) {
-    if (!server) return RR_ERROR_INVALID_PARAMETER;
+    if (!LiveServer(server)) return RR_ERROR_INVALID_PARAMETER;
     Server* sv = AsServer(server);
     std::unique_lock<std::mutex> lk(sv->mu);
     sv->drained.wait(lk, [sv] { return sv->completed == sv->submitted || sv->error != RR_SUCCESS; });
@@ -393,7 +410,7 @@ RRError rrxServerDrain(RRxServer server) {
 }
 
 RRError rrxServerGetTimings(RRxServer server, uint64_t first_ticket, uint32_t count, RRxServedTiming* timings) {
-    if (!server || (count && !timings)) return RR_ERROR_INVALID_PARAMETER;
+    if (!LiveServer(server) || (count && !timings)) return RR_ERROR_INVALID_PARAMETER;
     Server* sv = AsServer(server);
     std::lock_guard<std::mutex> lk(sv->mu);
     if (first_ticket + count > sv->submitted) return RR_ERROR_INVALID_PARAMETER;
@@ -406,7 +423,7 @@ RRError rrxServerGetTimings(RRxServer server, uint64_t first_ticket, uint32_t co
 }
 
 RRError rrxServerGetPeakAdmitted(RRxServer server, uint64_t* bytes) {
-    if (!server || !bytes) return RR_ERROR_INVALID_PARAMETER;
+    if (!LiveServer(server) || !bytes) return RR_ERROR_INVALID_PARAMETER;
     Server* sv = AsServer(server);
     std::lock_guard<std::mutex> lk(sv->mu);
     *bytes = sv->peak_bytes;

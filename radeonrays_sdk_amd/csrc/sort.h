@@ -46,6 +46,14 @@ SortShape ChooseSortShape(uint32_t n);
 int SortDigitBits();
 bool SetSortDigitBits(int bits);
 
+// Bound of a tile's look-back wait, in polls that find a predecessor's
+// word unpublished (process-wide; default 2^22, far beyond any real wait).
+// Past it the tile gives up and sets the look-back error word; 0 gives up
+// at the first such poll (a test hook for the failure report).
+constexpr uint32_t kSortDefaultSpinLimit = 1u << 22;
+uint32_t SortSpinLimit();
+void SetSortSpinLimit(uint32_t polls);
+
 // Scratch needed to sort n pairs: histograms, tile tickets and look-back
 // status words.  Everything in it is (re)initialised by the sort itself.
 size_t SortScratchBytes(uint32_t n);

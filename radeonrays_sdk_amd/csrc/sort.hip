@@ -11,9 +11,12 @@
 //     and resolves its global offsets by decoupled look-back, then scatters
 //     through LDS so that each digit's run is written contiguously.
 // Tickets make every tile's predecessors already running, so look-back only
-// waits on work that is guaranteed to publish; the spin is bounded anyway.
+// waits on work that is guaranteed to publish; the spin is bounded anyway,
+// and a tile that gives up sets the look-back error word (device_errors.h):
+// its offsets, and so the sort order, are then wrong, and the API reports it.
 #include <hip/hip_runtime.h>
 
+#include "device_errors.h"
 #include "rr_device.h"
 #include "sort.h"
 
@@ -30,7 +33,6 @@ constexpr int kBlock = 256;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagInclusive = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
-constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kLookback = 4;
 
 __device__ __forceinline__ uint32_t AtomicLoadAgent(const uint32_t* p) {
@@ -132,7 +134,7 @@ template <int BLOCK, int ITEMS, int BITS = kSortRadixBits>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void SortPassKernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, const uint32_t* __restrict__ hist, uint32_t* tickets,
-    uint32_t* status, uint32_t* next_status) {
+    uint32_t* status, uint32_t* next_status, uint32_t spin_limit, uint32_t* err) {
     constexpr int kRadix = 1 << BITS;
     static_assert(BLOCK >= kRadix && BLOCK % 64 == 0, "one thread per digit");
     constexpr int kWaves = BLOCK / 64;
@@ -213,7 +215,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 }
                 p -= used;
                 if (blocked) {
-                    if (++spins > kSpinLimit) break;
+                    if (++spins > spin_limit) {
+                        if (err) StG(err + kErrLookbackTimeout, 1u);
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
@@ -303,6 +308,10 @@ static bool KnownShape(int block, int items, int bits) {
     return bits == 8 && ((block == 1024 && (items == 16 || items == 8)) || (block == 512 && (items == 16 || items == 8)) ||
                          (block == 256 && (items == 16 || items == 8 || items == 4)));
 }
+
+static std::atomic<uint32_t> g_spin_limit{kSortDefaultSpinLimit};
+uint32_t SortSpinLimit() { return g_spin_limit.load(std::memory_order_relaxed); }
+void SetSortSpinLimit(uint32_t polls) { g_spin_limit.store(polls, std::memory_order_relaxed); }
 
 static std::atomic<int> g_digit_bits{8};
 int SortDigitBits() { return g_digit_bits.load(std::memory_order_relaxed); }
@@ -402,13 +411,16 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
             SortHistogramKernel<8><<<blocks, kBlock, 0, stream>>>(keys, n, passes, sc.hist, sc.status, status0);
     }
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
+    const uint32_t spin = SortSpinLimit();
+    uint32_t* err = DeviceErrorWords();
     for (int p = 0; p < passes; ++p) {
         uint32_t* st = sc.status + (size_t)p * tiles * radix;
         uint32_t* nst = p + 1 < passes ? st + tiles * radix : nullptr;
         const uint32_t* h = sc.hist + p * radix;
         const uint32_t shift = (uint32_t)(p * shape.bits);
 #define RR_SORT_PASS(B, I, D) \
-    SortPassKernel<B, I, D><<<(uint32_t)tiles, B, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst)
+    SortPassKernel<B, I, D><<<(uint32_t)tiles, B, 0, stream>>>(ki, vi, ko, vo, n, shift, h, sc.tickets + p, st, nst, \
+                                                              spin, err)
         if (shape.bits == kSortWideBits) {
             if (shape.items == 8) RR_SORT_PASS(1024, 8, kSortWideBits);
             else RR_SORT_PASS(1024, 4, kSortWideBits);

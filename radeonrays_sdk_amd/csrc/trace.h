@@ -13,7 +13,8 @@ namespace rr {
 // spilled to the lane's kSpillEntries slots of trace scratch (the reference
 // keeps 16 in LDS and 64 per ray in scratch, vlk/kernels/isect.comp:26-27,
 // geometry_trace.cpp:169).  The capacity is the same for every LDS depth, so
-// tuning never changes which pushes a too-deep tree drops.
+// tuning never changes which rays a too-deep tree overflows; such a ray
+// ends early and sets the stack-overflow error word (device_errors.h).
 constexpr int kStackCapacity = 80;
 constexpr int kLdsStackEntries = 16;
 constexpr int kMinLdsStackEntries = 8;
@@ -49,7 +50,10 @@ enum QueryKind : int { kQueryClosest = 0, kQueryAny = 1 };
 enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };
 
 // Trace ray_count rays (or min(ray_count, *indirect) when indirect is not
-// null) against the BLAS or TLAS at `accel`.
+// null) against the BLAS or TLAS at `accel`.  With kind kAccelUnknown both
+// levels' kernels are launched and the buffer's header selects the one that
+// runs, so the host never reads the buffer.  Failures are reported through
+// the device error words (device_errors.h).
 void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output, const void* rays, uint32_t ray_count,
            const uint32_t* indirect, void* hits, void* scratch, hipStream_t stream);
 

@@ -11,11 +11,18 @@
 //     ordered by (t, instance, primitive), so the result does not depend on
 //     the BVH's shape;
 //   * slab tests are conservative (widened by a few ulp);
-//   * the stack keeps 16 entries per lane in LDS as a ring and spills the
-//     oldest entry to scratch one at a time;
-//   * a miss writes a complete record (inst_id = prim_id = ~0).
+//   * the stack keeps 8, 16 or 32 entries per lane in LDS as a ring (8 by
+//     default) and spills the oldest entry to scratch one at a time; a ray
+//     that needs more than kStackCapacity entries ends and sets the
+//     stack-overflow error word instead of dropping entries silently;
+//   * a miss writes a complete record (inst_id = prim_id = ~0);
+//   * every kernel checks the buffer's header first: a buffer this library
+//     did not write sets the bad-buffer error word, and a kernel for the
+//     other level returns at once (rrCmdIntersect on a buffer the host has
+//     not seen built launches both levels and lets the header decide).
 #include <hip/hip_runtime.h>
 
+#include "device_errors.h"
 #include "rr_device.h"
 #include "trace.h"
 
@@ -48,6 +55,16 @@ __device__ __forceinline__ RayIn LoadRay(const float4* __restrict__ rays, uint32
     return r;
 }
 
+// True if the buffer's header names kind `want`.  A buffer this library did
+// not write (wrong magic or kind) sets the bad-buffer error word and gives
+// false, as does a valid buffer of the other kind (silently).
+__device__ __forceinline__ bool HeaderIs(const AccelHeader* h, uint32_t want, uint32_t* err) {
+    const uint32_t magic = h->magic, kind = h->kind;
+    if (magic == kAccelMagic && (kind == kAccelBlas || kind == kAccelTlas)) return kind == want;
+    if (err && blockIdx.x == 0 && threadIdx.x == 0) StG(err + kErrBadAccel, 1u);
+    return false;
+}
+
 // LDS ring of DEPTH entries + scratch spill stack for one lane.
 template <int DEPTH = kLdsStackEntries>
 struct LaneStack {
@@ -55,6 +72,7 @@ struct LaneStack {
     static_assert(DEPTH >= kMinLdsStackEntries && kStackCapacity - DEPTH <= kSpillEntries, "spill slots per lane");
     uint32_t (*lds)[kBlock];
     uint32_t* spill;  // this block's spill slots (uniform): lane t's are [t * kSpillEntries, (t + 1) * kSpillEntries)
+    uint32_t* err;    // device error words (may be null)
     int sp;     // entries on the stack
     int gbase;  // entries [0, gbase) live in scratch, [gbase, sp) in LDS
 
@@ -63,7 +81,15 @@ struct LaneStack {
 
     __device__ __forceinline__ bool empty() const { return sp == 0; }
     __device__ __forceinline__ void push(uint32_t v) {
-        if (sp == kStackCapacity) return;  // full: drop (tree deeper than the stack)
+        if (sp == kStackCapacity) {
+            // Tree deeper than the stack: report it and end the ray (its
+            // deferred nodes, and in a scene the instance sentinel, are
+            // discarded, so the lane finishes the current descent and stops).
+            if (err) StG(err + kErrStackOverflow, 1u);
+            sp = 0;
+            gbase = 0;
+            return;
+        }
         if (sp - gbase == DEPTH) {
             StG(slot(gbase), lds[gbase & (DEPTH - 1)][threadIdx.x]);
             ++gbase;
@@ -88,7 +114,7 @@ template <bool ANY, int OUT, bool TWO_LEVEL, bool STATS>
 __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict__ accel, const float4* __restrict__ rays,
                                                       uint32_t ray_count, const uint32_t* __restrict__ indirect,
                                                       void* __restrict__ hits, uint32_t* __restrict__ scratch,
-                                                      uint32_t* __restrict__ stats) {
+                                                      uint32_t* __restrict__ stats, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t n = ray_count;
@@ -96,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     if (i >= n) return;
 
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
+    if (!HeaderIs(hdr, TWO_LEVEL ? kAccelTlas : kAccelBlas, err)) return;
     const uint32_t top_count = hdr->node_count;
     const BvhNode* top_nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
     bool boxes = !TWO_LEVEL && (hdr->flags & kAccelFlagAabbs) != 0;  // leaf kind of the level traversed
@@ -117,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void TraceKernel(const uint8_t* __restrict_
     float limit = WidenLimit(tmax);
     uint32_t n_internal_visits = 0, n_leaf_visits = 0, n_box_tests = 0, n_tri_tests = 0;
 
-    LaneStack<> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries, 0, 0};
+    LaneStack<> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
     const BvhNode* nodes = top_nodes;
     uint32_t node_count = top_count;  // of the level being traversed (bounds every node index)
     uint32_t inst = kInvalid;  // kInvalid while in the top level
@@ -264,11 +291,12 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                                                                 const float4* __restrict__ rays, uint32_t ray_count,
                                                                 const uint32_t* __restrict__ indirect,
                                                                 void* __restrict__ hits, uint32_t* __restrict__ scratch,
-                                                                uint32_t min_busy, uint32_t fetch) {
+                                                                uint32_t min_busy, uint32_t fetch, uint32_t* err) {
     __shared__ uint32_t s_stack[DEPTH][kBlock];
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
+    if (!HeaderIs(hdr, kAccelBlas, err)) return;
     const uint32_t total = hdr->node_count;
     const uint32_t n_int = hdr->prim_count ? hdr->prim_count - 1 : 0;
     const BvhNode* nodes = reinterpret_cast<const BvhNode*>(accel + hdr->nodes_offset);
@@ -289,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
     LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries,
-                        0, 0};
+                        err, 0, 0};
 
     for (;;) {
         // Refill idle lanes.
@@ -428,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                                                                   const uint32_t* __restrict__ indirect,
                                                                   void* __restrict__ hits,
                                                                   uint32_t* __restrict__ scratch, uint32_t min_busy,
-                                                                  uint32_t fetch) {
+                                                                  uint32_t fetch, uint32_t* err) {
     __shared__ uint32_t s_stack[DEPTH][kBlock];
     // Each lane's world ray: origin xyz, direction xyz and reciprocal
     // direction, restored without divisions when the lane leaves an
@@ -439,6 +467,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
+    if (!HeaderIs(hdr, kAccelTlas, err)) return;
     const uint32_t top_total = hdr->node_count;
     const uint32_t top_prims = hdr->prim_count;
     const uint32_t top_int = top_prims ? top_prims - 1 : 0;
@@ -467,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
     uint32_t cur_int = top_int, cur_total = top_total;
     bool boxes = false;
     LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries,
-                        0, 0};
+                        err, 0, 0};
 
     for (;;) {
         // Refill idle lanes.
@@ -651,15 +680,16 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
 template <bool ANY, bool TWO_LEVEL, bool STATS>
 void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, const float4* rays, uint32_t count,
                const uint32_t* indirect, void* hits, uint32_t* scratch, uint32_t* stats) {
+    uint32_t* err = DeviceErrorWords();
     switch (out) {
         case kOutFullHit:
-            TraceKernel<ANY, kOutFullHit, TWO_LEVEL, STATS><<<g, kBlock, 0, s>>>(accel, rays, count, indirect, hits, scratch, stats);
+            TraceKernel<ANY, kOutFullHit, TWO_LEVEL, STATS><<<g, kBlock, 0, s>>>(accel, rays, count, indirect, hits, scratch, stats, err);
             break;
         case kOutInstanceId:
-            TraceKernel<ANY, kOutInstanceId, TWO_LEVEL, STATS><<<g, kBlock, 0, s>>>(accel, rays, count, indirect, hits, scratch, stats);
+            TraceKernel<ANY, kOutInstanceId, TWO_LEVEL, STATS><<<g, kBlock, 0, s>>>(accel, rays, count, indirect, hits, scratch, stats, err);
             break;
         default:
-            TraceKernel<ANY, kOutFullHitT, TWO_LEVEL, STATS><<<g, kBlock, 0, s>>>(accel, rays, count, indirect, hits, scratch, stats);
+            TraceKernel<ANY, kOutFullHitT, TWO_LEVEL, STATS><<<g, kBlock, 0, s>>>(accel, rays, count, indirect, hits, scratch, stats, err);
             break;
     }
 }
@@ -671,7 +701,7 @@ void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, cons
 // BVH, query and output, so lanes of a wave may serve different requests.
 __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* __restrict__ table, uint32_t requests,
                                                            uint32_t total_chunks, uint32_t* __restrict__ scratch,
-                                                           uint32_t min_busy) {
+                                                           uint32_t min_busy, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
     uint32_t* counter = scratch;
     const uint32_t lane = __lane_id();
@@ -690,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;  // best_t is t_max until a hit; culling limit WidenLimit(best_t)
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack<> st{s_stack, scratch + spill_base + (size_t)blockIdx.x * kBlock * kSpillEntries, 0, 0};
+    LaneStack<> st{s_stack, scratch + spill_base + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
 
     for (;;) {
         const uint64_t idle = __ballot(rid == kInvalid);
@@ -723,8 +753,13 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 const uint8_t* accel = reinterpret_cast<const uint8_t*>(rq.accel);
                 const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
                 total = LdG(&hdr->node_count);
+                if (LdG(&hdr->magic) != kAccelMagic || LdG(&hdr->kind) != kAccelBlas) {
+                    // Not a geometry this library built: report it, answer misses.
+                    if (err) StG(err + kErrBadAccel, 1u);
+                    total = 0;
+                }
                 const uint32_t prims = LdG(&hdr->prim_count);
-                n_int = prims ? prims - 1 : 0;
+                n_int = (prims && total) ? prims - 1 : 0;
                 nodes = reinterpret_cast<const BvhNode*>(accel + LdG(&hdr->nodes_offset));
                 boxes = (LdG(&hdr->flags) & kAccelFlagAabbs) != 0;
                 const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
@@ -926,8 +961,9 @@ uint32_t FetchFor(uint32_t n, uint32_t blocks) {
 template <bool ANY, bool TWO, int OUT, int DEPTH>
 void LaunchPersistentKernel(uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
                             const uint32_t* ind, void* hits, uint32_t* sc, uint32_t mb, uint32_t f) {
-    if (TWO) TracePersistent2LKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
-    else TracePersistentKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f);
+    uint32_t* err = DeviceErrorWords();
+    if (TWO) TracePersistent2LKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f, err);
+    else TracePersistentKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f, err);
 }
 
 template <bool ANY, bool TWO, int DEPTH>
@@ -944,7 +980,6 @@ void LaunchPersistentDepth(OutputKind out, uint32_t blocks, hipStream_t s, const
 template <bool ANY, bool TWO>
 void LaunchPersistent(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
                       const uint32_t* ind, void* hits, uint32_t* sc) {
-    QueueInitKernel<<<1, 64, 0, s>>>(sc, n, ind);
     switch (LdsStackDepth()) {
         case 8: LaunchPersistentDepth<ANY, TWO, 8>(out, blocks, s, a, r, n, ind, hits, sc); break;
         case 32: LaunchPersistentDepth<ANY, TWO, 32>(out, blocks, s, a, r, n, ind, hits, sc); break;
@@ -960,27 +995,34 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
     const uint8_t* a = static_cast<const uint8_t*>(accel);
     const float4* r = static_cast<const float4*>(rays);
     uint32_t* sc = static_cast<uint32_t*>(scratch);
-    const bool two = kind == kAccelTlas;
+    // kind 0 (not known on the host): launch both levels; each kernel reads
+    // the header and the one for the other level returns at once.
+    const bool single = kind != kAccelTlas, two = kind != kAccelBlas;
     if (TraceMode() != kTraceModePlain) {
         // At most ceil(ray_count / kBlock) blocks: TraceScratchBytes holds
-        // spill slots for that many lanes.
-        const uint32_t blocks = std::min(PersistentBlocks(two), (ray_count + kBlock - 1) / kBlock);
-        if (two) {
-            if (query == kQueryAny) LaunchPersistent<true, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
-            else LaunchPersistent<false, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
-        } else {
+        // spill slots for that many lanes.  The queues are set up once: a
+        // kernel of the wrong level returns before touching them.
+        QueueInitKernel<<<1, 64, 0, s>>>(sc, ray_count, indirect);
+        const uint32_t cap = (ray_count + kBlock - 1) / kBlock;
+        if (single) {
+            const uint32_t blocks = std::min(PersistentBlocks(false), cap);
             if (query == kQueryAny) LaunchPersistent<true, false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
             else LaunchPersistent<false, false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        }
+        if (two) {
+            const uint32_t blocks = std::min(PersistentBlocks(true), cap);
+            if (query == kQueryAny) LaunchPersistent<true, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+            else LaunchPersistent<false, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
         }
         return;
     }
     const dim3 g((ray_count + kBlock - 1) / kBlock);
     if (query == kQueryAny) {
+        if (single) LaunchOut<true, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
         if (two) LaunchOut<true, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
-        else LaunchOut<true, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
     } else {
+        if (single) LaunchOut<false, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
         if (two) LaunchOut<false, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
-        else LaunchOut<false, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
     }
 }
 
@@ -991,13 +1033,14 @@ void TraceWithStats(const void* accel, AccelKind kind, QueryKind query, const vo
     const uint8_t* a = static_cast<const uint8_t*>(accel);
     const float4* r = static_cast<const float4*>(rays);
     uint32_t* sc = static_cast<uint32_t*>(scratch);
-    const bool two = kind == kAccelTlas;
+    const bool single = kind != kAccelTlas, two = kind != kAccelBlas;
+    uint32_t* err = DeviceErrorWords();
     if (query == kQueryAny) {
-        if (two) TraceKernel<true, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats);
-        else TraceKernel<true, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats);
+        if (single) TraceKernel<true, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
+        if (two) TraceKernel<true, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
     } else {
-        if (two) TraceKernel<false, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats);
-        else TraceKernel<false, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats);
+        if (single) TraceKernel<false, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
+        if (two) TraceKernel<false, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
     }
 }
 
@@ -1015,7 +1058,7 @@ void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chu
     uint32_t* sc = static_cast<uint32_t*>(scratch);
     const uint32_t blocks = std::min<uint64_t>(BatchBlocks(), ((uint64_t)total_chunks * kBatchChunk + kBlock - 1) / kBlock);
     BatchInitKernel<<<1, 64, 0, s>>>(sc);
-    TraceBatchKernel<<<blocks, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes());
+    TraceBatchKernel<<<blocks, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes(), DeviceErrorWords());
 }
 
 }  // namespace rr

@@ -56,6 +56,11 @@ RRX_TUNING_TRACE_LDS_STACK = 2
 RRX_TUNING_TRACE_PLAIN = 3
 RRX_TUNING_SORT_DIGIT_BITS = 4
 RRX_TUNING_TREELET_LEAVES = 5
+RRX_TUNING_SORT_SPIN_LIMIT = 6
+
+RRX_DEVICE_ERROR_STACK_OVERFLOW = 1 << 0
+RRX_DEVICE_ERROR_SORT_LOOKBACK = 1 << 1
+RRX_DEVICE_ERROR_INVALID_BUFFER = 1 << 2
 
 RRX_SERVE_BATCHED = 0
 RRX_SERVE_PLAIN = 1
@@ -161,6 +166,7 @@ _SIGNATURES = {
     "rrxCmdUpdateScene": [c_void_p, POINTER(RRSceneBuildInput), c_void_p, c_void_p, c_void_p],
     "rrxSetTuning": [c_int, c_uint32],
     "rrxGetTuning": [c_int, POINTER(c_uint32)],
+    "rrxGetDeviceErrors": [c_void_p, POINTER(c_uint32)],
     "rrxGetGeometryBatchBuildMemoryRequirements": [c_void_p, c_void_p, c_uint32, POINTER(RRBuildOptions),
                                                    POINTER(c_size_t)],
     "rrxCmdBuildGeometryBatch": [c_void_p, c_int, c_void_p, c_uint32, POINTER(RRBuildOptions), c_void_p, c_void_p,

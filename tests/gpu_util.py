@@ -53,3 +53,22 @@ def hits_equal(gpu: dict, ref: np.ndarray, fields=("prim_id", "inst_id", "t", "u
             k = bad[0]
             return False, f"{f}: {bad.size} rays differ; first ray {k}: gpu {gpu[f][k]} oracle {ref[f][k]}"
     return True, ""
+
+
+def tlas_tables_equal(gpu_buf: np.ndarray, ref_buf: np.ndarray) -> tuple[bool, str]:
+    """Bit-exact comparison of a TLAS's per-instance transforms (inverse at
+    2i, forward at 2i+1, 48 bytes each) and BLAS address table (8 bytes per
+    instance), located through each buffer's own header."""
+    n = int(ref_buf[8:12].view(np.uint32)[0])
+    if int(gpu_buf[8:12].view(np.uint32)[0]) != n:
+        return False, "instance counts differ"
+    for name, field, per in (("transforms", slice(24, 32), 96), ("BLAS table", slice(32, 40), 8)):
+        go, ro = int(gpu_buf[field].view(np.uint64)[0]), int(ref_buf[field].view(np.uint64)[0])
+        if go != ro:
+            return False, f"{name} offset {go} vs {ro}"
+        g, r = gpu_buf[go:go + per * n], ref_buf[ro:ro + per * n]
+        bad = np.nonzero(g != r)[0]
+        if bad.size:
+            inst = bad[0] // per
+            return False, f"{name}: {np.unique(bad // per).size} instances differ; first {inst}"
+    return True, ""

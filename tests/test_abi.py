@@ -12,7 +12,9 @@ from radeonrays_sdk_amd import rr
 
 ROOT = Path(__file__).resolve().parent.parent
 HEADERS = sorted((ROOT / "include").glob("radeonrays*.h"))
-# A context handle that is never dereferenced by the paths under test.
+# A context handle the library never handed out: size queries accept it (they
+# read nothing from the context), every other entry point must reject it
+# without dereferencing it.
 FAKE_CTX = c_void_p(0x1000)
 
 
@@ -162,8 +164,6 @@ def test_aabb_list_requirements_and_unknown_primitive_type():
     assert req.temporary_update_buffer_size >= 10 * 36
     prim, bad = _mesh_input(10, ptype=7)
     assert lib.rrGetGeometryBuildMemoryRequirements(FAKE_CTX, byref(bad), None, byref(req)) == rr.RR_ERROR_NOT_IMPLEMENTED
-    assert (lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(bad), None, c_void_p(1), c_void_p(1),
-                                   c_void_p(1)) == rr.RR_ERROR_NOT_IMPLEMENTED)
 
 
 def test_log_level_validation():
@@ -193,10 +193,14 @@ def test_memory_requirements(tris):
         assert req.result_buffer_size == 128 + (nodes * 64 + 255) // 256 * 256
         assert req.temporary_update_buffer_size == 0
         assert req.temporary_build_buffer_size >= 16 * tris
-    scene = rr.RRSceneBuildInput(None, 4096)
-    req = rr.RRMemoryRequirements()
-    assert lib.rrGetSceneBuildMemoryRequirements(FAKE_CTX, byref(scene), None, byref(req)) == rr.RR_SUCCESS
-    assert req.result_buffer_size >= 128 + 8191 * 64 + 8192 * 48 + 4096 * 8
+    for inst in (1, 37, 4096):
+        scene = rr.RRSceneBuildInput(None, inst)
+        req = rr.RRMemoryRequirements()
+        assert lib.rrGetSceneBuildMemoryRequirements(FAKE_CTX, byref(scene), None, byref(req)) == rr.RR_SUCCESS
+        assert req.result_buffer_size >= 128 + (2 * inst - 1) * 64 + 2 * inst * 48 + inst * 8
+        # A scene refit stages 64 bytes per instance and a 32-byte world box
+        # per instance after a 256-byte head.
+        assert req.temporary_update_buffer_size >= 96 * inst + 256
     size = c_size_t()
     assert lib.rrGetTraceMemoryRequirements(FAKE_CTX, 1000, byref(size)) == rr.RR_SUCCESS
     assert size.value >= 1000 * 256
@@ -216,3 +220,68 @@ def test_multiple_meshes_per_build_input():
     assert req.result_buffer_size == one.result_buffer_size
     assert req.temporary_build_buffer_size > one.temporary_build_buffer_size
     assert req.temporary_update_buffer_size >= 20 * 36
+
+
+def test_foreign_handles_are_rejected_without_dereference():
+    """Handles the library never handed out (or already released) give
+    RR_ERROR_INVALID_PARAMETER: none of these calls may touch the memory the
+    handle points at (1 and 0x1000 are unmapped addresses)."""
+    lib = rr.load()
+    E = rr.RR_ERROR_INVALID_PARAMETER
+    bogus = c_void_p(1)
+    prim, inp = _mesh_input(10)
+    assert lib.rrDestroyContext(FAKE_CTX) == E
+    assert lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(inp), None, bogus, bogus, bogus) == E
+    prim, bad = _mesh_input(10, ptype=7)
+    assert lib.rrCmdBuildGeometry(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, byref(bad), None, bogus, bogus, bogus) == E
+    scene = rr.RRSceneBuildInput(None, 0)
+    assert lib.rrCmdBuildScene(FAKE_CTX, byref(scene), None, bogus, bogus, bogus) == E
+    assert lib.rrCmdIntersect(FAKE_CTX, bogus, 0, bogus, 1, None, 0, bogus, bogus, bogus) == E
+    out = c_void_p()
+    assert lib.rrAllocateCommandStream(FAKE_CTX, byref(out)) == E
+    assert lib.rrReleaseCommandStream(FAKE_CTX, bogus) == E
+    assert lib.rrSumbitCommandStream(FAKE_CTX, bogus, None, byref(out)) == E
+    assert lib.rrReleaseEvent(FAKE_CTX, bogus) == E
+    assert lib.rrWaitEvent(FAKE_CTX, bogus) == E
+    assert lib.rrReleaseDevicePtr(FAKE_CTX, bogus) == E
+    assert lib.rrReleaseExternalCommandStream(FAKE_CTX, bogus) == E
+    assert lib.rrGetDevicePtrFromHipPtr(FAKE_CTX, bogus, 0, byref(out)) == E
+    assert lib.rrGetCommandStreamFromHipStream(FAKE_CTX, None, byref(out)) == E
+    assert lib.rrAllocateDeviceBuffer(FAKE_CTX, 64, byref(out)) == E
+    assert lib.rrMapDevicePtr(FAKE_CTX, bogus, byref(out)) == E
+    assert lib.rrUnmapDevicePtr(FAKE_CTX, bogus, byref(out)) == E
+    assert lib.rrGetHipAddress(bogus) is None
+    assert lib.rrGetHipStream(bogus) is None
+    assert lib.rrxCmdSortPairs(FAKE_CTX, bogus, bogus, bogus, bogus, 10, 32, bogus, bogus) == E
+    assert lib.rrxCmdIntersectWithStats(FAKE_CTX, bogus, 0, bogus, 1, bogus, bogus, bogus, bogus) == E
+    assert lib.rrxCmdUpdateScene(FAKE_CTX, byref(scene), bogus, bogus, bogus) == E
+    assert lib.rrxCmdBuildGeometryBatch(FAKE_CTX, rr.RR_BUILD_OPERATION_BUILD, None, 0, None, bogus, None, bogus) == E
+    assert lib.rrxWriteAnalyzerDump(FAKE_CTX, bogus, None, 0, 0, b"a", b"b", b"c") == E
+    assert lib.rrxGetIntersectBatchMemoryRequirements(FAKE_CTX, 1, 1, 1, byref(c_size_t())) == E
+    from ctypes import c_uint32
+    assert lib.rrxGetDeviceErrors(FAKE_CTX, byref(c_uint32())) == E
+    opts = rr.RRxServerOptions(rr.RRX_SERVE_BATCHED, 1, 16, 1 << 20, 1 << 30)
+    assert lib.rrxCreateServer(FAKE_CTX, byref(opts), byref(out)) == E
+    assert lib.rrxDestroyServer(bogus) == E
+    assert lib.rrxServerDrain(bogus) == E
+    assert lib.rrxServerNow(bogus) == 0
+    assert lib.rrxServerGetPeakAdmitted(bogus, byref(ctypes.c_uint64())) == E
+
+
+def test_sort_spin_limit_tuning_round_trip():
+    lib = rr.load()
+    from ctypes import c_uint32
+    v = c_uint32()
+    assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_SPIN_LIMIT, byref(v)) == rr.RR_SUCCESS and v.value == 1 << 22
+    assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_SPIN_LIMIT, 0) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_SPIN_LIMIT, byref(v)) == rr.RR_SUCCESS and v.value == 0
+    assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_SPIN_LIMIT, 1 << 22) == rr.RR_SUCCESS
+
+
+def test_only_the_c_abi_is_exported():
+    """The shared library exports the rr*/rrx* entry points and nothing else
+    (no C++ internals, kernel stubs or template instantiations)."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", str(rr.LIB_PATH)], capture_output=True, text=True, check=True)
+    names = {line.split()[-1] for line in out.stdout.splitlines() if line.strip()}
+    assert names == declared_symbols(), sorted(names ^ declared_symbols())

@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 from radeonrays_sdk_amd import api, rr, scenes
-from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, to_dev
+from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, tlas_tables_equal, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -102,10 +102,16 @@ def test_scene_update_follows_moved_instances(ctx, dev):
     torch.cuda.synchronize()
     buf, fb = tlas.buffer.cpu().numpy(), fresh.buffer.cpu().numpy()
     assert oracle.is_valid(buf) and oracle.check_consistency(buf)
-    # Transforms (inverse, forward per instance) and the BLAS table as a fresh build's.
-    xo, to = int(buf[24:32].view(np.uint64)[0]), int(buf[32:40].view(np.uint64)[0])
-    assert np.array_equal(buf[xo:xo + 37 * 96], fb[xo:xo + 37 * 96])
-    assert np.array_equal(buf[to:to + 37 * 8], fb[to:to + 37 * 8])
+    # Transforms (inverse, forward per instance) and the BLAS table as the
+    # oracle's scene over the moved instances, bit for bit; the fresh GPU
+    # build equals the oracle's node for node.
+    ref = oracle.build_tlas(xf1, [blases[k].address for k in pick], [host[blases[k].address] for k in pick])
+    ok, msg = tlas_tables_equal(buf, ref)
+    assert ok, "refit: " + msg
+    ok, msg = nodes_equal(fb, ref)
+    assert ok, "fresh build: " + msg
+    ok, msg = tlas_tables_equal(fb, ref)
+    assert ok, "fresh build: " + msg
     o = rng.uniform(-8, 8, size=(20000, 3)).astype(np.float32)
     rays = scenes.make_rays(o, scenes.unit_sphere(20000, rng))
     hits = ctx.intersect(tlas, to_dev(rays, dev), "closest", "full_t")
@@ -113,6 +119,24 @@ def test_scene_update_follows_moved_instances(ctx, dev):
     g = api.decode_hits(hits, "full_t")
     ok, msg = hits_equal(g, oracle.trace(buf, rays, blas_map=host))
     assert ok, msg
-    ok, msg = hits_equal({k: a[:3000] for k, a in g.items()}, oracle.brute_force(fb, rays[:3000], blas_map=host))
-    assert ok, "brute force on the fresh scene: " + msg
+    ok, msg = hits_equal({k: a[:3000] for k, a in g.items()}, oracle.brute_force(ref, rays[:3000], blas_map=host))
+    assert ok, "brute force on the oracle's scene: " + msg
     assert (g["inst_id"] != RR_INVALID).sum() > 50
+    # The refit's scratch is exactly what rrGetSceneBuildMemoryRequirements
+    # reports (instances and world boxes are staged there).
+    req = ctx.scene_requirements(37)
+    assert req.temporary_update_buffer_size >= 96 * 37 + 256
+
+
+def test_scene_update_with_another_instance_count_is_rejected(ctx, dev):
+    import torch
+
+    say("rrxCmdUpdateScene with a different instance count")
+    v, i = scenes.triangle_soup(500, scenes.rng_for(55))
+    b = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev))
+    xf = scenes.random_transforms(scenes.rng_for(56), 8, trans=3.0)
+    tlas = ctx.build_scene([b] * 8, xf)
+    torch.cuda.synchronize()
+    with pytest.raises(rr.RRError) as e:
+        ctx.build_scene([b] * 9, np.concatenate([xf, xf[:1]]), update=tlas)
+    assert e.value.code == rr.RR_ERROR_INVALID_PARAMETER

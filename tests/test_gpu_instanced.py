@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from radeonrays_sdk_amd import api, scenes
-from tests.gpu_util import RR_INVALID, hits_equal, say, to_dev
+from tests.gpu_util import RR_INVALID, hits_equal, nodes_equal, say, tlas_tables_equal, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +22,16 @@ def _build(ctx, dev, w):
     tlas = ctx.build_scene([blases[m] for m in w.mesh_of_instance], w.transforms)
     torch.cuda.synchronize()
     host = {b.address: b.buffer.cpu().numpy() for b in blases}
-    return tlas, tlas.buffer.cpu().numpy(), host
+    buf = tlas.buffer.cpu().numpy()
+    # The oracle's TLAS over the same BLASes: nodes, transforms and BLAS
+    # table equal to the GPU's bit for bit; every check below traces it.
+    ref = oracle.build_tlas(w.transforms, [blases[m].address for m in w.mesh_of_instance],
+                            [host[blases[m].address] for m in w.mesh_of_instance])
+    ok, msg = nodes_equal(buf, ref)
+    assert ok, msg
+    ok, msg = tlas_tables_equal(buf, ref)
+    assert ok, msg
+    return tlas, ref, host
 
 
 @pytest.mark.parametrize("size", ["reduced", "full"])
