@@ -122,6 +122,103 @@ def cpu_baseline_reference(ctx, accel, rays_dev, threads, side=1024):
                                             f"first {n} rays of the same BVH; best of 2 minus a 1x1-ray run"}
 
 
+def host_cores():
+    """CPUs this process may run on: the affinity set (what `nproc` prints),
+    cut to the cgroup's CPU quota when one is set.  Returns (threads, info)."""
+    nproc = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = min(nproc, quota) if quota else nproc
+    return threads, {"nproc": nproc, "os_cpu_count": os.cpu_count(), "cgroup_cpu_quota": quota,
+                     "cpu_model": model, "threads_used": threads}
+
+
+def cpu_build_baseline(w, batched_w, threads, log, reps=3):
+    """The oracle's LBVH build (the C++ restatement of the reference's GPU
+    build: the reference has no CPU builder) timed on the host cores: per
+    stage on the 1M-triangle heightfield, whole builds on the 65,536-triangle
+    soup and a sample of the 8,192-mesh batch, and the heightfield's 8-frame
+    refit.  Bounds, Morton codes, sort and emission run on `threads` OpenMP
+    threads; refit and restructuring are serial.  Median of `reps` after one
+    warm-up.  A baseline only: its output is not used."""
+    import oracle
+
+    from radeonrays_sdk_amd import scenes
+
+    oracle.set_threads(threads)
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    v, i = w.vertices, w.indices
+    n = i.shape[0]
+    log("cpu build baseline: heightfield stages")
+    codes = oracle.morton_codes(v, i)
+    keys, prims = oracle.sort_pairs(codes, np.arange(n, dtype=np.uint32))
+    fast = oracle.build_blas(v, i, False)
+    stages = {
+        "bounds_ms": med(lambda: oracle.mesh_bounds(v, i)),
+        "morton_ms": med(lambda: oracle.morton_codes(v, i)),
+        "sort_ms": med(lambda: oracle.sort_pairs(codes, np.arange(n, dtype=np.uint32))),
+        "emit_ms": med(lambda: oracle.emit_hierarchy(keys, prims)),
+        "refit_ms": med(lambda: oracle.refit(fast, v, i)),
+        "restructure_ms": med(lambda: oracle.restructure(fast, n)),
+    }
+    out = {"heightfield": {"triangles": int(n), **{k: round(x, 2) for k, x in stages.items()},
+                           "fast_build_ms": round(med(lambda: oracle.build_blas(v, i, False)), 2),
+                           "build_ms": round(med(lambda: oracle.build_blas(v, i, True)), 2)}}
+    frames = [scenes.perturbed_heights(v, f) for f in range(8)]
+    t0 = time.perf_counter()
+    for vf in frames:
+        oracle.refit(fast, vf, i)
+    out["heightfield"]["refit_8_frames_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+
+    log("cpu build baseline: soup and batch sample")
+    sv, si = scenes.triangle_soup(65536, scenes.rng_for(0))
+    out["soup"] = {"triangles": 65536, "fast_build_ms": round(med(lambda: oracle.build_blas(sv, si, False)), 2)}
+    if batched_w is not None:
+        counts = batched_w.tri_counts
+        m = 512
+        sample_tris = int(counts[:m].sum())
+
+        def build_sample():
+            for k in range(m):
+                o, c = int(batched_w.vertex_offsets[k]), int(counts[k])
+                oracle.build_blas(batched_w.vertices[o:o + 3 * c], np.arange(3 * c, dtype=np.uint32).reshape(-1, 3),
+                                  False)
+
+        ms = med(build_sample)
+        total = int(counts.sum())
+        out["batched"] = {"sample": f"first {m} of {len(counts)} meshes ({sample_tris} of {total} triangles), "
+                                    f"one fast build each", "sample_ms": round(ms, 2),
+                          "mtris_s": round(sample_tris / (ms * 1e-3) / 1e6, 2),
+                          "all_meshes_ms_extrapolated": round(ms * total / sample_tris, 1)}
+    out["threads"] = threads
+    out["kind"] = "port"
+    out["note"] = ("oracle/oracle.cpp build (restatement of the reference's GPU LBVH; the reference has no CPU "
+                   "builder); bounds, Morton, sort and emission on the threads, refit and restructuring serial")
+    return out
+
+
 def serving_bench(ctx, dev, n_requests, log):
     """The serving stream: 32 resident scenes (log-uniform 10k-2M triangles,
     soups and heightfields), requests of log-uniform 1k-1M rays sliced from
@@ -388,7 +485,7 @@ def batched_bench(ctx, dev, log, frames=16):
             "per_call_build_ms_wall": round(per_call_ms, 1), "batch_equals_per_call": bool(same),
             "refit_ms": round(refit_ms, 3), "refit_gbs": round(tris * 304 / (refit_ms * 1e-3) / 1e9, 1),
             "query_ms": round(query_ms, 3), "query_mrays_s": round(R / query_ms / 1e3, 1),
-            "graph_frame_ms": round(fm, 3), "graph_frame_mrays_s": round(R / fm / 1e3, 1)}
+            "graph_frame_ms": round(fm, 3), "graph_frame_mrays_s": round(R / fm / 1e3, 1)}, w
 
 
 def main():
@@ -549,8 +646,9 @@ def main():
         out["soup"], *soup = soup_bench(ctx, dev, log)
     if not args.no_instanced:
         out["instanced"] = instanced_bench(ctx, dev, log)
+    batched_w = None
     if not args.no_batched:
-        out["batched"] = batched_bench(ctx, dev, log)
+        out["batched"], batched_w = batched_bench(ctx, dev, log)
     if not args.no_serving:
         out["serving"] = serving_bench(ctx, dev, args.serving_requests, log)
 
@@ -564,7 +662,8 @@ def main():
 
     if not args.no_cpu:
         log("cpu baseline")
-        threads = min(16, os.cpu_count() or 1)
+        threads, host = host_cores()
+        out["host"] = host
         buf = accel.buffer.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(buf, w.rays, threads)
         ref = cpu_baseline_reference(ctx, accel, rays, threads)
@@ -586,6 +685,7 @@ def main():
             if sref is not None:
                 out["soup"]["cpu_reference_mrays_s"] = sref["value"]
             out["soup"]["cpu_cores"] = threads
+        out["cpu_build_baseline"] = cpu_build_baseline(w, batched_w, threads, log)
     ctx.close()
     print(json.dumps(out), flush=True)
 

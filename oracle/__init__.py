@@ -35,6 +35,7 @@ def load():
             "orc_morton_codes": ([P, c_uint32, P, c_uint32, P], None),
             "orc_morton3": ([c_float, c_float, c_float], c_uint32),
             "orc_sort_pairs": ([P, P, c_uint32], None),
+            "orc_set_threads": ([c_int], None),
             "orc_emit_hierarchy": ([P, P, c_uint32, P], None),
             "orc_refit": ([P, c_uint32, P, c_uint32, P], None),
             "orc_restructure": ([c_uint32, P], None),
@@ -110,6 +111,11 @@ def sort_pairs(keys, vals):
     k, v = _c(keys, np.uint32).copy(), _c(vals, np.uint32).copy()
     load().orc_sort_pairs(_p(k), _p(v), k.size)
     return k, v
+
+
+def set_threads(threads: int) -> None:
+    """OpenMP threads of the parallel build stages (results do not depend on it)."""
+    load().orc_set_threads(int(threads))
 
 
 def emit_hierarchy(sorted_codes, sorted_prims):

@@ -114,16 +114,28 @@ void Tri(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t t, flo
     }
 }
 
+// Min and max of ordered uints are exact, so the thread count and the order
+// of the reduction do not change the result.
 void Bounds(const float* vtx, uint32_t stride, const uint32_t* idx, uint32_t tris, float lo[3], float hi[3]) {
     uint32_t mn[3] = {kInv, kInv, kInv}, mx[3] = {0, 0, 0};
-    for (uint32_t t = 0; t < tris; ++t) {
-        float v[9];
-        Tri(vtx, stride, idx, t, v);
-        for (int k = 0; k < 3; ++k)
-            for (int a = 0; a < 3; ++a) {
-                mn[a] = std::min(mn[a], ToOrdered(v[3 * k + a]));
-                mx[a] = std::max(mx[a], ToOrdered(v[3 * k + a]));
-            }
+#pragma omp parallel
+    {
+        uint32_t tmn[3] = {kInv, kInv, kInv}, tmx[3] = {0, 0, 0};
+#pragma omp for schedule(static) nowait
+        for (int64_t t = 0; t < (int64_t)tris; ++t) {
+            float v[9];
+            Tri(vtx, stride, idx, (uint32_t)t, v);
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    tmn[a] = std::min(tmn[a], ToOrdered(v[3 * k + a]));
+                    tmx[a] = std::max(tmx[a], ToOrdered(v[3 * k + a]));
+                }
+        }
+#pragma omp critical
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], tmn[a]);
+            mx[a] = std::max(mx[a], tmx[a]);
+        }
     }
     for (int a = 0; a < 3; ++a) {
         lo[a] = FromOrdered(mn[a]);
@@ -138,8 +150,11 @@ int Delta(const uint32_t* c, int n, int i, int j) {
     return 32 + __builtin_clz((uint32_t)(i ^ j));
 }
 
+// Every node is written by one iteration only (its own, and the parent
+// link by its one parent's), so the loops run in parallel.
 void Emit(const uint32_t* c, const uint32_t* prims, uint32_t n, Node* nodes) {
-    for (uint32_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
         Node& l = nodes[n - 1 + i];
         l.c0 = kInv;
         l.c1 = prims[i];
@@ -150,6 +165,7 @@ void Emit(const uint32_t* c, const uint32_t* prims, uint32_t n, Node* nodes) {
         return;
     }
     const int N = (int)n;
+#pragma omp parallel for schedule(static)
     for (int I = 0; I < N - 1; ++I) {
         // FindSpan (lbvh_emit_hierarchy_mesh.comp:105-140).
         const int d = (Delta(c, N, I, I + 1) - Delta(c, N, I, I - 1)) > 0 ? 1 : -1;
@@ -729,9 +745,10 @@ void orc_morton_codes(const float* vtx, uint32_t stride, const uint32_t* idx, ui
     Bounds(vtx, stride, idx, tris, lo, hi);
     float ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = hi[a] - lo[a];
-    for (uint32_t t = 0; t < tris; ++t) {
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < (int64_t)tris; ++t) {
         float v[9], c[3];
-        Tri(vtx, stride, idx, t, v);
+        Tri(vtx, stride, idx, (uint32_t)t, v);
         for (int a = 0; a < 3; ++a) {
             const float l = Min(Min(v[a], v[3 + a]), v[6 + a]);
             const float h = Max(Max(v[a], v[3 + a]), v[6 + a]);
@@ -741,14 +758,49 @@ void orc_morton_codes(const float* vtx, uint32_t stride, const uint32_t* idx, ui
     }
 }
 
+// Stable sort of (key, value) pairs by key: LSD radix sort, 8-bit digits,
+// four passes.  Each thread counts and then scatters one contiguous chunk;
+// a digit's output run takes the chunks in order, which keeps equal keys in
+// input order (the property std::stable_sort gives, algos_test.h:378).
 void orc_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n) {
-    std::vector<std::pair<uint32_t, uint32_t>> kv(n);
-    for (uint32_t i = 0; i < n; ++i) kv[i] = {keys[i], vals[i]};
-    std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-    for (uint32_t i = 0; i < n; ++i) {
-        keys[i] = kv[i].first;
-        vals[i] = kv[i].second;
+    if (n < 2) return;
+    std::vector<uint32_t> k2(n), v2(n);
+    uint32_t *ki = keys, *vi = vals, *ko = k2.data(), *vo = v2.data();
+    const int threads = std::max(1, std::min(omp_get_max_threads(), (int)(n / 4096) + 1));
+    std::vector<uint64_t> count((size_t)threads * 256);
+    for (int shift = 0; shift < 32; shift += 8) {
+#pragma omp parallel num_threads(threads)
+        {
+            const int t = omp_get_thread_num();
+            const uint64_t b = (uint64_t)n * t / threads, e = (uint64_t)n * (t + 1) / threads;
+            uint64_t* c = count.data() + (size_t)t * 256;
+            std::fill(c, c + 256, 0);
+            for (uint64_t i = b; i < e; ++i) ++c[(ki[i] >> shift) & 255u];
+#pragma omp barrier
+#pragma omp single
+            {
+                uint64_t run = 0;
+                for (int d = 0; d < 256; ++d)
+                    for (int u = 0; u < threads; ++u) {
+                        const uint64_t x = count[(size_t)u * 256 + d];
+                        count[(size_t)u * 256 + d] = run;
+                        run += x;
+                    }
+            }
+            for (uint64_t i = b; i < e; ++i) {
+                const uint64_t o = c[(ki[i] >> shift) & 255u]++;
+                ko[o] = ki[i];
+                vo[o] = vi[i];
+            }
+        }
+        std::swap(ki, ko);
+        std::swap(vi, vo);
     }
+    // Four passes: the sorted pairs are back in keys / vals.
+}
+
+void orc_set_threads(int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
 }
 
 void orc_emit_hierarchy(const uint32_t* codes, const uint32_t* prims, uint32_t n, void* nodes) {

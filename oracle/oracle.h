@@ -41,6 +41,11 @@ uint32_t orc_morton3(float x, float y, float z);
 /* Stable sort of (key, value) pairs by key (a4-a6). */
 void orc_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n);
 
+/* OpenMP threads of the build stages that run in parallel (bounds, Morton
+ * codes, sort, emission) and of traversal without a thread count; results
+ * do not depend on it. */
+void orc_set_threads(int threads);
+
 /* Karras hierarchy over sorted codes into nodes[2n-1] (a7): writes child0,
  * child1, parent, update of internal nodes and child0/child1/parent of leaves. */
 void orc_emit_hierarchy(const uint32_t* sorted_codes, const uint32_t* sorted_prims, uint32_t n, void* nodes);
