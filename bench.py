@@ -488,6 +488,69 @@ def batched_bench(ctx, dev, log, frames=16):
             "graph_frame_ms": round(fm, 3), "graph_frame_mrays_s": round(R / fm / 1e3, 1)}, w
 
 
+def profile_section(which: str, reps: int = 3) -> None:
+    """A few launches of one kernel's workload and nothing else, so that a
+    rocprofv3 counter pass (one run per counter set) stays short:
+      sort         rrxCmdSortPairs of 16M random 30-bit keys
+      refit        the 1,045,458-triangle heightfield's refit (UPDATE)
+      batch-refit  the 8,192-mesh batched refit (rrxCmdBuildGeometryBatch UPDATE)
+      trace        the heightfield's closest-hit query of 4,194,304 rays
+    Prints one JSON line naming the workload and its algorithmic bytes per
+    launch."""
+    import torch
+
+    from radeonrays_sdk_amd import api, scenes
+
+    dev = torch.device("cuda", 0)
+    ctx = api.Context(0)
+    out = {"profile": which, "launches": reps}
+    if which == "sort":
+        n = 16 << 20
+        keys = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, device=dev)
+        vals = torch.arange(n, dtype=torch.int32, device=dev)
+        tmp = (torch.empty_like(keys), torch.empty_like(vals))
+        k2, v2 = keys.clone(), vals.clone()
+        for _ in range(reps):
+            k2.copy_(keys)
+            v2.copy_(vals)
+            ctx.sort_pairs(k2, v2, 30, tmp=tmp)
+        out.update(workload="16,777,216 random 30-bit keys + uint32 values", algorithmic_bytes=68 * n)
+    elif which in ("refit", "trace"):
+        w = scenes.heightfield()
+        v = torch.from_numpy(w.vertices).to(dev)
+        idx = torch.from_numpy(w.indices).to(dev)
+        accel = ctx.build_geometry(v, idx, restructure=True)
+        n = idx.shape[0]
+        if which == "refit":
+            v2 = torch.from_numpy(scenes.perturbed_heights(w.vertices, 0)).to(dev)
+            for _ in range(reps):
+                ctx.update_geometry(accel, v2, idx)
+            out.update(workload=f"heightfield refit, {n} triangles", algorithmic_bytes=n * 112 + (n - 1) * 192)
+        else:
+            rays = torch.from_numpy(w.rays).to(dev)
+            hits = ctx.empty(rays.shape[0] * HIT_BYTES)
+            tscratch = ctx.trace_scratch(rays.shape[0])
+            for _ in range(reps):
+                ctx.intersect(accel, rays, "closest", "full", hits=hits, scratch=tscratch)
+            out.update(workload=f"heightfield closest hit, {rays.shape[0]} rays, {n} triangles")
+    elif which == "batch-refit":
+        w = scenes.batched()
+        V = torch.from_numpy(w.vertices).to(dev)
+        I = torch.arange(3 * int(w.tri_counts.max()), dtype=torch.int32, device=dev).reshape(-1, 3)
+        spans = [(int(o), int(n)) for o, n in zip(w.vertex_offsets, w.tri_counts)]
+        meshes = [(V[o:o + 3 * n], I[:n]) for o, n in spans]
+        scratch = ctx.empty(ctx.batch_build_requirements(meshes))
+        accels = ctx.build_geometry_batch(meshes, scratch=scratch)
+        for _ in range(reps):
+            ctx.update_geometry_batch(accels, meshes, scratch=scratch)
+        tris = int(w.tri_counts.sum())
+        out.update(workload=f"{len(spans)} meshes, {tris} triangles, batched refit",
+                   algorithmic_bytes=tris * 112 + (tris - len(spans)) * 192)
+    torch.cuda.synchronize()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -503,7 +566,12 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
+    ap.add_argument("--profile", choices=["sort", "refit", "batch-refit", "trace"], default=None,
+                    help="run only this kernel's workload a few times (for a rocprofv3 counter pass) and exit")
     args = ap.parse_args()
+    if args.profile:
+        profile_section(args.profile)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(rays would shard across GPUs over a replicated BVH).", file=sys.stderr)

@@ -16,12 +16,14 @@ extern "C" {
 /* Scratch bytes for rrxCmdSortPairs on n pairs. */
 RR_API RRError rrxGetSortMemoryRequirements(RRContext context, uint32_t n, size_t* scratch_size);
 
-/* Stable LSD radix sort of n (uint32 key, uint32 value) pairs by the low
- * key_bits bits of the key, in place in keys/values, using keys_tmp /
- * values_tmp (n uint32 each) and scratch.  The same onesweep sort the BVH
- * builder runs on Morton codes (reference counterpart: the internal
- * RadixSortKeyValue of vlk/radix_sort.cpp, which the public API does not
- * expose). */
+/* Stable radix sort of n (uint32 key, uint32 value) pairs whose keys are
+ * below 2^key_bits (bits above key_bits must be zero), in place in
+ * keys/values, using keys_tmp / values_tmp (n uint32 each) and scratch.
+ * The onesweep LSD sort the BVH builder runs on Morton codes, or on
+ * 2^22..2^24 keys one most-significant-digit pass followed by a per-bucket
+ * sort (RRX_TUNING_SORT_HYBRID); the order is the same.  Reference
+ * counterpart: the internal RadixSortKeyValue of vlk/radix_sort.cpp, which
+ * the public API does not expose. */
 RR_API RRError rrxCmdSortPairs(RRContext       context,
                                RRDevicePtr     keys,
                                RRDevicePtr     values,
@@ -65,11 +67,18 @@ typedef enum RRxTuning
                                              5, 7 (default, the reference's) or 9; changes the
                                              tree (so any-hit may report another hit), not the
                                              closest hits */
-    RRX_TUNING_SORT_SPIN_LIMIT = 6        /* test hook, not a speed setting: bound of a sort
+    RRX_TUNING_SORT_SPIN_LIMIT = 6,       /* test hook, not a speed setting: bound of a sort
                                              tile's wait for its predecessors, in polls that find
                                              one unpublished (default 2^22; 0 gives up at the
                                              first).  A tile that gives up sorts wrongly and the
                                              failure is reported (rrxGetDeviceErrors) */
+    RRX_TUNING_SORT_HYBRID = 7            /* 1 (default): rrxCmdSortPairs on 2^22..2^24 keys takes
+                                             one 10-bit most-significant-digit pass and then sorts
+                                             each bucket in one workgroup's LDS (two passes over
+                                             the data instead of four); a bucket of more than
+                                             24,576 keys is sorted by its workgroup through memory,
+                                             so keys concentrated in a few buckets sort slower.
+                                             0: the LSD passes.  Same order either way */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

@@ -7,13 +7,15 @@ function restates and which reference file it follows.
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# RR_ORACLE_LIB names another build (the host-sanitizer build, liboracle_asan.so).
+LIB = Path(os.environ.get("RR_ORACLE_LIB") or HERE / "liboracle.so")
 REF_ANALYZER = HERE / "_ref" / "bvh_analyzer"
 
 HIT_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("inst_id", "<u4"), ("prim_id", "<u4"), ("t", "<f4"),

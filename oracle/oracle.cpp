@@ -771,8 +771,11 @@ void orc_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n) {
     for (int shift = 0; shift < 32; shift += 8) {
 #pragma omp parallel num_threads(threads)
         {
+            // The runtime may start fewer threads than asked for: split the
+            // keys over the threads that run.
+            const int nt = omp_get_num_threads();
             const int t = omp_get_thread_num();
-            const uint64_t b = (uint64_t)n * t / threads, e = (uint64_t)n * (t + 1) / threads;
+            const uint64_t b = (uint64_t)n * t / nt, e = (uint64_t)n * (t + 1) / nt;
             uint64_t* c = count.data() + (size_t)t * 256;
             std::fill(c, c + 256, 0);
             for (uint64_t i = b; i < e; ++i) ++c[(ki[i] >> shift) & 255u];
@@ -781,7 +784,7 @@ void orc_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n) {
             {
                 uint64_t run = 0;
                 for (int d = 0; d < 256; ++d)
-                    for (int u = 0; u < threads; ++u) {
+                    for (int u = 0; u < nt; ++u) {
                         const uint64_t x = count[(size_t)u * 256 + d];
                         count[(size_t)u * 256 + d] = run;
                         run += x;

@@ -18,6 +18,14 @@ CSRC = PKG_DIR / "csrc"
 INCLUDE = PKG_DIR.parent / "include"
 LIB_PATH = PKG_DIR / "libradeonrays.so"
 OBJ_DIR = PKG_DIR / "_obj"
+# Host AddressSanitizer + UndefinedBehaviorSanitizer build of the same
+# sources (host code only: each -fsanitize= goes to the host pass through
+# -Xarch_host; device code is compiled as usual).
+ASAN_LIB_PATH = PKG_DIR / "libradeonrays_asan.so"
+ASAN_OBJ_DIR = PKG_DIR / "_obj_asan"
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+                 "-Xarch_host", "-shared-libsan"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -41,13 +49,15 @@ def _sources():
     return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
 
-def _compile(src: Path, verbose: bool) -> Path:
-    obj = OBJ_DIR / (src.stem + ".o")
+def _compile(src: Path, verbose: bool, sanitize: bool = False) -> Path:
+    obj = (ASAN_OBJ_DIR if sanitize else OBJ_DIR) / (src.stem + ".o")
     cmd = [HIPCC, *DEVICE_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden",
                "-fvisibility-inlines-hidden", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o",
                str(obj)]
+    if sanitize:
+        cmd[1:1] = HOST_SANITIZE
     proc = subprocess.run(cmd, capture_output=True, text=True)
     errs = "\n".join(l for l in proc.stderr.splitlines() if "packed-fp32-ops" not in l)
     if proc.returncode != 0:
@@ -57,22 +67,26 @@ def _compile(src: Path, verbose: bool) -> Path:
     return obj
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> Path:
-    """Compile every source and link LIB_PATH; always rebuilds."""
-    OBJ_DIR.mkdir(exist_ok=True)
+def build(verbose: bool = False, jobs: int | None = None, sanitize: bool = False) -> Path:
+    """Compile every source and link LIB_PATH (ASAN_LIB_PATH with sanitize);
+    always rebuilds."""
+    obj_dir, lib_path = (ASAN_OBJ_DIR, ASAN_LIB_PATH) if sanitize else (OBJ_DIR, LIB_PATH)
+    obj_dir.mkdir(exist_ok=True)
     srcs = _sources()
     jobs = jobs or min(len(srcs), os.cpu_count() or 4)
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
-    tmp = LIB_PATH.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda s: _compile(s, verbose, sanitize), srcs))
+    tmp = lib_path.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", f"-Wl,--version-script={CSRC / 'exports.map'}",
            *map(str, objs), "-o", str(tmp)]
+    if sanitize:
+        cmd[1:1] = HOST_SANITIZE
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, sanitize="--sanitize" in sys.argv))

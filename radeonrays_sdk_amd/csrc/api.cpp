@@ -367,13 +367,13 @@ RRError Guarded(const char* name, F&& f) {
 // With resolve == false only the counts are read (memory requirements never
 // look at the device pointers, as in the reference).
 RRError MeshFromInput(const RRGeometryBuildInput* in, rr::MeshDesc& m, const char* name, bool resolve) {
-    if (in->primitive_type != RR_PRIMITIVE_TYPE_TRIANGLE_MESH) {
+    if (rr::Raw(in->primitive_type) != RR_PRIMITIVE_TYPE_TRIANGLE_MESH) {
         RR_LOG(RR_LOG_LEVEL_ERROR, "%s: build input type not supported", name);
         return RR_ERROR_NOT_IMPLEMENTED;
     }
     RR_REQUIRE(in->primitive_count == 1 && in->triangle_mesh_primitives, name);
     const RRTriangleMeshPrimitive& p = in->triangle_mesh_primitives[0];
-    RR_REQUIRE(p.index_type == RR_INDEX_TYPE_UINT32 || p.index_type == RR_INDEX_TYPE_UINT16, name);
+    RR_REQUIRE(rr::Raw(p.index_type) == RR_INDEX_TYPE_UINT32 || rr::Raw(p.index_type) == RR_INDEX_TYPE_UINT16, name);
     RR_REQUIRE(p.triangle_count < (1u << 30), name);
     m.tri_count = p.triangle_count;
     if (!resolve) return RR_SUCCESS;
@@ -385,7 +385,7 @@ RRError MeshFromInput(const RRGeometryBuildInput* in, rr::MeshDesc& m, const cha
     m.vertex_count = p.vertex_count;
     m.vertex_stride = p.vertex_stride;
     m.indices = Addr(p.triangle_indices);
-    m.index16 = p.index_type == RR_INDEX_TYPE_UINT16 ? 1u : 0u;
+    m.index16 = rr::Raw(p.index_type) == RR_INDEX_TYPE_UINT16 ? 1u : 0u;
     m.tri_count = p.triangle_count;
     return RR_SUCCESS;
 }
@@ -401,7 +401,7 @@ long WaitSpinMicros() {
 
 // An AABB list (radeonrays.h:201-206; the reference declares it and answers
 // RR_ERROR_NOT_IMPLEMENTED, radeonrays.cpp:286-289).  One list per input.
-bool IsAabbList(const RRGeometryBuildInput* in) { return in->primitive_type == RR_PRIMITIVE_TYPE_AABB_LIST; }
+bool IsAabbList(const RRGeometryBuildInput* in) { return rr::Raw(in->primitive_type) == RR_PRIMITIVE_TYPE_AABB_LIST; }
 
 RRError AabbsFromInput(const RRGeometryBuildInput* in, rr::AabbDesc& a, const char* name, bool resolve) {
     RR_REQUIRE(in->primitive_count == 1 && in->aabb_primitives, name);
@@ -418,7 +418,7 @@ RRError AabbsFromInput(const RRGeometryBuildInput* in, rr::AabbDesc& a, const ch
 // Several triangle meshes in one build input (primitive_count > 1; the
 // reference asserts a single mesh, intersector.cpp:110,146).
 bool IsMultiMesh(const RRGeometryBuildInput* in) {
-    return in->primitive_type == RR_PRIMITIVE_TYPE_TRIANGLE_MESH && in->primitive_count > 1;
+    return rr::Raw(in->primitive_type) == RR_PRIMITIVE_TYPE_TRIANGLE_MESH && in->primitive_count > 1;
 }
 
 RRError SubMeshesFromInput(const RRGeometryBuildInput* in, std::vector<rr::SubMesh>& out, uint32_t& tris,
@@ -531,7 +531,7 @@ extern "C" {
 RRError rrCreateContext(uint32_t api_version, RRApi api, RRContext* context) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrCreateContext(%u)", api_version);
     RR_REQUIRE(context, "rrCreateContext");
-    if (api != RR_API_HIP) {
+    if (rr::Raw(api) != RR_API_HIP) {
         RR_LOG(RR_LOG_LEVEL_ERROR, "rrCreateContext: only RR_API_HIP is available");
         return RR_ERROR_UNSUPPORTED_API;
     }
@@ -568,9 +568,10 @@ RRError rrDestroyContext(RRContext context) {
 }
 
 RRError rrSetLogLevel(RRLogLevel log_level) {
-    if (log_level < RR_LOG_LEVEL_DEBUG || log_level > RR_LOG_LEVEL_OFF) return RR_ERROR_INVALID_PARAMETER;
-    Log().level = log_level;
-    RR_LOG(RR_LOG_LEVEL_INFO, "rrSetLogLevel(%d)", (int)log_level);
+    const uint32_t level = rr::Raw(log_level);
+    if (level < RR_LOG_LEVEL_DEBUG || level > RR_LOG_LEVEL_OFF) return RR_ERROR_INVALID_PARAMETER;
+    Log().level = (int)level;
+    RR_LOG(RR_LOG_LEVEL_INFO, "rrSetLogLevel(%d)", (int)level);
     return RR_SUCCESS;
 }
 
@@ -656,14 +657,14 @@ RRError rrCmdBuildGeometry(RRContext context, RRBuildOperation build_operation, 
                            RRDevicePtr geometry_buffer, RRCommandStream command_stream) {
     RR_LOG(RR_LOG_LEVEL_INFO, "rrCmdBuildGeometry");
     RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && build_input, "rrCmdBuildGeometry");
-    RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
+    RR_REQUIRE(rr::Raw(build_operation) == RR_BUILD_OPERATION_BUILD || rr::Raw(build_operation) == RR_BUILD_OPERATION_UPDATE,
                "rrCmdBuildGeometry");
     RR_REQUIRE(OptPtr(temporary_buffer) && OptPtr(geometry_buffer), "rrCmdBuildGeometry");
     return Guarded("rrCmdBuildGeometry", [&]() -> RRError {
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
         const bool restructure = WantsRestructure(build_options);
-        const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
+        const bool update = rr::Raw(build_operation) == RR_BUILD_OPERATION_UPDATE;
         if (IsAabbList(build_input)) {
             rr::AabbDesc a;
             const RRError e = AabbsFromInput(build_input, a, "rrCmdBuildGeometry", true);
@@ -803,9 +804,10 @@ RRError rrCmdIntersect(RRContext context, RRDevicePtr scene_buffer, RRIntersectQ
     RR_REQUIRE(LiveCtx(context) && LivePtr(scene_buffer) && LivePtr(rays) && LivePtr(hits) && LivePtr(scratch) &&
                    LiveStream(command_stream) && OptPtr(indirect_ray_count),
                "rrCmdIntersect");
-    RR_REQUIRE(query == RR_INTERSECT_QUERY_CLOSEST || query == RR_INTERSECT_QUERY_ANY, "rrCmdIntersect");
-    RR_REQUIRE(query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT || query_output == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
-                   query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
+    RR_REQUIRE(rr::Raw(query) == RR_INTERSECT_QUERY_CLOSEST || rr::Raw(query) == RR_INTERSECT_QUERY_ANY, "rrCmdIntersect");
+    RR_REQUIRE(rr::Raw(query_output) == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT ||
+                   rr::Raw(query_output) == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
+                   rr::Raw(query_output) == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
                "rrCmdIntersect");
     return Guarded("rrCmdIntersect", [&]() -> RRError {
         Context* c = AsCtx(context);
@@ -1132,10 +1134,10 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
         const RRxIntersectRequest& q = requests[i];
         RR_REQUIRE(LivePtr(q.scene_buffer) && (q.ray_count == 0 || (LivePtr(q.rays) && LivePtr(q.hits))),
                    "rrxCmdIntersectBatch");
-        RR_REQUIRE(q.query == RR_INTERSECT_QUERY_CLOSEST || q.query == RR_INTERSECT_QUERY_ANY, "rrxCmdIntersectBatch");
-        RR_REQUIRE(q.query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT ||
-                       q.query_output == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
-                       q.query_output == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
+        RR_REQUIRE(rr::Raw(q.query) == RR_INTERSECT_QUERY_CLOSEST || rr::Raw(q.query) == RR_INTERSECT_QUERY_ANY, "rrxCmdIntersectBatch");
+        RR_REQUIRE(rr::Raw(q.query_output) == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT ||
+                       rr::Raw(q.query_output) == RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID ||
+                       rr::Raw(q.query_output) == RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T,
                    "rrxCmdIntersectBatch");
     }
     return Guarded("rrxCmdIntersectBatch", [&]() -> RRError {
@@ -1262,7 +1264,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
     RR_LOG(RR_LOG_LEVEL_INFO, "rrxCmdBuildGeometryBatch");
     RR_REQUIRE(LiveCtx(context) && LiveStream(command_stream) && LivePtr(temporary_buffer), "rrxCmdBuildGeometryBatch");
     RR_REQUIRE(count == 0 || (build_inputs && geometry_buffers), "rrxCmdBuildGeometryBatch");
-    RR_REQUIRE(build_operation == RR_BUILD_OPERATION_BUILD || build_operation == RR_BUILD_OPERATION_UPDATE,
+    RR_REQUIRE(rr::Raw(build_operation) == RR_BUILD_OPERATION_BUILD || rr::Raw(build_operation) == RR_BUILD_OPERATION_UPDATE,
                "rrxCmdBuildGeometryBatch");
     for (uint32_t i = 0; i < count; ++i) RR_REQUIRE(LivePtr(geometry_buffers[i]), "rrxCmdBuildGeometryBatch");
     return Guarded("rrxCmdBuildGeometryBatch", [&]() -> RRError {
@@ -1276,7 +1278,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
         void* scratch = Addr(temporary_buffer);
         std::vector<void*> results(count);
         for (uint32_t i = 0; i < count; ++i) results[i] = Addr(geometry_buffers[i]);
-        const bool update = build_operation == RR_BUILD_OPERATION_UPDATE;
+        const bool update = rr::Raw(build_operation) == RR_BUILD_OPERATION_UPDATE;
         if (!update)
             for (uint32_t i = 0; i < count; ++i) RecordAccel(c, results[i], rr::kAccelBlas, plan.meshes[i].tri_count);
         // The mesh table, packed now into pinned memory that the recorded
@@ -1334,7 +1336,7 @@ RRError rrxCmdBuildGeometryBatch(RRContext context, RRBuildOperation build_opera
 }
 
 RRError rrxSetTuning(RRxTuning key, uint32_t value) {
-    switch (key) {
+    switch (rr::Raw(key)) {
         case RRX_TUNING_TRACE_MIN_BUSY:
             RR_REQUIRE(value >= 1 && value <= 64, "rrxSetTuning");
             rr::SetMinBusyLanes(value);
@@ -1354,13 +1356,17 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
         case RRX_TUNING_SORT_SPIN_LIMIT:
             rr::SetSortSpinLimit(value);
             return RR_SUCCESS;
+        case RRX_TUNING_SORT_HYBRID:
+            RR_REQUIRE(value <= 1, "rrxSetTuning");
+            rr::SetSortHybrid(value != 0);
+            return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
 
 RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
     RR_REQUIRE(value, "rrxGetTuning");
-    switch (key) {
+    switch (rr::Raw(key)) {
         case RRX_TUNING_TRACE_MIN_BUSY: *value = rr::MinBusyLanes(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_RAYS_PER_FETCH: *value = rr::RaysPerFetch(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_LDS_STACK: *value = rr::LdsStackDepth(); return RR_SUCCESS;
@@ -1368,6 +1374,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_SORT_DIGIT_BITS: *value = (uint32_t)rr::SortDigitBits(); return RR_SUCCESS;
         case RRX_TUNING_TREELET_LEAVES: *value = (uint32_t)rr::TreeletLeaves(); return RR_SUCCESS;
         case RRX_TUNING_SORT_SPIN_LIMIT: *value = rr::SortSpinLimit(); return RR_SUCCESS;
+        case RRX_TUNING_SORT_HYBRID: *value = rr::SortHybrid() ? 1u : 0u; return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

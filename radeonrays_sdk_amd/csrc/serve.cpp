@@ -296,7 +296,7 @@ extern "C" {
 RRError rrxCreateServer(RRContext context, const RRxServerOptions* options, RRxServer* server) {
     if (!rr::IsLiveContext(context) || !options || !server) return RR_ERROR_INVALID_PARAMETER;
     const RRxServerOptions& o = *options;
-    if (o.mode != RRX_SERVE_BATCHED && o.mode != RRX_SERVE_PLAIN) return RR_ERROR_INVALID_PARAMETER;
+    if (rr::Raw(o.mode) != RRX_SERVE_BATCHED && rr::Raw(o.mode) != RRX_SERVE_PLAIN) return RR_ERROR_INVALID_PARAMETER;
     if (o.max_batch_rays == 0 || o.memory_budget == 0) return RR_ERROR_INVALID_PARAMETER;
     if (o.mode == RRX_SERVE_BATCHED && (o.max_inflight == 0 || o.max_inflight > 16 || o.max_batch_requests == 0))
         return RR_ERROR_INVALID_PARAMETER;
@@ -371,9 +371,10 @@ RRError rrxServerSubmit(RRxServer server, const RRxIntersectRequest* requests, c
     for (uint32_t i = 0; i < count; ++i) {
         const RRxIntersectRequest& q = requests[i];
         if (!q.scene_buffer || (q.ray_count && (!q.rays || !q.hits))) return RR_ERROR_INVALID_PARAMETER;
-        if (q.query != RR_INTERSECT_QUERY_CLOSEST && q.query != RR_INTERSECT_QUERY_ANY) return RR_ERROR_INVALID_PARAMETER;
-        if (q.query_output != RR_INTERSECT_QUERY_OUTPUT_FULL_HIT && q.query_output != RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID &&
-            q.query_output != RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T)
+        if (rr::Raw(q.query) != RR_INTERSECT_QUERY_CLOSEST && rr::Raw(q.query) != RR_INTERSECT_QUERY_ANY) return RR_ERROR_INVALID_PARAMETER;
+        if (rr::Raw(q.query_output) != RR_INTERSECT_QUERY_OUTPUT_FULL_HIT &&
+            rr::Raw(q.query_output) != RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID &&
+            rr::Raw(q.query_output) != RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T)
             return RR_ERROR_INVALID_PARAMETER;
         if (q.ray_count > sv->opt.max_batch_rays) return RR_ERROR_INVALID_PARAMETER;
         if (uint64_t(q.ray_count) * (32 + HitBytes(q.query_output)) > sv->opt.memory_budget)

@@ -54,6 +54,15 @@ constexpr uint32_t kSortDefaultSpinLimit = 1u << 22;
 uint32_t SortSpinLimit();
 void SetSortSpinLimit(uint32_t polls);
 
+// Sorts of kSortHybridMinKeys..kSortHybridMaxKeys pairs whose caller did not
+// prepare the histogram take one most-significant-digit pass (up to 10
+// bits) and then sort each bucket in one workgroup's LDS, instead of the
+// LSD passes (process-wide switch; the order is the same either way).
+constexpr uint32_t kSortHybridMinKeys = 1u << 22;
+constexpr uint32_t kSortHybridMaxKeys = 1u << 24;
+bool SortHybrid();
+void SetSortHybrid(bool on);
+
 // Scratch needed to sort n pairs: histograms, tile tickets and look-back
 // status words.  Everything in it is (re)initialised by the sort itself.
 size_t SortScratchBytes(uint32_t n);
@@ -64,6 +73,7 @@ __host__ __device__ inline int SortPasses(int key_bits) { return (key_bits + kSo
 struct SortScratch {
     uint32_t* hist;     // [kSortHistCopies][kSortMaxPasses][256] global digit counts
     uint32_t* tickets;  // [kSortNarrowPasses] tile tickets, one per pass
+    uint32_t* bucket_off;  // [1025] hybrid sort: first position of each MSD bucket, then n
     uint32_t* status;   // [kSortMaxPasses][tiles][256] look-back words
     size_t status_words;  // words of pass 0's look-back region, which must be zero before a sort
 };

@@ -301,6 +301,467 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// ---------------------------------------------------------------------------
+// Hybrid sort of large inputs: one most-significant-digit pass, then every
+// bucket sorted on its own by one workgroup.
+//
+// The MSD pass is the onesweep pass above with a digit of up to 10 bits
+// taken from the top of the key: it moves each pair to its bucket, stably,
+// through the same decoupled look-back.  A bucket of up to kBucketCapacity
+// pairs is then loaded by one workgroup, sorted by the remaining low bits in
+// LDS (stable LSD rounds of up to 8 bits, ranked with ballots) and written
+// back contiguously; a larger bucket (skewed keys) is sorted by its
+// workgroup in capacity-sized chunks through memory.  Two passes over the
+// data instead of four, and the second reads and writes whole runs.
+
+constexpr int kMsdBits = 10;
+constexpr int kMsdRadix = 1 << kMsdBits;
+constexpr int kMsdBlock = 1024;  // one thread per digit
+constexpr int kMsdItems = 16;    // 16,384-key tiles
+constexpr int kBucketBlock = 1024;
+constexpr int kBucketItems = 24;
+constexpr uint32_t kBucketCapacity = kBucketBlock * kBucketItems;
+constexpr int kBucketRoundBits = 8;  // widest digit of one LDS round
+static_assert(kMsdBlock == kMsdRadix, "the MSD pass gives each thread one digit");
+
+namespace {
+
+// Lanes of this wave whose digit d (of `bits` bits, at most MAXBITS) equals
+// this lane's: an AND over the digit's bits of the bit's ballot or its
+// complement.
+template <int MAXBITS>
+__device__ __forceinline__ void MatchDigit(uint32_t d, int bits, uint32_t& m_lo, uint32_t& m_hi) {
+    m_lo = ~0u;
+    m_hi = ~0u;
+#pragma unroll
+    for (int b = 0; b < MAXBITS; ++b) {
+        if (b >= bits) continue;
+        const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);  // 0 or ~0
+        const uint64_t bal = __ballot(neg != 0u);
+        m_lo &= ~((uint32_t)bal ^ neg);
+        m_hi &= ~((uint32_t)(bal >> 32) ^ neg);
+    }
+}
+
+}  // namespace
+
+// Counts of the MSD digit (key >> shift) & mask; clears the MSD pass's
+// look-back words.
+__global__ __launch_bounds__(kBlock) void MsdHistogramKernel(const uint32_t* __restrict__ keys, uint32_t n,
+                                                             uint32_t shift, uint32_t mask, uint32_t* __restrict__ hist,
+                                                             uint32_t* __restrict__ status, size_t status_words) {
+    __shared__ uint32_t s_h[kMsdRadix];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kMsdRadix; i += kBlock) s_h[i] = 0;
+    __syncthreads();
+    constexpr uint32_t kStep = kBlock * kHistKeysPerThread;
+    const uint32_t full = (reinterpret_cast<uintptr_t>(keys) & 15u) ? 0u : n / kStep;
+    for (uint32_t step = blockIdx.x; step < full; step += gridDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(keys + (size_t)step * kStep);
+        uint4 q[kHistKeysPerThread / 4];
+#pragma unroll
+        for (int j = 0; j < kHistKeysPerThread / 4; ++j) q[j] = src[j * kBlock + threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < kHistKeysPerThread / 4; ++j) {
+            atomicAdd(&s_h[(q[j].x >> shift) & mask], 1u);
+            atomicAdd(&s_h[(q[j].y >> shift) & mask], 1u);
+            atomicAdd(&s_h[(q[j].z >> shift) & mask], 1u);
+            atomicAdd(&s_h[(q[j].w >> shift) & mask], 1u);
+        }
+    }
+    for (uint32_t i = full * kStep + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
+    __syncthreads();
+    uint32_t* copy = HistCopyOfBlock(hist);
+    for (uint32_t i = threadIdx.x; i <= mask; i += kBlock) {
+        const uint32_t c = s_h[i];
+        if (c) atomicAdd(&copy[i], c);
+    }
+    for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < status_words; w += (size_t)gridDim.x * kBlock)
+        status[w] = 0;
+}
+
+// The MSD pass: as SortPassKernel with a digit of `bits` (<= 10) bits at
+// `shift`, one thread per digit, and keys and values staged one after the
+// other through one LDS buffer (the per-wave counts of 1,024 digits take
+// the other half of the LDS).  Tile 0 also writes the buckets' start
+// offsets (bucket_off[d], and n at bucket_off[1 << bits]) for the bucket
+// kernel.
+__global__ __launch_bounds__(kMsdBlock) __attribute__((amdgpu_waves_per_eu(4))) void MsdPassKernel(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift, int bits, const uint32_t* __restrict__ hist,
+    uint32_t* tickets, uint32_t* status, uint32_t* __restrict__ bucket_off, uint32_t spin_limit, uint32_t* err) {
+    constexpr int kWaves = kMsdBlock / 64;
+    constexpr int kTile = kMsdBlock * kMsdItems;
+    __shared__ uint32_t s_buf[kTile];
+    __shared__ uint32_t s_wave_hist[kWaves][kMsdRadix];
+    __shared__ uint32_t s_count[kMsdRadix];
+    __shared__ uint32_t s_digit_start[kMsdRadix];
+    __shared__ uint32_t s_global_off[kMsdRadix];
+    __shared__ uint32_t s_wave_tot[kWaves];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t mask = (1u << bits) - 1u;
+    if (tid == 0) s_tile = atomicAdd(tickets, 1u);
+    for (uint32_t i = tid; i < kWaves * kMsdRadix; i += kMsdBlock) (&s_wave_hist[0][0])[i] = 0;
+    s_count[tid] = 0;
+    uint32_t hist_d = 0;
+#pragma unroll
+    for (int c = 0; c < kSortHistCopies; ++c) hist_d += hist[c * kSortHistStride + tid];
+    if (tid > mask) hist_d = 0;
+    const uint32_t hist_scan = BlockExclusiveScan<kMsdBlock>(hist_d, s_wave_tot);  // contains __syncthreads
+    const uint32_t tile = s_tile;
+    if (tile == 0) {
+        if (tid <= mask) bucket_off[tid] = hist_scan;
+        if (tid == 0) bucket_off[mask + 1] = n;
+    }
+    const uint32_t base = tile * kTile + wave * (64 * kMsdItems) + lane;
+
+    uint32_t k[kMsdItems], rank[kMsdItems];
+    const bool full = (tile + 1) * kTile <= n;
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j) k[j] = keys_in[base + (uint32_t)j * 64];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j) {
+            const uint32_t idx = base + (uint32_t)j * 64;
+            k[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;  // padding: the last digit, ranked after every real key
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kMsdItems; ++j) atomicAdd(&s_count[(k[j] >> shift) & mask], 1u);
+    __syncthreads();
+
+    // Decoupled look-back for digit `tid` (digits above mask count zero).
+    {
+        const uint32_t count = s_count[tid];
+        uint32_t excl = 0;
+        if (tile == 0) {
+            AtomicStoreAgent(&status[tid], kFlagInclusive | count);
+        } else {
+            AtomicStoreAgent(&status[(size_t)tile * kMsdRadix + tid], kFlagAggregate | count);
+            int64_t p = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            bool done = false;
+            while (!done && p >= 0) {
+                uint32_t w[kLookback];
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q)
+                    w[q] = (p - q >= 0) ? AtomicLoadAgent(&status[(size_t)(p - q) * kMsdRadix + tid]) : kFlagInclusive;
+                int used = 0;
+                bool blocked = false;
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q) {
+                    if (done || blocked) continue;
+                    const uint32_t f = w[q] & ~kCountMask;
+                    if (f == 0) {
+                        blocked = true;
+                        continue;
+                    }
+                    excl += w[q] & kCountMask;
+                    used = q + 1;
+                    done = f == kFlagInclusive;
+                }
+                p -= used;
+                if (blocked) {
+                    if (++spins > spin_limit) {
+                        if (err) StG(err + kErrLookbackTimeout, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            AtomicStoreAgent(&status[(size_t)tile * kMsdRadix + tid], kFlagInclusive | (excl + count));
+        }
+        s_global_off[tid] = hist_scan + excl;
+    }
+
+    // Stable rank within each wave.
+#pragma unroll
+    for (int j = 0; j < kMsdItems; ++j) {
+        const uint32_t d = (k[j] >> shift) & mask;
+        const uint32_t pre = s_wave_hist[wave][d];
+        uint32_t m_lo, m_hi;
+        MatchDigit<kMsdBits>(d, bits, m_lo, m_hi);
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
+        rank[j] = pre + before;
+        __builtin_amdgcn_wave_barrier();
+        if (before == 0) s_wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    uint32_t count = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_wave_hist[w][tid];
+        s_wave_hist[w][tid] = count;
+        count += c;
+    }
+    const uint32_t dstart = BlockExclusiveScan<kMsdBlock>(count, s_wave_tot);  // contains __syncthreads
+    s_digit_start[tid] = dstart;
+    __syncthreads();
+    // Tile positions (the key's rank in the tile's stable digit order).
+#pragma unroll
+    for (int j = 0; j < kMsdItems; ++j) {
+        const uint32_t d = (k[j] >> shift) & mask;
+        rank[j] += s_digit_start[d] + s_wave_hist[wave][d];
+        s_buf[rank[j]] = k[j];
+    }
+    __syncthreads();
+    // Keys out, each digit's run contiguous; remember each slot's destination.
+    uint32_t dst[kMsdItems];
+#pragma unroll
+    for (int j = 0; j < kMsdItems; ++j) {
+        const uint32_t i = (uint32_t)j * kMsdBlock + tid;
+        const uint32_t key = s_buf[i];
+        const uint32_t d = (key >> shift) & mask;
+        dst[j] = s_global_off[d] + (i - s_digit_start[d]);
+        if (dst[j] < n) keys_out[dst[j]] = key;
+    }
+    // Values: loaded only now, through the same buffer.
+    uint32_t v[kMsdItems];
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j) v[j] = vals_in[base + (uint32_t)j * 64];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j) {
+            const uint32_t idx = base + (uint32_t)j * 64;
+            v[j] = idx < n ? vals_in[idx] : 0u;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kMsdItems; ++j) s_buf[rank[j]] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kMsdItems; ++j) {
+        const uint32_t i = (uint32_t)j * kMsdBlock + tid;
+        if (dst[j] < n) vals_out[dst[j]] = s_buf[i];
+    }
+}
+
+namespace {
+
+// LDS of the bucket kernel.
+struct BucketLds {
+    uint32_t buf[kBucketCapacity];
+    uint32_t wave_hist[kBucketBlock / 64][1 << kBucketRoundBits];
+    uint32_t digit_start[1 << kBucketRoundBits];
+    uint32_t run[1 << kBucketRoundBits];
+    uint32_t wave_tot[kBucketBlock / 64];
+    uint8_t slot_digit[kBucketCapacity];  // chunked path: digit of the key at each tile slot
+};
+
+// Tile positions are below kBucketCapacity < 2^16: two per register.
+__device__ __forceinline__ uint32_t GetPos(const uint32_t (&pk)[kBucketItems / 2], int j) {
+    return (pk[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+}
+__device__ __forceinline__ void SetPos(uint32_t (&pk)[kBucketItems / 2], int j, uint32_t v) {
+    if (j & 1) pk[j >> 1] = (pk[j >> 1] & 0xFFFFu) | (v << 16);
+    else pk[j >> 1] = (pk[j >> 1] & 0xFFFF0000u) | v;
+}
+
+// Stable ranks of `it` keys per thread (item j of wave w, lane l at tile
+// position w * 64 * it + j * 64 + l) by the digit (key >> sh) & ((1 << bits)
+// - 1): position j becomes the key's position in the tile's stable order.
+// Positions are packed two per register (GetPos / SetPos).  Returns the
+// tile's count of digit threadIdx.x (0 for threads past the digits);
+// lds.digit_start holds each digit's first position.
+__device__ __forceinline__ uint32_t RankBucketTile(const uint32_t (&k)[kBucketItems], uint32_t (&pk)[kBucketItems / 2],
+                                                  uint32_t it, uint32_t sh, int bits, BucketLds& lds) {
+    constexpr int kWaves = kBucketBlock / 64;
+    constexpr int kR = 1 << kBucketRoundBits;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    const uint32_t mask = (1u << bits) - 1u;
+    for (uint32_t i = tid; i < kWaves * kR; i += kBucketBlock) (&lds.wave_hist[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBucketItems; ++j) {
+        if ((uint32_t)j >= it) continue;
+        const uint32_t d = (k[j] >> sh) & mask;
+        const uint32_t pre = lds.wave_hist[wave][d];
+        uint32_t m_lo, m_hi;
+        MatchDigit<kBucketRoundBits>(d, bits, m_lo, m_hi);
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
+        SetPos(pk, j, pre + before);
+        __builtin_amdgcn_wave_barrier();
+        if (before == 0) lds.wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    uint32_t count = 0;
+    if (tid <= mask) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = lds.wave_hist[w][tid];
+            lds.wave_hist[w][tid] = count;
+            count += c;
+        }
+    }
+    const uint32_t dstart = BlockExclusiveScan<kBucketBlock>(count, lds.wave_tot);  // contains __syncthreads
+    if (tid <= mask) lds.digit_start[tid] = dstart;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBucketItems; ++j) {
+        if ((uint32_t)j >= it) continue;
+        const uint32_t d = (k[j] >> sh) & mask;
+        SetPos(pk, j, GetPos(pk, j) + lds.digit_start[d] + lds.wave_hist[wave][d]);
+    }
+    return count;
+}
+
+// Digit width of round r of `rounds` over `bits` low bits (balanced).
+__device__ __forceinline__ int RoundBits(uint32_t bits, int rounds, int r, uint32_t done) {
+    const uint32_t rest = bits - done;
+    return (int)((rest + (uint32_t)(rounds - r) - 1) / (uint32_t)(rounds - r));
+}
+
+}  // namespace
+
+// One workgroup per MSD bucket: sort the bucket's pairs (read from keys_in /
+// vals_in, where the MSD pass put them) by their low `low_bits` bits into
+// keys_out / vals_out.
+__global__ __launch_bounds__(kBucketBlock) void BucketSortKernel(uint32_t* __restrict__ keys_in,
+                                                                 uint32_t* __restrict__ vals_in,
+                                                                 uint32_t* __restrict__ keys_out,
+                                                                 uint32_t* __restrict__ vals_out,
+                                                                 const uint32_t* __restrict__ bucket_off,
+                                                                 uint32_t low_bits) {
+    __shared__ BucketLds lds;
+    const uint32_t start = bucket_off[blockIdx.x];
+    const uint32_t size = bucket_off[blockIdx.x + 1] - start;
+    if (size == 0) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const int rounds = (int)((low_bits + kBucketRoundBits - 1) / kBucketRoundBits);
+    uint32_t k[kBucketItems], v[kBucketItems], pk[kBucketItems / 2];
+
+    if (size <= kBucketCapacity) {
+        // Whole bucket in registers; each LDS round reorders it stably.
+        const uint32_t it = (size + kBucketBlock - 1) / kBucketBlock;
+        const uint32_t wbase = wave * 64u * it + lane;
+#pragma unroll
+        for (int j = 0; j < kBucketItems; ++j) {
+            if ((uint32_t)j >= it) continue;
+            const uint32_t p = wbase + (uint32_t)j * 64;
+            const bool ok = p < size;
+            k[j] = ok ? keys_in[start + p] : 0xFFFFFFFFu;  // padding ranks after every real key in every round
+            v[j] = ok ? vals_in[start + p] : 0u;
+        }
+        uint32_t sh = 0;
+        for (int r = 0; r < rounds; ++r) {
+            const int bits = RoundBits(low_bits, rounds, r, sh);
+            RankBucketTile(k, pk, it, sh, bits, lds);
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                lds.buf[GetPos(pk, j)] = k[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                k[j] = lds.buf[wbase + (uint32_t)j * 64];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                lds.buf[GetPos(pk, j)] = v[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                v[j] = lds.buf[wbase + (uint32_t)j * 64];
+            }
+            __syncthreads();
+            sh += (uint32_t)bits;
+        }
+#pragma unroll
+        for (int j = 0; j < kBucketItems; ++j) {
+            if ((uint32_t)j >= it) continue;
+            const uint32_t p = wbase + (uint32_t)j * 64;
+            if (p < size) {
+                keys_out[start + p] = k[j];
+                vals_out[start + p] = v[j];
+            }
+        }
+        return;
+    }
+
+    // A bucket larger than the LDS: LSD rounds through memory, each over
+    // the bucket in capacity-sized chunks (stable: chunks in order, ranks
+    // within a chunk stable, each digit's output advancing chunk by chunk).
+    // Rounds alternate between the two arrays; the last lands in keys_out.
+    uint32_t *sk = keys_in + start, *sv = vals_in + start, *dk = keys_out + start, *dv = vals_out + start;
+    const int passes = rounds + ((rounds & 1) ? 0 : 1);  // odd, so the data ends in keys_out
+    uint32_t sh = 0;
+    for (int r = 0; r < passes; ++r) {
+        const int bits = r < rounds ? RoundBits(low_bits, rounds, r, sh) : 0;  // a 0-bit round copies
+        const uint32_t mask = (1u << bits) - 1u;
+        for (uint32_t i = tid; i < (1u << kBucketRoundBits); i += kBucketBlock) lds.run[i] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < size; i += kBucketBlock) atomicAdd(&lds.run[(sk[i] >> sh) & mask], 1u);
+        __syncthreads();
+        const uint32_t c = tid <= mask ? lds.run[tid] : 0u;
+        const uint32_t excl = BlockExclusiveScan<kBucketBlock>(c, lds.wave_tot);  // contains __syncthreads
+        if (tid <= mask) lds.run[tid] = excl;
+        __syncthreads();
+        for (uint32_t cbase = 0; cbase < size; cbase += kBucketCapacity) {
+            const uint32_t csize = min(kBucketCapacity, size - cbase);
+            const uint32_t it = (csize + kBucketBlock - 1) / kBucketBlock;
+            const uint32_t wbase = wave * 64u * it + lane;
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                const uint32_t p = wbase + (uint32_t)j * 64;
+                const bool ok = p < csize;
+                k[j] = ok ? sk[cbase + p] : 0xFFFFFFFFu;
+                v[j] = ok ? sv[cbase + p] : 0u;
+            }
+            const uint32_t cnt = RankBucketTile(k, pk, it, sh, bits, lds);
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                lds.buf[GetPos(pk, j)] = k[j];
+            }
+            __syncthreads();
+            // Keys out by tile slot (slot i = tid + j * block; padding, ranked
+            // last, falls at or past csize and is not written); each slot's
+            // digit is kept for the values.
+            for (uint32_t i = tid; i < csize; i += kBucketBlock) {
+                const uint32_t key = lds.buf[i];
+                const uint32_t d = (key >> sh) & mask;
+                lds.slot_digit[i] = (uint8_t)d;
+                dk[lds.run[d] + (i - lds.digit_start[d])] = key;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kBucketItems; ++j) {
+                if ((uint32_t)j >= it) continue;
+                lds.buf[GetPos(pk, j)] = v[j];
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < csize; i += kBucketBlock) {
+                const uint32_t d = lds.slot_digit[i];
+                dv[lds.run[d] + (i - lds.digit_start[d])] = lds.buf[i];
+            }
+            __syncthreads();
+            if (tid <= mask) lds.run[tid] += cnt;
+            __syncthreads();
+        }
+        uint32_t* t = sk;
+        sk = dk;
+        dk = t;
+        t = sv;
+        sv = dv;
+        dv = t;
+        sh += (uint32_t)bits;
+    }
+}
+
 // Tile shapes the pass kernel is instantiated for.
 static bool KnownShape(int block, int items, int bits) {
     if (bits == kSortWideBits) return block == 1024 && (items == 8 || items == 4);
@@ -359,6 +820,26 @@ static size_t StatusWords(uint32_t n, const SortShape& s) {
     return (size_t)PassesOf(32, s) * TilesOf(n, s) * ((size_t)1 << s.bits);
 }
 
+static std::atomic<int> g_hybrid{1};
+bool SortHybrid() { return g_hybrid.load(std::memory_order_relaxed) != 0; }
+void SetSortHybrid(bool on) { g_hybrid.store(on ? 1 : 0, std::memory_order_relaxed); }
+
+// MSD digit width of the hybrid sort for n keys of key_bits bits, or 0 when
+// the LSD passes sort them.  The digit is as wide as gives buckets of about
+// 4,096 keys on uniform keys, up to 10 bits (16,384 keys at 2^24 keys:
+// well inside the 24,576 of one workgroup's LDS sort for such keys).
+static int HybridBits(uint32_t n, int key_bits, bool hist_ready) {
+    if (hist_ready || !SortHybrid() || n < kSortHybridMinKeys || n > kSortHybridMaxKeys || key_bits < 2) return 0;
+    int lg = 0;
+    while (lg < 31 && (1u << lg) < n) ++lg;
+    return std::max(1, std::min({lg - 12, kMsdBits, key_bits}));
+}
+
+static size_t MsdTiles(uint32_t n) { return ((size_t)n + kMsdBlock * kMsdItems - 1) / (kMsdBlock * kMsdItems); }
+
+// Words before the look-back words: histogram copies, tickets, bucket offsets.
+constexpr size_t kSortHeadWords = (size_t)kSortHistCopies * kSortHistStride + 64 + (kMsdRadix + 1 + 63) / 64 * 64;
+
 size_t SortScratchBytes(uint32_t n) {
     // Enough for any shape a sort of n keys may take, whatever the digit
     // width is set to when it runs.
@@ -367,8 +848,8 @@ size_t SortScratchBytes(uint32_t n) {
         status = std::max(status, StatusWords(n, SortShape{1024, 8, kSortWideBits}));
         status = std::max(status, StatusWords(n, SortShape{1024, 16, kSortNarrowBits}));
     }
-    const size_t words = (size_t)kSortHistCopies * kSortHistStride + 64 + status;
-    return AlignUp(words * 4, 256);
+    if (n >= kSortHybridMinKeys && n <= kSortHybridMaxKeys) status = std::max(status, MsdTiles(n) * kMsdRadix);
+    return AlignUp((kSortHeadWords + status) * 4, 256);
 }
 
 SortScratch CarveSortScratch(void* base, uint32_t n) {
@@ -376,7 +857,8 @@ SortScratch CarveSortScratch(void* base, uint32_t n) {
     uint32_t* p = static_cast<uint32_t*>(base);
     s.hist = p;
     s.tickets = p + kSortHistCopies * kSortHistStride;
-    s.status = s.tickets + 64;
+    s.bucket_off = s.tickets + 64;
+    s.status = p + kSortHeadWords;
     // Pass 0's look-back words for 8-bit digits (what callers that build the
     // histogram themselves clear); each pass clears the next pass's.
     const SortShape narrow = ChooseShape(n, false);
@@ -388,9 +870,31 @@ void LaunchSortClear(const SortScratch& sc, hipStream_t stream) {
     SortClearKernel<<<1, kBlock, 0, stream>>>(sc.hist, sc.tickets, kSortHistWords);
 }
 
+// The hybrid sort: MSD pass into the tmp arrays, then the bucket kernel
+// back into keys/vals.
+static void HybridSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
+                       int key_bits, int bits, const SortScratch& sc, hipStream_t stream) {
+    const uint32_t shift = (uint32_t)(key_bits - bits);
+    const uint32_t mask = (1u << bits) - 1u;
+    const size_t tiles = MsdTiles(n);
+    SortClearKernel<<<(kSortHistCopies * kMsdRadix + kBlock - 1) / kBlock, kBlock, 0, stream>>>(sc.hist, sc.tickets,
+                                                                                               kMsdRadix);
+    const uint32_t blocks =
+        (uint32_t)std::max<size_t>(1, std::min<size_t>(1024, (size_t)n / (kBlock * kHistKeysPerThread)));
+    MsdHistogramKernel<<<blocks, kBlock, 0, stream>>>(keys, n, shift, mask, sc.hist, sc.status, tiles * kMsdRadix);
+    MsdPassKernel<<<(uint32_t)tiles, kMsdBlock, 0, stream>>>(keys, vals, keys_tmp, vals_tmp, n, shift, bits, sc.hist,
+                                                             sc.tickets, sc.status, sc.bucket_off, SortSpinLimit(),
+                                                             DeviceErrorWords());
+    BucketSortKernel<<<1u << bits, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift);
+}
+
 bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
                     const SortScratch& sc, bool hist_ready, hipStream_t stream) {
     if (n <= 1) return true;
+    if (const int bits = HybridBits(n, key_bits, hist_ready)) {
+        HybridSort(keys, vals, keys_tmp, vals_tmp, n, key_bits, bits, sc, stream);
+        return true;
+    }
     const SortShape shape = ChooseShape(n, !hist_ready);
     const int passes = PassesOf(key_bits, shape);
     const size_t tiles = TilesOf(n, shape);

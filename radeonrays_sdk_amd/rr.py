@@ -9,10 +9,13 @@ the helpers in radeonrays_sdk_amd.api take them from torch tensors.
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import POINTER, Structure, c_char_p, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libradeonrays.so"
+# RADEONRAYS_LIB names another build of the library (the host-sanitizer
+# build, libradeonrays_asan.so, in the sanitizer test).
+LIB_PATH = Path(os.environ.get("RADEONRAYS_LIB") or Path(__file__).resolve().parent / "libradeonrays.so")
 
 # --- enums (radeonrays.h) ---------------------------------------------------
 RR_SUCCESS = 0
@@ -57,6 +60,7 @@ RRX_TUNING_TRACE_PLAIN = 3
 RRX_TUNING_SORT_DIGIT_BITS = 4
 RRX_TUNING_TREELET_LEAVES = 5
 RRX_TUNING_SORT_SPIN_LIMIT = 6
+RRX_TUNING_SORT_HYBRID = 7
 
 RRX_DEVICE_ERROR_STACK_OVERFLOW = 1 << 0
 RRX_DEVICE_ERROR_SORT_LOOKBACK = 1 << 1

@@ -99,6 +99,7 @@ def _check_blas(ctx, dev, v, i, rays, restructure, tag):
     assert ok, f"{tag}: {msg}"
     assert oracle.is_valid(buf) and oracle.check_consistency(buf)
     drays = to_dev(rays, dev)
+    closest = None
     for query in ("closest", "any"):
         hits = ctx.intersect(accel, drays, query, "full_t")
         torch.cuda.synchronize()
@@ -107,6 +108,7 @@ def _check_blas(ctx, dev, v, i, rays, restructure, tag):
         ok, msg = hits_equal(g, r)
         assert ok, f"{tag} {query}: {msg}"
         if query == "closest":
+            closest = g
             bf = oracle.brute_force(ref, rays)
             ok, msg = hits_equal(g, bf)
             assert ok, f"{tag} brute force: {msg}"
@@ -114,7 +116,7 @@ def _check_blas(ctx, dev, v, i, rays, restructure, tag):
     torch.cuda.synchronize()
     # Single level: the instance-id output is the primitive id (isect.comp:235).
     assert np.array_equal(api.decode_hits(ids, "instance")["id"], oracle.trace(ref, rays)["prim_id"])
-    return accel, ref, g
+    return accel, ref, closest
 
 
 @pytest.mark.parametrize("scale", [1.0, 20.0])
