@@ -669,12 +669,15 @@ def main():
         autotune.apply(ctx.lib, start)  # the other sections run with the defaults
         out["autotune"] = {"default": t["default"], "best": t["best"], "configs": len(t["table"]),
                            "note": "hits identical under every configuration"}
-        log("autotune sweep (sort digit width, 16M keys)")
+        log("autotune sweep (sort variants, 16M keys)")
         start_bits = autotune.get(ctx.lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS)
+        start_hybrid = autotune.get(ctx.lib, autotune.rr.RRX_TUNING_SORT_HYBRID)
         ts = autotune.tune_sort(ctx, 16 << 20, reps=3)
         autotune.set_(ctx.lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start_bits)
-        out["autotune"]["sort_digit_bits"] = {"table": ts["table"], "best": ts["best"]["digit_bits"],
-                                              "note": "identical sorted keys and values for every width"}
+        autotune.set_(ctx.lib, autotune.rr.RRX_TUNING_SORT_HYBRID, start_hybrid)
+        out["autotune"]["sort_variants"] = {"table": ts["table"], "best": ts["best"],
+                                            "note": "identical sorted keys and values for every variant "
+                                                    "(LSD digit width; hybrid = MSD pass + per-bucket LDS sort)"}
         log("autotune sweep (treelet size)")
         start_leaves = autotune.get(ctx.lib, autotune.rr.RRX_TUNING_TREELET_LEAVES)
         tt = autotune.tune_treelet(ctx, v, idx, rays)

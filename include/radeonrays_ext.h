@@ -72,13 +72,13 @@ typedef enum RRxTuning
                                              one unpublished (default 2^22; 0 gives up at the
                                              first).  A tile that gives up sorts wrongly and the
                                              failure is reported (rrxGetDeviceErrors) */
-    RRX_TUNING_SORT_HYBRID = 7            /* 1 (default): rrxCmdSortPairs on 2^22..2^24 keys takes
-                                             one 10-bit most-significant-digit pass and then sorts
-                                             each bucket in one workgroup's LDS (two passes over
-                                             the data instead of four); a bucket of more than
-                                             24,576 keys is sorted by its workgroup through memory,
-                                             so keys concentrated in a few buckets sort slower.
-                                             0: the LSD passes.  Same order either way */
+    RRX_TUNING_SORT_HYBRID = 7            /* 1: rrxCmdSortPairs on 2^22..2^24 keys takes one
+                                             10-bit most-significant-digit pass and then sorts each
+                                             bucket in one workgroup's LDS (two passes over the data
+                                             instead of four); a bucket of more than 20,480 keys is
+                                             sorted by its workgroup through memory, so keys
+                                             concentrated in a few buckets sort slower.  0 (default):
+                                             the LSD passes.  Same order either way */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

@@ -7,17 +7,16 @@ threshold (lanes still traversing below which a wave fetches new rays),
 checks that every combination returns the same hits as the current setting,
 and leaves the fastest one set.
 
-The sort's radix digit width is swept by tune_sort: 8-bit digits (four
-passes over 30-bit Morton codes) or 10-bit digits (three passes, 1024-bin
-tiles), on a caller-given number of keys; both give the same stable order.
+The sort's variants are swept by tune_sort on a caller-given number of
+keys: the LSD passes with 4-, 8- or 10-bit digits (eight, four or three
+passes over 30-bit Morton codes) and, on 2^22..2^24 keys, the hybrid (one
+most-significant-digit pass, then every bucket sorted in LDS); all give the
+same stable order.
 
 The treelet size of the restructuring (5, 7 or 9 leaves) is swept by
 tune_treelet: it trades build time (the subset dynamic programme grows as
 3^leaves) against tree quality (traversal time); closest hits are the same
 for every size.
-
-Not swept, with the reason: 4-bit digits (they double the passes over
-memory of a sort that is bandwidth-bound).
 """
 from __future__ import annotations
 
@@ -122,13 +121,15 @@ DIGIT_BITS = (4, 8, 10)
 
 
 def tune_sort(ctx, n: int = 16 << 20, reps: int = 5, digit_bits=DIGIT_BITS, seed: int = 0x52520005) -> dict:
-    """Sweep the radix digit width on a sort of n random 30-bit keys with
-    index values (rrxCmdSortPairs); returns {"best", "default", "table"} and
-    leaves the fastest width set.  Raises if any width's output differs."""
+    """Sweep the sort's variants on n random 30-bit keys with index values
+    (rrxCmdSortPairs): the LSD passes with each digit width, and the hybrid
+    (one most-significant-digit pass, then each bucket sorted in LDS; only
+    on 2^22..2^24 keys).  Returns {"best", "default", "table"} and leaves the
+    fastest variant set.  Raises if any variant's output differs."""
     import torch
 
     lib = ctx.lib
-    start = get(lib, rr.RRX_TUNING_SORT_DIGIT_BITS)
+    start = (get(lib, rr.RRX_TUNING_SORT_DIGIT_BITS), get(lib, rr.RRX_TUNING_SORT_HYBRID))
     dev = ctx.torch_device
     g = torch.Generator(device="cpu").manual_seed(seed)
     keys = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, generator=g).to(dev)
@@ -152,24 +153,29 @@ def tune_sort(ctx, n: int = 16 << 20, reps: int = 5, digit_bits=DIGIT_BITS, seed
         torch.cuda.synchronize()
         return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
+    def restore(setting):
+        set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, setting[0])
+        set_(lib, rr.RRX_TUNING_SORT_HYBRID, setting[1])
+
     copy_ms = timed(lambda: (k2.copy_(keys), v2.copy_(vals)))
+    variants = [(bits, 0) for bits in digit_bits] + ([(8, 1)] if (1 << 22) <= n <= (1 << 24) else [])
     reference = None
     table = []
-    for bits in digit_bits:
-        set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, bits)
+    for bits, hybrid in variants:
+        restore((bits, hybrid))
         run()
         torch.cuda.synchronize()
         got = (k2.cpu().numpy(), v2.cpu().numpy())
         if reference is None:
             reference = got
         elif not (np.array_equal(got[0], reference[0]) and np.array_equal(got[1], reference[1])):
-            set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, start)
-            raise RuntimeError(f"sort output differs with {bits}-bit digits")
+            restore(start)
+            raise RuntimeError(f"sort output differs with {bits}-bit digits, hybrid={hybrid}")
         ms = timed(run) - copy_ms
-        table.append({"digit_bits": bits, "ms": round(ms, 4)})
+        table.append({"digit_bits": bits, "hybrid": hybrid, "ms": round(ms, 4)})
     best = min(table, key=lambda r: r["ms"])
-    default = next((r for r in table if r["digit_bits"] == start), None)
-    set_(lib, rr.RRX_TUNING_SORT_DIGIT_BITS, best["digit_bits"])
+    default = next((r for r in table if (r["digit_bits"], r["hybrid"]) == start), None)
+    restore((best["digit_bits"], best["hybrid"]))
     return {"best": best, "default": default, "table": table}
 
 

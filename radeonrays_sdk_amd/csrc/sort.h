@@ -54,10 +54,11 @@ constexpr uint32_t kSortDefaultSpinLimit = 1u << 22;
 uint32_t SortSpinLimit();
 void SetSortSpinLimit(uint32_t polls);
 
-// Sorts of kSortHybridMinKeys..kSortHybridMaxKeys pairs whose caller did not
-// prepare the histogram take one most-significant-digit pass (up to 10
-// bits) and then sort each bucket in one workgroup's LDS, instead of the
-// LSD passes (process-wide switch; the order is the same either way).
+// With the hybrid switch on, sorts of kSortHybridMinKeys..kSortHybridMaxKeys
+// pairs whose caller did not prepare the histogram take one
+// most-significant-digit pass (up to 10 bits) and then sort each bucket in
+// one workgroup's LDS, instead of the LSD passes (process-wide; off by
+// default; the order is the same either way).
 constexpr uint32_t kSortHybridMinKeys = 1u << 22;
 constexpr uint32_t kSortHybridMaxKeys = 1u << 24;
 bool SortHybrid();

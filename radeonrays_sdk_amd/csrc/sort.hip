@@ -319,7 +319,7 @@ constexpr int kMsdRadix = 1 << kMsdBits;
 constexpr int kMsdBlock = 1024;  // one thread per digit
 constexpr int kMsdItems = 16;    // 16,384-key tiles
 constexpr int kBucketBlock = 1024;
-constexpr int kBucketItems = 24;
+constexpr int kBucketItems = 20;
 constexpr uint32_t kBucketCapacity = kBucketBlock * kBucketItems;
 constexpr int kBucketRoundBits = 8;  // widest digit of one LDS round
 static_assert(kMsdBlock == kMsdRadix, "the MSD pass gives each thread one digit");
@@ -620,76 +620,154 @@ __device__ __forceinline__ int RoundBits(uint32_t bits, int rounds, int r, uint3
 
 }  // namespace
 
-// One workgroup per MSD bucket: sort the bucket's pairs (read from keys_in /
-// vals_in, where the MSD pass put them) by their low `low_bits` bits into
-// keys_out / vals_out.
-__global__ __launch_bounds__(kBucketBlock) void BucketSortKernel(uint32_t* __restrict__ keys_in,
-                                                                 uint32_t* __restrict__ vals_in,
-                                                                 uint32_t* __restrict__ keys_out,
-                                                                 uint32_t* __restrict__ vals_out,
-                                                                 const uint32_t* __restrict__ bucket_off,
-                                                                 uint32_t low_bits) {
+namespace {
+
+// One stable LSD round over a bucket held in registers, ITEMS pairs per
+// thread (item j of wave w, lane l at bucket position w * 64 * ITEMS + j * 64
+// + l), by the digit (key >> sh) & mask of at most BITS bits: rank within
+// each wave by ballots, offsets per digit across waves and digits, then keys
+// and values reordered through LDS, after which each thread again holds the
+// items at its positions, now in the round's order.  In the last round
+// (out_k not null) the reordered keys and values go straight from LDS to
+// out_k / out_v, `size` of each.
+template <int ITEMS, int BITS>
+__device__ __forceinline__ void BucketRound(uint32_t (&k)[ITEMS], uint32_t (&v)[ITEMS], uint32_t sh, uint32_t mask,
+                                            BucketLds& lds, uint32_t* __restrict__ out_k, uint32_t* __restrict__ out_v,
+                                            uint32_t size) {
+    constexpr int kWaves = kBucketBlock / 64;
+    constexpr int kR = 1 << BITS;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t wbase = wave * 64u * ITEMS + lane;
+    for (uint32_t i = tid; i < kWaves * kR; i += kBucketBlock) lds.wave_hist[i / kR][i % kR] = 0;
+    __syncthreads();
+    uint32_t pos[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = (k[j] >> sh) & mask;
+        uint32_t m_lo = ~0u, m_hi = ~0u;
+#pragma unroll
+        for (int b = 0; b < BITS; ++b) {
+            const uint32_t neg = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+            const uint64_t bal = __ballot(neg != 0u);
+            m_lo &= ~((uint32_t)bal ^ neg);
+            m_hi &= ~((uint32_t)(bal >> 32) ^ neg);
+        }
+        const uint32_t pre = lds.wave_hist[wave][d];
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi(m_hi, __builtin_amdgcn_mbcnt_lo(m_lo, 0u));
+        pos[j] = pre + before;
+        __builtin_amdgcn_wave_barrier();
+        if (before == 0) lds.wave_hist[wave][d] = pre + (uint32_t)(__popc(m_lo) + __popc(m_hi));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    uint32_t count = 0;
+    if (tid < (uint32_t)kR) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = lds.wave_hist[w][tid];
+            lds.wave_hist[w][tid] = count;
+            count += c;
+        }
+    }
+    const uint32_t dstart = BlockExclusiveScan<kBucketBlock>(count, lds.wave_tot);  // contains __syncthreads
+    if (tid < (uint32_t)kR) lds.digit_start[tid] = dstart;
+    __syncthreads();
+    // Each key's digit is computed again below rather than kept from the
+    // ranking loop (with its LDS address, two more registers per key): an
+    // empty asm statement hides from the compiler that the keys are unchanged.
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) asm volatile("" : "+v"(k[j]));
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = (k[j] >> sh) & mask;
+        pos[j] += lds.digit_start[d] + lds.wave_hist[wave][d];
+        lds.buf[pos[j]] = k[j];
+    }
+    __syncthreads();
+    // One base address per array, items at constant offsets from it.
+    const uint32_t* kb = lds.buf + wbase;
+    if (out_k) {
+        for (uint32_t i = tid; i < size; i += kBucketBlock) out_k[i] = lds.buf[i];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) k[j] = kb[j * 64];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) lds.buf[pos[j]] = v[j];
+    __syncthreads();
+    if (out_k) {
+        for (uint32_t i = tid; i < size; i += kBucketBlock) out_v[i] = lds.buf[i];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) v[j] = kb[j * 64];
+    }
+    __syncthreads();
+}
+
+// A bucket of at most 1024 * ITEMS pairs: load (padding with all-ones keys,
+// which rank after every real key in every round), the LSD rounds, store.
+template <int ITEMS>
+__device__ __forceinline__ void SortBucketInLds(const uint32_t* __restrict__ keys_in,
+                                                const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
+                                                uint32_t* __restrict__ vals_out, uint32_t start, uint32_t size,
+                                                uint32_t low_bits, BucketLds& lds) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t wbase = wave * 64u * ITEMS + lane;
+    uint32_t k[ITEMS], v[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t p = wbase + (uint32_t)j * 64;
+        const bool ok = p < size;
+        k[j] = ok ? keys_in[start + p] : 0xFFFFFFFFu;
+        v[j] = ok ? vals_in[start + p] : 0u;
+    }
+    // At least one round (a 0-bit one when there are no low bits: a copy).
+    const int rounds = (int)max(1u, (low_bits + kBucketRoundBits - 1) / kBucketRoundBits);
+    uint32_t sh = 0;
+    for (int r = 0; r < rounds; ++r) {
+        const int bits = RoundBits(low_bits, rounds, r, sh);
+        const uint32_t mask = (1u << bits) - 1u;
+        uint32_t* ok = r + 1 == rounds ? keys_out + start : nullptr;
+        BucketRound<ITEMS, kBucketRoundBits>(k, v, sh, mask, lds, ok, vals_out + start, size);
+        sh += (uint32_t)bits;
+    }
+}
+
+}  // namespace
+
+// One workgroup per MSD bucket of at most 1024 * ITEMS pairs: sort the
+// bucket's pairs (read from keys_in / vals_in, where the MSD pass put them)
+// by their low `low_bits` bits into keys_out / vals_out, in LDS.  Larger
+// buckets are left to BucketSortChunkedKernel.
+template <int ITEMS>
+__global__ __launch_bounds__(kBucketBlock) void BucketSortLdsKernel(const uint32_t* __restrict__ keys_in,
+                                                                    const uint32_t* __restrict__ vals_in,
+                                                                    uint32_t* __restrict__ keys_out,
+                                                                    uint32_t* __restrict__ vals_out,
+                                                                    const uint32_t* __restrict__ bucket_off,
+                                                                    uint32_t low_bits) {
     __shared__ BucketLds lds;
     const uint32_t start = bucket_off[blockIdx.x];
     const uint32_t size = bucket_off[blockIdx.x + 1] - start;
-    if (size == 0) return;
+    if (size == 0 || size > kBucketBlock * ITEMS) return;
+    SortBucketInLds<ITEMS>(keys_in, vals_in, keys_out, vals_out, start, size, low_bits, lds);
+}
+
+// One workgroup per MSD bucket larger than `cap` pairs (skewed keys).
+__global__ __launch_bounds__(kBucketBlock) void BucketSortChunkedKernel(uint32_t* __restrict__ keys_in,
+                                                                        uint32_t* __restrict__ vals_in,
+                                                                        uint32_t* __restrict__ keys_out,
+                                                                        uint32_t* __restrict__ vals_out,
+                                                                        const uint32_t* __restrict__ bucket_off,
+                                                                        uint32_t low_bits, uint32_t cap) {
+    __shared__ BucketLds lds;
+    const uint32_t start = bucket_off[blockIdx.x];
+    const uint32_t size = bucket_off[blockIdx.x + 1] - start;
+    if (size <= cap) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const int rounds = (int)((low_bits + kBucketRoundBits - 1) / kBucketRoundBits);
     uint32_t k[kBucketItems], v[kBucketItems], pk[kBucketItems / 2];
-
-    if (size <= kBucketCapacity) {
-        // Whole bucket in registers; each LDS round reorders it stably.
-        const uint32_t it = (size + kBucketBlock - 1) / kBucketBlock;
-        const uint32_t wbase = wave * 64u * it + lane;
-#pragma unroll
-        for (int j = 0; j < kBucketItems; ++j) {
-            if ((uint32_t)j >= it) continue;
-            const uint32_t p = wbase + (uint32_t)j * 64;
-            const bool ok = p < size;
-            k[j] = ok ? keys_in[start + p] : 0xFFFFFFFFu;  // padding ranks after every real key in every round
-            v[j] = ok ? vals_in[start + p] : 0u;
-        }
-        uint32_t sh = 0;
-        for (int r = 0; r < rounds; ++r) {
-            const int bits = RoundBits(low_bits, rounds, r, sh);
-            RankBucketTile(k, pk, it, sh, bits, lds);
-#pragma unroll
-            for (int j = 0; j < kBucketItems; ++j) {
-                if ((uint32_t)j >= it) continue;
-                lds.buf[GetPos(pk, j)] = k[j];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kBucketItems; ++j) {
-                if ((uint32_t)j >= it) continue;
-                k[j] = lds.buf[wbase + (uint32_t)j * 64];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kBucketItems; ++j) {
-                if ((uint32_t)j >= it) continue;
-                lds.buf[GetPos(pk, j)] = v[j];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kBucketItems; ++j) {
-                if ((uint32_t)j >= it) continue;
-                v[j] = lds.buf[wbase + (uint32_t)j * 64];
-            }
-            __syncthreads();
-            sh += (uint32_t)bits;
-        }
-#pragma unroll
-        for (int j = 0; j < kBucketItems; ++j) {
-            if ((uint32_t)j >= it) continue;
-            const uint32_t p = wbase + (uint32_t)j * 64;
-            if (p < size) {
-                keys_out[start + p] = k[j];
-                vals_out[start + p] = v[j];
-            }
-        }
-        return;
-    }
 
     // A bucket larger than the LDS: LSD rounds through memory, each over
     // the bucket in capacity-sized chunks (stable: chunks in order, ranks
@@ -820,7 +898,7 @@ static size_t StatusWords(uint32_t n, const SortShape& s) {
     return (size_t)PassesOf(32, s) * TilesOf(n, s) * ((size_t)1 << s.bits);
 }
 
-static std::atomic<int> g_hybrid{1};
+static std::atomic<int> g_hybrid{0};
 bool SortHybrid() { return g_hybrid.load(std::memory_order_relaxed) != 0; }
 void SetSortHybrid(bool on) { g_hybrid.store(on ? 1 : 0, std::memory_order_relaxed); }
 
@@ -885,7 +963,21 @@ static void HybridSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint3
     MsdPassKernel<<<(uint32_t)tiles, kMsdBlock, 0, stream>>>(keys, vals, keys_tmp, vals_tmp, n, shift, bits, sc.hist,
                                                              sc.tickets, sc.status, sc.bucket_off, SortSpinLimit(),
                                                              DeviceErrorWords());
-    BucketSortKernel<<<1u << bits, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift);
+    // Buckets average n >> bits pairs; the LDS kernel takes those of up to
+    // 1024 * items (items: that average plus a quarter, in steps of four),
+    // the chunked kernel the rest.
+    const uint32_t avg = (uint32_t)((n >> bits) + 1);
+    const uint32_t items = std::min<uint32_t>(kBucketItems, (avg + avg / 4 + 4 * kBucketBlock - 1) / (4 * kBucketBlock) * 4);
+    const uint32_t buckets = 1u << bits;
+    switch (items) {
+        case 4: BucketSortLdsKernel<4><<<buckets, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift); break;
+        case 8: BucketSortLdsKernel<8><<<buckets, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift); break;
+        case 12: BucketSortLdsKernel<12><<<buckets, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift); break;
+        case 16: BucketSortLdsKernel<16><<<buckets, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift); break;
+        default: BucketSortLdsKernel<20><<<buckets, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift); break;
+    }
+    BucketSortChunkedKernel<<<buckets, kBucketBlock, 0, stream>>>(keys_tmp, vals_tmp, keys, vals, sc.bucket_off, shift,
+                                                                  kBucketBlock * std::max<uint32_t>(4, items));
 }
 
 bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,

@@ -135,10 +135,10 @@ def test_tuning_validation():
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, 1) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_PLAIN, byref(v)) == rr.RR_SUCCESS and v.value == 1
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, before) == rr.RR_SUCCESS
-    assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_HYBRID, byref(v)) == rr.RR_SUCCESS and v.value == 1
-    assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_HYBRID, 0) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_HYBRID, byref(v)) == rr.RR_SUCCESS and v.value == 0
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_HYBRID, 1) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_HYBRID, byref(v)) == rr.RR_SUCCESS and v.value == 1
+    assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_HYBRID, 0) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 8
     for bad in (0, 6, 9, 16):
         assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, bad) == rr.RR_ERROR_INVALID_PARAMETER

@@ -27,16 +27,21 @@ def test_tuning_sweep_keeps_results(ctx, dev):
         autotune.apply(ctx.lib, start)
 
 
-def test_sort_digit_width_sweep_keeps_results(ctx, dev):
-    say("autotune sort digit width on 2M keys")
+@pytest.mark.parametrize("n", [2 << 20, 4 << 20])
+def test_sort_variant_sweep_keeps_results(ctx, dev, n):
+    say(f"autotune sort variants on {n} keys")
     lib = ctx.lib
-    start = autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS)
+    start = (autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS),
+             autotune.get(lib, autotune.rr.RRX_TUNING_SORT_HYBRID))
     try:
-        out = autotune.tune_sort(ctx, 2 << 20, reps=2)
-        assert [r["digit_bits"] for r in out["table"]] == [4, 8, 10]
+        out = autotune.tune_sort(ctx, n, reps=2)
+        want = [(4, 0), (8, 0), (10, 0)] + ([(8, 1)] if n >= 1 << 22 else [])
+        assert [(r["digit_bits"], r["hybrid"]) for r in out["table"]] == want
         assert autotune.get(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS) == out["best"]["digit_bits"]
+        assert autotune.get(lib, autotune.rr.RRX_TUNING_SORT_HYBRID) == out["best"]["hybrid"]
     finally:
-        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start)
+        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_DIGIT_BITS, start[0])
+        autotune.set_(lib, autotune.rr.RRX_TUNING_SORT_HYBRID, start[1])
 
 
 @pytest.mark.parametrize("leaves", [5, 9])
