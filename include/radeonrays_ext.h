@@ -34,6 +34,26 @@ RR_API RRError rrxCmdSortPairs(RRContext       context,
                                RRDevicePtr     scratch,
                                RRCommandStream command_stream);
 
+/* rrxCmdSortPairs without a scratch argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * must not allocate).  No counterpart in the reference.  Its scratch
+ * (histograms, tickets, look-back words: 4,329,984 bytes) is a static device
+ * array that every GPU the library runs on holds until the process ends;
+ * the call allocates nothing.  At most 16,777,216 pairs (more gives
+ * RR_ERROR_INVALID_PARAMETER); the same stable order as rrxCmdSortPairs, with
+ * its default 8-bit LSD passes whatever the sort tuning.  All such sorts on
+ * one GPU share the array: calls that overlap in time on one GPU race
+ * (enqueue them on one stream, or order their streams).  The context must
+ * have been created on the GPU the call runs on. */
+RR_API RRError rrxCmdSortPairsNoAlloc(RRContext       context,
+                                      RRDevicePtr     keys,
+                                      RRDevicePtr     values,
+                                      RRDevicePtr     keys_tmp,
+                                      RRDevicePtr     values_tmp,
+                                      uint32_t        n,
+                                      uint32_t        key_bits,
+                                      RRCommandStream command_stream);
+
 /* Closest- or any-hit query like rrCmdIntersect with FULL_HIT_T output that
  * also writes per-ray traversal counters to `stats` (4 uint32 per ray: inner
  * nodes visited, leaves visited, box tests, triangle tests), the figures

@@ -469,6 +469,9 @@ RRError CreateContext(int device, hipStream_t stream, RRContext* out) {
             Check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         }
         rr::InitDeviceErrors();
+        // The no-allocation sort's scratch on this device (sort.hip).
+        if (!rr::InitSortNoAllocScratch())
+            RR_LOG(RR_LOG_LEVEL_WARN, "rrCreateContext: no-allocation sort scratch not found on device %d", device);
         LiveContexts().Add(c);
         *out = reinterpret_cast<RRContext>(c);
         return RR_SUCCESS;
@@ -1082,6 +1085,35 @@ RRError rrxCmdSortPairs(RRContext context, RRDevicePtr keys, RRDevicePtr values,
             const rr::SortScratch ss = rr::CarveSortScratch(sc, n);
             const bool in_place = rr::RadixSortPairs(k, v, kt, vt, n, (int)key_bits, ss, false, s);
             if (!in_place) {
+                Check(hipMemcpyAsync(k, kt, size_t(n) * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+                Check(hipMemcpyAsync(v, vt, size_t(n) * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+            }
+        });
+        return RR_SUCCESS;
+    });
+}
+
+RRError rrxCmdSortPairsNoAlloc(RRContext context, RRDevicePtr keys, RRDevicePtr values, RRDevicePtr keys_tmp,
+                               RRDevicePtr values_tmp, uint32_t n, uint32_t key_bits, RRCommandStream command_stream) {
+    RR_REQUIRE(LiveCtx(context) && LivePtr(keys) && LivePtr(values) && LivePtr(keys_tmp) && LivePtr(values_tmp) &&
+                   LiveStream(command_stream),
+               "rrxCmdSortPairsNoAlloc");
+    RR_REQUIRE(key_bits >= 1 && key_bits <= 32, "rrxCmdSortPairsNoAlloc");
+    if (n > rr::kSortNoAllocMaxKeys) {
+        RR_LOG(RR_LOG_LEVEL_ERROR, "rrxCmdSortPairsNoAlloc: invalid parameter (%u pairs; at most %u without allocation)",
+               n, rr::kSortNoAllocMaxKeys);
+        return RR_ERROR_INVALID_PARAMETER;
+    }
+    return Guarded("rrxCmdSortPairsNoAlloc", [&]() -> RRError {
+        DeviceGuard g(AsCtx(context)->device);
+        auto* k = static_cast<uint32_t*>(Addr(keys));
+        auto* v = static_cast<uint32_t*>(Addr(values));
+        auto* kt = static_cast<uint32_t*>(Addr(keys_tmp));
+        auto* vt = static_cast<uint32_t*>(Addr(values_tmp));
+        Record(AsStream(command_stream), [=](hipStream_t s) {
+            const int r = rr::SortPairsNoAlloc(k, v, kt, vt, n, (int)key_bits, s);
+            if (r < 0) throw std::runtime_error("no-allocation sort scratch unavailable on this device");
+            if (r == 0) {
                 Check(hipMemcpyAsync(k, kt, size_t(n) * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
                 Check(hipMemcpyAsync(v, vt, size_t(n) * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
             }

@@ -86,8 +86,23 @@ SortScratch CarveSortScratch(void* base, uint32_t n);
 // cleared the first status_words look-back words (the build fuses that into
 // its Morton kernel).
 // Returns true if the result ended in keys/vals, false if in the tmp arrays.
+// untuned: the default 8-bit LSD shapes whatever the tuning (the
+// no-allocation sort, whose scratch is sized for them).
 bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
-                    const SortScratch& sc, bool hist_ready, hipStream_t stream);
+                    const SortScratch& sc, bool hist_ready, hipStream_t stream, bool untuned = false);
+
+// Sort without allocation (rrxCmdSortPairsNoAlloc): the scratch is a static
+// device array of SortNoAllocScratchBytes() bytes per GPU, shared by every
+// such sort on that GPU (overlapping calls on one GPU race).  At most
+// kSortNoAllocMaxKeys pairs.  InitSortNoAllocScratch finds the current
+// device's array (context creation calls it with the device current).
+// SortPairsNoAlloc returns 1 if the result is in keys/vals, 0 if in the tmp
+// arrays, -1 if n is above the maximum, -2 if the array was not found.
+constexpr uint32_t kSortNoAllocMaxKeys = 1u << 24;
+bool InitSortNoAllocScratch();
+size_t SortNoAllocScratchBytes();
+int SortPairsNoAlloc(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
+                     hipStream_t stream);
 
 // Launch helpers exposed for the build's fused histogram.
 void LaunchSortClear(const SortScratch& sc, hipStream_t stream);

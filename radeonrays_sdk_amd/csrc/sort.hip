@@ -864,6 +864,18 @@ bool SetSortDigitBits(int bits) {
 // 4-bit digits (16384-key tiles).
 constexpr uint32_t kWideMinKeys = 1u << 20;
 
+// The 8-bit shape of a sort of n keys, without tuning.
+static SortShape DefaultShape(uint32_t n) {
+    // Large inputs: 16384-key tiles (fewer tiles, so fewer look-back rounds,
+    // and 64-key digit runs per tile on average, i.e. whole cache lines).
+    if (n >= (1u << 22)) return {1024, 16, 8};
+    // Mid-size inputs: 4096-key tiles (whole fast build of 1M triangles
+    // 0.269 against 0.295 ms with 2048-key tiles; of 262,144 triangles 0.160
+    // against 0.171 ms with 1024-key tiles; the same at 65,536).
+    if (n >= (1u << 17)) return {512, 8, 8};
+    return {kBlock, 4, 8};
+}
+
 // Shape of a sort of n keys.  wide_ok: digits other than 8 bits are allowed
 // (not for callers that prepared an 8-bit histogram themselves).
 static SortShape ChooseShape(uint32_t n, bool wide_ok) {
@@ -878,14 +890,7 @@ static SortShape ChooseShape(uint32_t n, bool wide_ok) {
     if (forced.block && (wide_ok || forced.bits == 8)) return forced;
     if (wide_ok && n >= kWideMinKeys && SortDigitBits() == kSortWideBits) return {1024, 8, kSortWideBits};
     if (wide_ok && n >= kWideMinKeys && SortDigitBits() == kSortNarrowBits) return {1024, 16, kSortNarrowBits};
-    // Large inputs: 16384-key tiles (fewer tiles, so fewer look-back rounds,
-    // and 64-key digit runs per tile on average, i.e. whole cache lines).
-    if (n >= (1u << 22)) return {1024, 16, 8};
-    // Mid-size inputs: 4096-key tiles (whole fast build of 1M triangles
-    // 0.269 against 0.295 ms with 2048-key tiles; of 262,144 triangles 0.160
-    // against 0.171 ms with 1024-key tiles; the same at 65,536).
-    if (n >= (1u << 17)) return {512, 8, 8};
-    return {kBlock, 4, 8};
+    return DefaultShape(n);
 }
 
 SortShape ChooseSortShape(uint32_t n) { return ChooseShape(n, true); }
@@ -981,13 +986,13 @@ static void HybridSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint3
 }
 
 bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
-                    const SortScratch& sc, bool hist_ready, hipStream_t stream) {
+                    const SortScratch& sc, bool hist_ready, hipStream_t stream, bool untuned) {
     if (n <= 1) return true;
-    if (const int bits = HybridBits(n, key_bits, hist_ready)) {
+    if (const int bits = untuned ? 0 : HybridBits(n, key_bits, hist_ready)) {
         HybridSort(keys, vals, keys_tmp, vals_tmp, n, key_bits, bits, sc, stream);
         return true;
     }
-    const SortShape shape = ChooseShape(n, !hist_ready);
+    const SortShape shape = untuned ? DefaultShape(n) : ChooseShape(n, !hist_ready);
     const int passes = PassesOf(key_bits, shape);
     const size_t tiles = TilesOf(n, shape);
     const size_t radix = (size_t)1 << shape.bits;
@@ -1038,6 +1043,65 @@ bool RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t
         std::swap(vi, vo);
     }
     return ki == keys;
+}
+
+// ---------------------------------------------------------------------------
+// Sort without allocation: the sort's scratch (histogram copies, tickets,
+// look-back words) in a static device array instead of a caller's buffer,
+// for callers that cannot allocate device memory.  The array is sized for
+// kSortNoAllocMaxKeys pairs with the untuned 8-bit shapes: four passes of at
+// most 1,024 tiles (16,384-key tiles from 2^22 keys, 4,096-key tiles below)
+// of 256 look-back words.  Every GPU the library runs on holds its own copy
+// until the process ends; all no-allocation sorts on one GPU share it, so
+// such sorts must not overlap in time on one GPU (enqueue them on one
+// stream, or order their streams).  The sort clears what must start at zero
+// (histograms and tickets by SortClearKernel, look-back words by the
+// histogram kernel), so a call does not depend on what the last one left.
+
+constexpr size_t kSortNoAllocStatusWords = (size_t)kSortMaxPasses * 1024 * kSortRadix;
+constexpr size_t kSortNoAllocWords = kSortHeadWords + kSortNoAllocStatusWords;
+
+static __device__ __align__(256) uint32_t scratch[kSortNoAllocWords];
+
+// Device address of this GPU's array, found at context creation (on the
+// host the array's name only stands in for it and must not be passed to a
+// kernel).
+constexpr int kMaxDevices = 64;
+static std::atomic<uint32_t*> g_noalloc_scratch[kMaxDevices];
+
+bool InitSortNoAllocScratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return false;
+    if (g_noalloc_scratch[dev].load(std::memory_order_acquire)) return true;
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return false;
+    }
+    g_noalloc_scratch[dev].store(static_cast<uint32_t*>(p), std::memory_order_release);
+    return true;
+}
+
+size_t SortNoAllocScratchBytes() { return sizeof(uint32_t) * kSortNoAllocWords; }
+
+int SortPairsNoAlloc(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
+                     hipStream_t stream) {
+    if (n > kSortNoAllocMaxKeys) return -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -2;
+    uint32_t* base = g_noalloc_scratch[dev].load(std::memory_order_acquire);
+    if (!base) return -2;
+    if (n <= 1) return 1;
+    SortScratch sc;
+    sc.hist = base;
+    sc.tickets = base + kSortHistCopies * kSortHistStride;
+    sc.bucket_off = sc.tickets + 64;
+    sc.status = base + kSortHeadWords;
+    const SortShape shape = DefaultShape(n);
+    sc.status_words = TilesOf(n, shape) * ((size_t)1 << shape.bits);
+    if (kSortHeadWords + StatusWords(n, shape) > kSortNoAllocWords) return -1;  // not reached for n <= the maximum
+    return RadixSortPairs(keys, vals, keys_tmp, vals_tmp, n, key_bits, sc, false, stream, true) ? 1 : 0;
 }
 
 }  // namespace rr

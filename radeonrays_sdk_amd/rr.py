@@ -166,6 +166,7 @@ _SIGNATURES = {
     # radeonrays_ext.h
     "rrxGetSortMemoryRequirements": [c_void_p, c_uint32, POINTER(c_size_t)],
     "rrxCmdSortPairs": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p],
+    "rrxCmdSortPairsNoAlloc": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p],
     "rrxCmdIntersectWithStats": [c_void_p, c_void_p, c_int, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p],
     "rrxCmdUpdateScene": [c_void_p, POINTER(RRSceneBuildInput), c_void_p, c_void_p, c_void_p],
     "rrxSetTuning": [c_int, c_uint32],
