@@ -546,6 +546,12 @@ def profile_section(which: str, reps: int = 3) -> None:
         tris = int(w.tri_counts.sum())
         out.update(workload=f"{len(spans)} meshes, {tris} triangles, batched refit",
                    algorithmic_bytes=tris * 112 + (tris - len(spans)) * 192)
+    # One 256 MiB device-to-device copy: the byte counters' calibration.
+    src = torch.ones(256 << 18, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    out["calibration_copy_bytes"] = int(src.numel() * 4)
     torch.cuda.synchronize()
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -420,7 +420,7 @@ __global__ __launch_bounds__(BLOCK) void BatchBuildLdsKernel(const BatchMesh* __
 // threads run them rather than one workgroup per mesh.
 __global__ __launch_bounds__(kBlock) void BatchEmitKernel(const BatchMesh* __restrict__ table, uint32_t meshes,
                                                           uint32_t total_chunks, uint32_t* scratch) {
-    const uint32_t c = blockIdx.x * (kBlock / kRefitChunk) + threadIdx.x / kRefitChunk;
+    const uint32_t c = XcdContiguousBlock(blockIdx.x, gridDim.x) * (kBlock / kRefitChunk) + threadIdx.x / kRefitChunk;
     if (c >= total_chunks) return;
     uint32_t lo = 0, hi = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)
     while (hi - lo > 1) {
@@ -449,7 +449,8 @@ __global__ __launch_bounds__(kBlock) void BatchRefitKernel(const BatchMesh* __re
                                                            uint32_t total_chunks, uint32_t* scratch) {
     __shared__ RefitLds<kBlock> lds;
     constexpr uint32_t kChunksPerBlock = kBlock / kRefitChunk;
-    const uint32_t cb = blockIdx.x * kChunksPerBlock;
+    // Consecutive chunks (one mesh's leaves) on one XCD, sharing its L2.
+    const uint32_t cb = XcdContiguousBlock(blockIdx.x, gridDim.x) * kChunksPerBlock;
     const uint32_t c = cb + threadIdx.x / kRefitChunk;
     bool active = c < total_chunks;
     uint32_t lo_m = 0, hi_m = meshes;  // last mesh with first_chunk <= c (empty meshes own no chunk)

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void EmitHierarchyKernel(const uint32_t* __
 __global__ __launch_bounds__(kBlock) void RefitMeshKernel(MeshView m, BvhNode* __restrict__ nodes) {
     __shared__ RefitLds<kBlock> lds;
     const uint32_t n = m.tri_count;
-    const uint32_t base = blockIdx.x * kBlock;
+    const uint32_t base = XcdContiguousBlock(blockIdx.x, gridDim.x) * kBlock;
     const uint32_t i = base + threadIdx.x;
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
     uint32_t parent = kInvalid;

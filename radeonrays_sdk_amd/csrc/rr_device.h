@@ -205,6 +205,20 @@ RR_DEV bool CloserHit(float t, uint32_t inst, uint32_t prim, float bt, uint32_t 
 
 // ---------------------------------------------------------------------------
 // 64-bit wave helpers (wave64 on CDNA).
+// The gfx950 dispatcher hands workgroups to its 8 XCDs round-robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  Kernels whose
+// neighbouring workgroups share cache lines (the refits: a mesh's triangles
+// and nodes) renumber their workgroups with this bijection of [0, grid) so
+// that each XCD takes one contiguous range of logical workgroups (XCD x the
+// x-th eighth, the first grid % 8 ranges one longer).  Only placement
+// depends on it, never results.
+constexpr uint32_t kXcds = 8;
+RR_DEV uint32_t XcdContiguousBlock(uint32_t block, uint32_t grid) {
+    const uint32_t x = block % kXcds, k = block / kXcds;
+    const uint32_t q = grid / kXcds, r = grid % kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 RR_DEV uint64_t LaneMaskLt() {
     const uint32_t lane = __lane_id();
     return lane ? (~0ull >> (64 - lane)) : 0ull;
