@@ -19,6 +19,23 @@ _OUTPUT = {"full": rr.RR_INTERSECT_QUERY_OUTPUT_FULL_HIT, "instance": rr.RR_INTE
 _HIT_BYTES = {"full": 16, "instance": 4, "full_t": 32}
 
 
+def _nbytes(t) -> int:
+    return int(t.numel()) * int(t.element_size())
+
+
+def _check_rays(rays, where: str) -> int:
+    """Ray count of a [R, 8] float32 device tensor (RRRay records); anything
+    else is refused here, before a kernel could read past the buffer."""
+    if rays.dim() != 2 or int(rays.shape[1]) != 8 or rays.element_size() != 4 or not rays.is_contiguous():
+        raise ValueError(f"{where}: rays must be a contiguous float32 [R, 8] tensor, got {tuple(rays.shape)}")
+    return int(rays.shape[0])
+
+
+def _check_size(buf, need: int, what: str, where: str) -> None:
+    if _nbytes(buf) < need:
+        raise ValueError(f"{where}: {what} holds {_nbytes(buf)} bytes, {need} needed")
+
+
 def _torch():
     import torch
     return torch
@@ -109,6 +126,10 @@ class Context:
     # -- builds ---------------------------------------------------------------
     @staticmethod
     def _mesh(vertices, indices, vptr, iptr):
+        if vertices.dim() != 2 or int(vertices.shape[1]) < 3 or vertices.element_size() != 4 or vertices.stride(1) != 1:
+            raise ValueError(f"vertices must be float32 [V, >=3] with unit column stride, got {tuple(vertices.shape)}")
+        if indices.dim() != 2 or int(indices.shape[1]) != 3 or indices.element_size() not in (2, 4):
+            raise ValueError(f"indices must be int32/uint16 [T, 3], got {tuple(indices.shape)}")
         v_stride = vertices.stride(0) * vertices.element_size()
         prim = rr.RRTriangleMeshPrimitive(vptr, int(vertices.shape[0]), int(v_stride), iptr, int(indices.shape[0]),
                                           rr.RR_INDEX_TYPE_UINT16 if indices.element_size() == 2 else rr.RR_INDEX_TYPE_UINT32)
@@ -138,6 +159,8 @@ class Context:
         req = self.geometry_requirements(tris, max(int(vertices.shape[0]), 1), restructure)
         result = result if result is not None else self.empty(req.result_buffer_size)
         scratch = scratch if scratch is not None else self.empty(req.temporary_build_buffer_size)
+        _check_size(result, req.result_buffer_size, "result", "build_geometry")
+        _check_size(scratch, req.temporary_build_buffer_size, "scratch", "build_geometry")
         with self.ptrs(vertices.data_ptr(), indices.data_ptr(), scratch.data_ptr(), result.data_ptr()) as (v, i, s, r):
             prim, inp = self._mesh(vertices, indices, v, i)
             opts = self._options(restructure)
@@ -149,6 +172,8 @@ class Context:
     def update_geometry(self, accel: Accel, vertices, indices) -> None:
         """Refit `accel` to moved vertices (same triangles)."""
         indices = indices.contiguous()
+        if int(indices.shape[0]) != accel.prim_count:
+            raise ValueError(f"update_geometry: {int(indices.shape[0])} triangles, the BVH has {accel.prim_count}")
         with self.ptrs(vertices.data_ptr(), indices.data_ptr(), accel.address) as (v, i, r):
             prim, inp = self._mesh(vertices, indices, v, i)
             rr.check(self.lib.rrCmdBuildGeometry(self.ctx, rr.RR_BUILD_OPERATION_UPDATE, byref(inp), None, None, r,
@@ -282,9 +307,14 @@ class Context:
     def intersect(self, accel: Accel, rays, query: str = "closest", output: str = "full", indirect=None, hits=None,
                   scratch=None):
         """rrCmdIntersect: rays float32 [R, 8] on the device; returns the hit buffer (uint8 tensor)."""
-        n = int(rays.shape[0])
+        n = _check_rays(rays, "intersect")
         hits = hits if hits is not None else self.empty(n * _HIT_BYTES[output])
         scratch = scratch if scratch is not None else self.trace_scratch(n)
+        _check_size(hits, n * _HIT_BYTES[output], "hits", "intersect")
+        if n:
+            size = c_size_t()
+            rr.check(self.lib.rrGetTraceMemoryRequirements(self.ctx, n, byref(size)), "rrGetTraceMemoryRequirements")
+            _check_size(scratch, size.value, "scratch", "intersect")
         ind = indirect.data_ptr() if indirect is not None else None
         with self.ptrs(accel.address, rays.data_ptr(), ind, hits.data_ptr(), scratch.data_ptr()) as (a, r, c, h, s):
             rr.check(self.lib.rrCmdIntersect(self.ctx, a, _QUERY[query], r, n, c, _OUTPUT[output], h, s, self.cs),
@@ -293,7 +323,7 @@ class Context:
 
     def intersect_with_stats(self, accel: Accel, rays, query: str = "closest"):
         torch = _torch()
-        n = int(rays.shape[0])
+        n = _check_rays(rays, "intersect_with_stats")
         hits = self.empty(n * 32)
         stats = torch.empty((n, 4), dtype=torch.int32, device=self.torch_device)
         scratch = self.trace_scratch(n)
@@ -324,13 +354,21 @@ class Context:
         outs, addrs = [], []
         for req in requests:
             accel, rays, query, output = req[:4]
-            n = int(rays.shape[0])
+            n = _check_rays(rays, "intersect_batch")
             hits = req[4] if len(req) > 4 and req[4] is not None else self.empty(n * _HIT_BYTES[output])
+            _check_size(hits, n * _HIT_BYTES[output], "hits", "intersect_batch")
             outs.append(hits)
             addrs += [accel.address, rays.data_ptr() if n else None, hits.data_ptr()]
         total = sum(int(r[1].shape[0]) for r in requests)
         largest = max([int(r[1].shape[0]) for r in requests] + [1])
-        scratch = scratch if scratch is not None else self.batch_scratch(len(requests), total, largest)
+        if scratch is None:
+            scratch = self.batch_scratch(len(requests), total, largest)
+        else:
+            size = c_size_t()
+            rr.check(self.lib.rrxGetIntersectBatchMemoryRequirements(self.ctx, len(requests), total, largest,
+                                                                     byref(size)),
+                     "rrxGetIntersectBatchMemoryRequirements")
+            _check_size(scratch, size.value, "scratch", "intersect_batch")
         with self.ptrs(*addrs, scratch.data_ptr()) as hs:
             arr = (rr.RRxIntersectRequest * max(len(requests), 1))()
             for k, req in enumerate(requests):
