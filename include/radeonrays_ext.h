@@ -229,7 +229,9 @@ typedef struct RRxServerOptions
     uint32_t     max_inflight;        /* concurrent launches (batched mode), >= 1 */
     uint32_t     max_batch_requests;  /* requests per launch */
     uint32_t     max_batch_rays;      /* rays per launch; also the largest request */
-    uint64_t     memory_budget;       /* bytes of admitted ray + hit buffers */
+    uint64_t     memory_budget;       /* bytes of admitted ray + hit buffers (accounting) */
+    uint64_t     staging_bytes;       /* device arena for host-memory requests (rrxServerSubmitHost),
+                                         allocated with the server; 0: host requests are refused */
 } RRxServerOptions;
 
 typedef struct RRxServedTiming
@@ -253,6 +255,31 @@ RR_API RRError rrxServerSubmit(RRxServer                  server,
                                const uint64_t*            release_ns,
                                uint32_t                   count,
                                uint64_t*                  first_ticket);
+
+/* A request whose rays and hits are in host memory (pinned memory copies
+ * fastest).  The server admits it only when space for its rays and hits is
+ * free in its device staging arena (staging_bytes), so the arena bounds the
+ * device memory of host requests in flight: at dispatch its rays are copied
+ * into the arena, traced there with the rest of the launch, and its hits
+ * copied back to `hits`; the space is freed when the hits are on the host. */
+typedef struct RRxHostIntersectRequest
+{
+    RRDevicePtr            scene_buffer;  /* geometry (BLAS) or scene (TLAS) result buffer */
+    const void*            rays;          /* ray_count RRRay records, host memory */
+    void*                  hits;          /* ray_count records of query_output's type, host memory */
+    uint32_t               ray_count;
+    RRIntersectQuery       query;
+    RRIntersectQueryOutput query_output;
+    uint32_t               padding;
+} RRxHostIntersectRequest;
+
+/* Queue host-memory requests, as rrxServerSubmit.  Requests larger than
+ * max_batch_rays or than the staging arena are rejected. */
+RR_API RRError rrxServerSubmitHost(RRxServer                      server,
+                                   const RRxHostIntersectRequest* requests,
+                                   const uint64_t*                release_ns,
+                                   uint32_t                       count,
+                                   uint64_t*                      first_ticket);
 
 /* Block until every request submitted so far has completed. */
 RR_API RRError rrxServerDrain(RRxServer server);

@@ -16,6 +16,12 @@
 //             rrSumbitCommandStream + rrWaitEvent per request, in order;
 //   complete  a stream's event is polled; its requests are stamped complete
 //             and their bytes leave the budget.
+// Requests whose rays and hits are in host memory (rrxServerSubmitHost) are
+// staged through a device arena owned by the server: a request is admitted
+// only when the arena has room for its rays and hits, its rays are copied in
+// on the launch's stream before the launch and its hits copied out after
+// it, and the room is freed at completion, so the arena bounds the device
+// memory such requests hold.
 // The thread talks to the GPU only through the library's public entry points
 // and HIP streams/events of default priority.
 #include <hip/hip_runtime.h>
@@ -25,6 +31,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <queue>
 #include <thread>
@@ -51,6 +58,63 @@ uint32_t HitBytes(RRIntersectQueryOutput o) {
     }
 }
 
+// First-fit allocator over the staging arena's bytes, in 256-byte granules;
+// freed ranges merge with free neighbours.
+class ArenaSpace {
+public:
+    static constexpr uint64_t kGranule = 256;
+    void Reset(uint64_t bytes) {
+        free_.clear();
+        if (bytes >= kGranule) free_[0] = bytes / kGranule * kGranule;
+    }
+    bool Alloc(uint64_t bytes, uint64_t& off) {
+        bytes = (bytes + kGranule - 1) / kGranule * kGranule;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            off = it->first;
+            const uint64_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            return true;
+        }
+        return false;
+    }
+    void Free(uint64_t off, uint64_t bytes) {
+        bytes = (bytes + kGranule - 1) / kGranule * kGranule;
+        auto next = free_.lower_bound(off);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                bytes += prev->second;
+                free_.erase(prev);
+            }
+        }
+        next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            free_.erase(next);
+        }
+        free_[off] = bytes;
+    }
+
+private:
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+};
+
+uint64_t StagedBytes(uint32_t rays, uint32_t hit_bytes) {
+    const uint64_t g = ArenaSpace::kGranule;
+    return ((uint64_t)rays * 32 + g - 1) / g * g + ((uint64_t)rays * hit_bytes + g - 1) / g * g;
+}
+
+// Host buffers of a staged request and its place in the arena.
+struct HostBuffers {
+    const void* rays = nullptr;
+    void* hits = nullptr;
+    uint64_t arena_off = 0;
+    bool staged = false;  // holds arena space
+};
+
 struct Pending {
     uint64_t release;
     uint64_t ticket;
@@ -67,6 +131,7 @@ struct Slot {
     RRCommandStream cs = nullptr;
     RRDevicePtr scratch = nullptr;
     std::vector<uint64_t> tickets;
+    std::vector<RRDevicePtr> handles;  // arena handles of the launch's staged requests
     bool busy = false;
 };
 
@@ -80,6 +145,7 @@ struct Server {
     std::condition_variable wake;     // submissions and shutdown
     std::condition_variable drained;  // completions
     std::vector<RRxIntersectRequest> reqs;  // by ticket
+    std::vector<HostBuffers> host;          // by ticket (rays == nullptr: device request)
     std::vector<uint64_t> bytes;
     std::vector<RRxServedTiming> timing;
     std::vector<uint8_t> complete;
@@ -95,6 +161,9 @@ struct Server {
     std::vector<Slot> slots;
     RRCommandStream plain_cs = nullptr;
     RRDevicePtr plain_scratch = nullptr;
+    RRDevicePtr arena = nullptr;  // staging arena for host requests
+    uint8_t* arena_base = nullptr;
+    ArenaSpace arena_space;       // guarded by mu
     std::thread worker;
 
     uint64_t Now() const {
@@ -110,6 +179,13 @@ struct Server {
         while (!released.empty()) {
             const uint64_t t = released.front();
             if (admitted_bytes + bytes[t] > opt.memory_budget) break;
+            HostBuffers& h = host[t];
+            if (h.rays && reqs[t].ray_count) {
+                // Head of line: wait for arena space rather than pass over it.
+                if (!arena_space.Alloc(StagedBytes(reqs[t].ray_count, HitBytes(reqs[t].query_output)), h.arena_off))
+                    break;
+                h.staged = true;
+            }
             admitted_bytes += bytes[t];
             peak_bytes = std::max(peak_bytes, admitted_bytes);
             admitted.push_back(t);
@@ -118,9 +194,13 @@ struct Server {
     }
 
     // Take the next batch off the admitted queue (caller holds mu).
-    void TakeBatch(uint32_t max_requests, std::vector<uint64_t>& tickets, std::vector<RRxIntersectRequest>& batch) {
+    // (The per-ticket vectors grow under mu as requests are submitted, so
+    // the batch takes copies of what the dispatch needs.)
+    void TakeBatch(uint32_t max_requests, std::vector<uint64_t>& tickets, std::vector<RRxIntersectRequest>& batch,
+                   std::vector<HostBuffers>& staged) {
         tickets.clear();
         batch.clear();
+        staged.clear();
         uint64_t rays = 0;
         while (!admitted.empty() && tickets.size() < max_requests) {
             const uint64_t t = admitted.front();
@@ -128,6 +208,7 @@ struct Server {
             rays += reqs[t].ray_count;
             tickets.push_back(t);
             batch.push_back(reqs[t]);
+            staged.push_back(host[t]);
             admitted.pop_front();
         }
     }
@@ -138,9 +219,65 @@ struct Server {
             timing[t].complete_ns = now;
             complete[t] = 1;
             admitted_bytes -= bytes[t];
+            HostBuffers& h = host[t];
+            if (h.staged) {
+                arena_space.Free(h.arena_off, StagedBytes(reqs[t].ray_count, HitBytes(reqs[t].query_output)));
+                h.staged = false;
+            }
             ++completed;
         }
         drained.notify_all();
+    }
+
+    // Point the staged requests of a batch at their arena space (new
+    // RRDevicePtr handles, appended to `handles`) and enqueue their ray
+    // uploads on `stream` (synchronously when stream is null).  The caller
+    // does not hold mu: `staged` holds copies taken with the batch.
+    RRError StageIn(const std::vector<HostBuffers>& staged, std::vector<RRxIntersectRequest>& batch,
+                    std::vector<RRDevicePtr>& handles, hipStream_t stream) {
+        for (size_t k = 0; k < staged.size(); ++k) {
+            const HostBuffers& h = staged[k];
+            if (!h.staged) continue;
+            const uint64_t ray_bytes = (uint64_t)batch[k].ray_count * 32;
+            const uint64_t hits_off = h.arena_off + (ray_bytes + ArenaSpace::kGranule - 1) / ArenaSpace::kGranule *
+                                                        ArenaSpace::kGranule;
+            RRDevicePtr r = nullptr, o = nullptr;
+            RRError e = rrGetDevicePtrFromHipPtr(ctx, arena_base, h.arena_off, &r);
+            if (e == RR_SUCCESS) handles.push_back(r);
+            if (e == RR_SUCCESS) e = rrGetDevicePtrFromHipPtr(ctx, arena_base, hits_off, &o);
+            if (e == RR_SUCCESS) handles.push_back(o);
+            if (e != RR_SUCCESS) return e;
+            batch[k].rays = r;
+            batch[k].hits = o;
+            const hipError_t c = stream ? hipMemcpyAsync(arena_base + h.arena_off, h.rays, ray_bytes,
+                                                         hipMemcpyHostToDevice, stream)
+                                        : hipMemcpy(arena_base + h.arena_off, h.rays, ray_bytes, hipMemcpyHostToDevice);
+            if (c != hipSuccess) return RR_ERROR_INTERNAL;
+        }
+        return RR_SUCCESS;
+    }
+
+    // Enqueue the staged requests' hit downloads (after the launch).
+    RRError StageOut(const std::vector<HostBuffers>& staged, const std::vector<RRxIntersectRequest>& batch,
+                     hipStream_t stream) {
+        for (size_t k = 0; k < staged.size(); ++k) {
+            const HostBuffers& h = staged[k];
+            if (!h.staged) continue;
+            const uint64_t ray_bytes = (uint64_t)batch[k].ray_count * 32;
+            const uint64_t hits_off = h.arena_off + (ray_bytes + ArenaSpace::kGranule - 1) / ArenaSpace::kGranule *
+                                                        ArenaSpace::kGranule;
+            const uint64_t hit_bytes = (uint64_t)batch[k].ray_count * HitBytes(batch[k].query_output);
+            const hipError_t c = stream ? hipMemcpyAsync(h.hits, arena_base + hits_off, hit_bytes,
+                                                         hipMemcpyDeviceToHost, stream)
+                                        : hipMemcpy(h.hits, arena_base + hits_off, hit_bytes, hipMemcpyDeviceToHost);
+            if (c != hipSuccess) return RR_ERROR_INTERNAL;
+        }
+        return RR_SUCCESS;
+    }
+
+    void ReleaseHandles(std::vector<RRDevicePtr>& handles) {
+        for (RRDevicePtr p : handles) (void)rrReleaseDevicePtr(ctx, p);
+        handles.clear();
     }
 
     void Fail(RRError e) {
@@ -162,6 +299,7 @@ struct Server {
     void RunPlain() {
         std::vector<uint64_t> tickets;
         std::vector<RRxIntersectRequest> batch;
+        std::vector<HostBuffers> staged;
         for (;;) {
             {
                 std::unique_lock<std::mutex> lk(mu);
@@ -182,20 +320,27 @@ struct Server {
                         wake.wait(lk);
                     }
                 }
-                TakeBatch(1, tickets, batch);
+                TakeBatch(1, tickets, batch, staged);
             }
-            const RRxIntersectRequest& q = batch[0];
             Stamp(tickets, Now(), batches++);
-            if (q.ray_count == 0) {
+            if (batch[0].ray_count == 0) {
                 Finish(tickets, Now());
                 continue;
             }
+            // A staged request: upload, trace, download, each waited for, as
+            // a client of the per-call API does.
+            std::vector<RRDevicePtr> handles;
+            RRError e = StageIn(staged, batch, handles, nullptr);
+            const RRxIntersectRequest& q = batch[0];
             RREvent ev = nullptr;
-            RRError e = rrCmdIntersect(ctx, q.scene_buffer, q.query, q.rays, q.ray_count, nullptr, q.query_output,
-                                       q.hits, plain_scratch, plain_cs);
+            if (e == RR_SUCCESS)
+                e = rrCmdIntersect(ctx, q.scene_buffer, q.query, q.rays, q.ray_count, nullptr, q.query_output, q.hits,
+                                   plain_scratch, plain_cs);
             if (e == RR_SUCCESS) e = rrSumbitCommandStream(ctx, plain_cs, nullptr, &ev);
             if (e == RR_SUCCESS) e = rrWaitEvent(ctx, ev);
             if (ev) (void)rrReleaseEvent(ctx, ev);
+            if (e == RR_SUCCESS) e = StageOut(staged, batch, nullptr);
+            ReleaseHandles(handles);
             if (e != RR_SUCCESS) return Fail(e);
             Finish(tickets, Now());
         }
@@ -204,6 +349,7 @@ struct Server {
     void RunBatched() {
         std::vector<uint64_t> tickets;
         std::vector<RRxIntersectRequest> batch;
+        std::vector<HostBuffers> staged;
         for (;;) {
             // Completions.
             for (Slot& s : slots) {
@@ -212,6 +358,7 @@ struct Server {
                 if (q == hipErrorNotReady) continue;
                 if (q != hipSuccess) return Fail(RR_ERROR_INTERNAL);
                 s.busy = false;
+                ReleaseHandles(s.handles);
                 Finish(s.tickets, Now());
             }
             Slot* free_slot = nullptr;
@@ -225,7 +372,7 @@ struct Server {
                 if (stop) break;
                 ReleaseAndAdmit(Now());
                 if (free_slot && !admitted.empty()) {
-                    TakeBatch(opt.max_batch_requests, tickets, batch);
+                    TakeBatch(opt.max_batch_requests, tickets, batch, staged);
                 } else {
                     tickets.clear();
                     if (!any_busy) {
@@ -246,15 +393,24 @@ struct Server {
             }
             Slot& s = *free_slot;
             Stamp(tickets, Now(), batches++);
-            RRError e = rrxCmdIntersectBatch(ctx, batch.data(), (uint32_t)batch.size(), s.scratch, s.cs);
+            RRError e = StageIn(staged, batch, s.handles, s.stream);
+            if (e == RR_SUCCESS) e = rrxCmdIntersectBatch(ctx, batch.data(), (uint32_t)batch.size(), s.scratch, s.cs);
+            if (e == RR_SUCCESS) e = StageOut(staged, batch, s.stream);
             if (e == RR_SUCCESS && hipEventRecord(s.done, s.stream) != hipSuccess) e = RR_ERROR_INTERNAL;
-            if (e != RR_SUCCESS) return Fail(e);
+            if (e != RR_SUCCESS) {
+                (void)hipStreamSynchronize(s.stream);
+                ReleaseHandles(s.handles);
+                return Fail(e);
+            }
             s.tickets.swap(tickets);
             s.busy = true;
         }
         // Shutdown: wait for what is still on the GPU.
         for (Slot& s : slots)
-            if (s.busy) (void)hipEventSynchronize(s.done);
+            if (s.busy) {
+                (void)hipEventSynchronize(s.done);
+                ReleaseHandles(s.handles);
+            }
     }
 
     void Run() {
@@ -287,6 +443,7 @@ void DestroyResources(Server* sv) {
     sv->slots.clear();
     if (sv->plain_cs) (void)rrReleaseCommandStream(sv->ctx, sv->plain_cs);
     if (sv->plain_scratch) (void)rrReleaseDevicePtr(sv->ctx, sv->plain_scratch);
+    if (sv->arena) (void)rrReleaseDevicePtr(sv->ctx, sv->arena);
 }
 
 }  // namespace
@@ -328,6 +485,13 @@ RRError rrxCreateServer(RRContext context, const RRxServerOptions* options, RRxS
             }
             e = rrGetCommandStreamFromHipStream(context, s.stream, &s.cs);
             if (e == RR_SUCCESS) e = rrAllocateDeviceBuffer(context, bytes, &s.scratch);
+        }
+    }
+    if (e == RR_SUCCESS && o.staging_bytes) {
+        e = rrAllocateDeviceBuffer(context, o.staging_bytes, &sv->arena);
+        if (e == RR_SUCCESS) {
+            sv->arena_base = static_cast<uint8_t*>(rrGetHipAddress(sv->arena));
+            sv->arena_space.Reset(o.staging_bytes);
         }
     }
     (void)hipSetDevice(prev);
@@ -390,6 +554,56 @@ RRError rrxServerSubmit(RRxServer server, const RRxIntersectRequest* requests, c
             const uint64_t t = sv->submitted++;
             const uint64_t rel = release_ns ? std::max(release_ns[i], now) : now;
             sv->reqs.push_back(q);
+            sv->host.push_back(HostBuffers{});
+            sv->bytes.push_back(uint64_t(q.ray_count) * (32 + HitBytes(q.query_output)));
+            RRxServedTiming tm{};
+            tm.arrival_ns = rel;
+            sv->timing.push_back(tm);
+            sv->complete.push_back(0);
+            sv->unreleased.push({rel, t});
+        }
+    }
+    sv->wake.notify_all();
+    return RR_SUCCESS;
+}
+
+RRError rrxServerSubmitHost(RRxServer server, const RRxHostIntersectRequest* requests, const uint64_t* release_ns,
+                            uint32_t count, uint64_t* first_ticket) {
+    if (!LiveServer(server) || !first_ticket || (count && !requests)) return RR_ERROR_INVALID_PARAMETER;
+    Server* sv = AsServer(server);
+    for (uint32_t i = 0; i < count; ++i) {
+        const RRxHostIntersectRequest& q = requests[i];
+        if (!q.scene_buffer || (q.ray_count && (!q.rays || !q.hits))) return RR_ERROR_INVALID_PARAMETER;
+        if (rr::Raw(q.query) != RR_INTERSECT_QUERY_CLOSEST && rr::Raw(q.query) != RR_INTERSECT_QUERY_ANY) return RR_ERROR_INVALID_PARAMETER;
+        if (rr::Raw(q.query_output) != RR_INTERSECT_QUERY_OUTPUT_FULL_HIT &&
+            rr::Raw(q.query_output) != RR_INTERSECT_QUERY_OUTPUT_INSTANCE_ID &&
+            rr::Raw(q.query_output) != RR_INTERSECT_QUERY_OUTPUT_FULL_HIT_T)
+            return RR_ERROR_INVALID_PARAMETER;
+        if (q.ray_count > sv->opt.max_batch_rays) return RR_ERROR_INVALID_PARAMETER;
+        if (uint64_t(q.ray_count) * (32 + HitBytes(q.query_output)) > sv->opt.memory_budget)
+            return RR_ERROR_INVALID_PARAMETER;
+        if (q.ray_count && (!sv->arena || StagedBytes(q.ray_count, HitBytes(q.query_output)) > sv->opt.staging_bytes))
+            return RR_ERROR_INVALID_PARAMETER;
+    }
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->error != RR_SUCCESS) return sv->error;
+        const uint64_t now = sv->Now();
+        *first_ticket = sv->submitted;
+        for (uint32_t i = 0; i < count; ++i) {
+            const RRxHostIntersectRequest& q = requests[i];
+            const uint64_t t = sv->submitted++;
+            const uint64_t rel = release_ns ? std::max(release_ns[i], now) : now;
+            RRxIntersectRequest d{};
+            d.scene_buffer = q.scene_buffer;
+            d.ray_count = q.ray_count;
+            d.query = q.query;
+            d.query_output = q.query_output;
+            sv->reqs.push_back(d);
+            HostBuffers h;
+            h.rays = q.ray_count ? q.rays : nullptr;
+            h.hits = q.hits;
+            sv->host.push_back(h);
             sv->bytes.push_back(uint64_t(q.ray_count) * (32 + HitBytes(q.query_output)));
             RRxServedTiming tm{};
             tm.arrival_ns = rel;

@@ -123,7 +123,12 @@ class RRxIntersectRequest(Structure):
 
 class RRxServerOptions(Structure):
     _fields_ = [("mode", c_int), ("max_inflight", c_uint32), ("max_batch_requests", c_uint32),
-                ("max_batch_rays", c_uint32), ("memory_budget", c_uint64)]
+                ("max_batch_rays", c_uint32), ("memory_budget", c_uint64), ("staging_bytes", c_uint64)]
+
+
+class RRxHostIntersectRequest(Structure):
+    _fields_ = [("scene_buffer", c_void_p), ("rays", c_void_p), ("hits", c_void_p), ("ray_count", c_uint32),
+                ("query", c_int), ("query_output", c_int), ("padding", c_uint32)]
 
 
 class RRxServedTiming(Structure):
@@ -183,6 +188,8 @@ _SIGNATURES = {
     "rrxDestroyServer": [c_void_p],
     "rrxServerNow": [c_void_p],
     "rrxServerSubmit": [c_void_p, POINTER(RRxIntersectRequest), POINTER(c_uint64), c_uint32, POINTER(c_uint64)],
+    "rrxServerSubmitHost": [c_void_p, POINTER(RRxHostIntersectRequest), POINTER(c_uint64), c_uint32,
+                            POINTER(c_uint64)],
     "rrxServerDrain": [c_void_p],
     "rrxServerGetTimings": [c_void_p, c_uint64, c_uint32, POINTER(RRxServedTiming)],
     "rrxServerGetPeakAdmitted": [c_void_p, POINTER(c_uint64)],

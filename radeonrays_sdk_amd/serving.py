@@ -37,13 +37,25 @@ class Request:
     hits: object
 
 
+@dataclass
+class HostRequest:
+    """One ray query whose rays and hits are host tensors (pinned memory
+    copies fastest): `rays` float32 [count, 8], `hits` uint8 of at least
+    count * record bytes.  The server stages them through its device arena."""
+    accel: Accel
+    rays: object
+    query: str
+    output: str
+    hits: object
+
+
 class Server:
     def __init__(self, ctx: Context, mode: str = "batched", max_inflight: int = 2, max_batch_requests: int = 256,
-                 max_batch_rays: int = 4 << 20, memory_budget: int = 200 << 30):
+                 max_batch_rays: int = 4 << 20, memory_budget: int = 200 << 30, staging_bytes: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
         opts = rr.RRxServerOptions(rr.RRX_SERVE_PLAIN if mode == "plain" else rr.RRX_SERVE_BATCHED, max_inflight,
-                                   max_batch_requests, max_batch_rays, memory_budget)
+                                   max_batch_requests, max_batch_rays, memory_budget, staging_bytes)
         h = c_void_p()
         rr.check(self.lib.rrxCreateServer(ctx.ctx, byref(opts), byref(h)), "rrxCreateServer")
         self.handle = h
@@ -69,7 +81,10 @@ class Server:
         for k, q in enumerate(requests):
             if q.accel.address not in accel_ptrs:
                 accel_ptrs[q.accel.address] = self._ptr(q.accel.address)
-            assert q.hits.numel() >= q.count * _HIT_BYTES[q.output]
+            if q.hits.numel() * q.hits.element_size() < q.count * _HIT_BYTES[q.output]:
+                raise ValueError(f"request {k}: hit buffer too small")
+            if q.ray_offset + 32 * q.count > q.rays.numel() * q.rays.element_size():
+                raise ValueError(f"request {k}: rays run past the ray buffer")
             arr[k] = rr.RRxIntersectRequest(accel_ptrs[q.accel.address], self._ptr(q.rays.data_ptr(), q.ray_offset),
                                             self._ptr(q.hits.data_ptr()), q.count, _QUERY[q.query],
                                             _OUTPUT[q.output], 0)
@@ -78,6 +93,32 @@ class Server:
             rel = (c_uint64 * max(n, 1))(*[int(x) for x in release_ns])
         first = c_uint64()
         rr.check(self.lib.rrxServerSubmit(self.handle, arr, rel, n, byref(first)), "rrxServerSubmit")
+        return first.value
+
+    def submit_host(self, requests, release_ns=None) -> int:
+        """Queue host-memory requests (rrxServerSubmitHost); returns the first
+        ticket.  The tensors must stay alive until the requests complete."""
+        n = len(requests)
+        arr = (rr.RRxHostIntersectRequest * max(n, 1))()
+        accel_ptrs = {}
+        for k, q in enumerate(requests):
+            if q.rays.is_cuda or q.hits.is_cuda:
+                raise ValueError(f"request {k}: host requests take host tensors")
+            count = int(q.rays.shape[0])
+            if q.rays.dim() != 2 or int(q.rays.shape[1]) != 8 or q.rays.element_size() != 4 or not q.rays.is_contiguous():
+                raise ValueError(f"request {k}: rays must be a contiguous float32 [R, 8] tensor")
+            if q.hits.numel() * q.hits.element_size() < count * _HIT_BYTES[q.output] or not q.hits.is_contiguous():
+                raise ValueError(f"request {k}: hit buffer too small")
+            if q.accel.address not in accel_ptrs:
+                accel_ptrs[q.accel.address] = self._ptr(q.accel.address)
+            arr[k] = rr.RRxHostIntersectRequest(accel_ptrs[q.accel.address], c_void_p(q.rays.data_ptr()),
+                                                c_void_p(q.hits.data_ptr()), count, _QUERY[q.query],
+                                                _OUTPUT[q.output], 0)
+        rel = None
+        if release_ns is not None:
+            rel = (c_uint64 * max(n, 1))(*[int(x) for x in release_ns])
+        first = c_uint64()
+        rr.check(self.lib.rrxServerSubmitHost(self.handle, arr, rel, n, byref(first)), "rrxServerSubmitHost")
         return first.value
 
     def drain(self) -> None:

@@ -55,7 +55,8 @@ def test_struct_layouts_match_the_abi():
     assert ctypes.sizeof(rr.RRMemoryRequirements) == 24
     # include/radeonrays_ext.h
     assert ctypes.sizeof(rr.RRxIntersectRequest) == 40
-    assert ctypes.sizeof(rr.RRxServerOptions) == 24
+    assert ctypes.sizeof(rr.RRxServerOptions) == 32
+    assert ctypes.sizeof(rr.RRxHostIntersectRequest) == 40
     assert ctypes.sizeof(rr.RRxServedTiming) == 32
 
 

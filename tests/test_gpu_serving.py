@@ -166,3 +166,54 @@ def test_server_rejects_oversized_requests(ctx, dev, small_world):
     with serving.Server(ctx, mode="batched", max_batch_rays=1 << 16, memory_budget=1000 * 48 - 1) as srv:
         with pytest.raises(Exception):
             srv.submit([big])
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_host_requests_staged_through_a_bounded_arena(ctx, dev, small_world, mode):
+    """Host-memory requests (rrxServerSubmitHost) under a staging arena far
+    smaller than the requests in all: every request's hits equal its own
+    rrCmdIntersect call, admission never holds more than the arena, and the
+    server coalesces requests that were released together."""
+    import torch
+
+    blas, bufs, tlas, pools, dpools, kinds = small_world
+    say(f"host requests, {mode} server, 2 MiB staging arena")
+    fake_scenes = [scenes.ServingScene(k, None, None) for k in kinds]
+    reqs = scenes.serving_requests(150, fake_scenes, 1 << 16, lo=10, hi=1 << 15, stream=7)
+    outputs = [("full", "full_t", "instance")[k % 3] for k in range(len(reqs))]
+    host_rays = [torch.from_numpy(pools[q.pool][q.offset:q.offset + q.count].copy()).pin_memory() for q in reqs]
+    host_hits = [torch.zeros(q.count * HIT_BYTES[o], dtype=torch.uint8).pin_memory() for q, o in zip(reqs, outputs)]
+    served = [serving.HostRequest(blas[q.scene], r, q.query, o, h)
+              for q, r, o, h in zip(reqs, host_rays, outputs, host_hits)]
+    arena = 2 << 20
+    total = sum(q.count * (32 + HIT_BYTES[o]) for q, o in zip(reqs, outputs))
+    assert total > 4 * arena
+    with serving.Server(ctx, mode=mode, max_inflight=3, max_batch_requests=64, max_batch_rays=1 << 15,
+                        memory_budget=arena, staging_bytes=arena) as srv:
+        first = srv.submit_host(served)
+        srv.drain()
+        t = srv.timings(first, len(served))
+        peak = srv.peak_admitted()
+    assert 0 < peak <= arena
+    if mode == "batched":
+        assert t["batch_size"].max() > 1
+    else:
+        assert (t["batch_size"] == 1).all()
+    for k, (q, o) in enumerate(zip(reqs, outputs)):
+        ref = _alone(ctx, blas[q.scene], dpools[q.pool][q.offset:q.offset + q.count], q.query, o)
+        assert np.array_equal(host_hits[k].numpy(), ref[: q.count * HIT_BYTES[o]]), f"request {k}"
+
+
+def test_host_requests_need_an_arena(ctx, dev, small_world):
+    import torch
+
+    blas, bufs, tlas, pools, dpools, kinds = small_world
+    rays = torch.from_numpy(pools[0][:100].copy())
+    hits = torch.zeros(100 * 16, dtype=torch.uint8)
+    req = serving.HostRequest(blas[0], rays, "closest", "full", hits)
+    with serving.Server(ctx, mode="batched") as srv:  # no staging arena
+        with pytest.raises(Exception):
+            srv.submit_host([req])
+    with serving.Server(ctx, mode="batched", staging_bytes=4096) as srv:  # arena smaller than the request
+        with pytest.raises(Exception):
+            srv.submit_host([req])
