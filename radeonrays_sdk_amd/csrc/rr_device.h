@@ -224,6 +224,11 @@ RR_DEV uint64_t LaneMaskLt() {
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
+// Number of set bits of m below this lane (v_mbcnt: no lane mask held in registers).
+RR_DEV uint32_t RankIn(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 template <typename T>
 RR_DEV T WaveMin(T v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));

@@ -65,6 +65,30 @@ __device__ __forceinline__ bool HeaderIs(const AccelHeader* h, uint32_t want, ui
     return false;
 }
 
+// Per-lane helper state of the persistent closest-hit kernel.
+constexpr uint32_t kNoHelp = 0x7Fu, kHelpDone = 0x80u, kWaiting = 0x100u, kHelper1 = 0x10000u;
+
+// A lane that owns a ray it has not finished, or that helps with another
+// lane's ray and has not finished that subtree.
+__device__ __forceinline__ bool Traversing(uint32_t rid, uint32_t help) {
+    return rid != kInvalid ? (help & kWaiting) == 0 : help != kNoHelp && (help & kHelpDone) == 0;
+}
+
+// Position of the n-th (from 0) set bit of m; m must have more than n set bits.
+__device__ __forceinline__ uint32_t NthSetBit(uint64_t m, uint32_t n) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << w) - 1ull));
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += (uint32_t)w;
+        }
+    }
+    return pos;
+}
+
 // LDS ring of DEPTH entries + scratch spill stack for one lane.
 template <int DEPTH = kLdsStackEntries>
 struct LaneStack {
@@ -73,21 +97,22 @@ struct LaneStack {
     uint32_t (*lds)[kBlock];
     uint32_t* spill;  // this block's spill slots (uniform): lane t's are [t * kSpillEntries, (t + 1) * kSpillEntries)
     uint32_t* err;    // device error words (may be null)
-    int sp;     // entries on the stack
-    int gbase;  // entries [0, gbase) live in scratch, [gbase, sp) in LDS
+    int sp;     // entries pushed so far (index of the next push)
+    int gbase;  // entries [bot, gbase) live in scratch, [gbase, sp) in LDS
+    int bot = 0;  // entries [0, bot) were handed to other lanes (take_bottom)
 
     // A 32-bit lane offset from the block's base: one register, not a 64-bit pointer per lane.
     __device__ __forceinline__ uint32_t* slot(int k) const { return spill + (threadIdx.x * (uint32_t)kSpillEntries + (uint32_t)k); }
 
-    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ bool empty() const { return sp == bot; }
+    __device__ __forceinline__ void clear() { sp = gbase = bot = 0; }
     __device__ __forceinline__ void push(uint32_t v) {
         if (sp == kStackCapacity) {
             // Tree deeper than the stack: report it and end the ray (its
             // deferred nodes, and in a scene the instance sentinel, are
             // discarded, so the lane finishes the current descent and stops).
             if (err) StG(err + kErrStackOverflow, 1u);
-            sp = 0;
-            gbase = 0;
+            clear();
             return;
         }
         if (sp - gbase == DEPTH) {
@@ -106,6 +131,19 @@ struct LaneStack {
             gbase = sp;
             v = LdG(slot(sp));
         }
+        return v;
+    }
+    // The oldest deferred entry (the largest remaining subtree), removed from
+    // the bottom; the stack must not be empty.
+    __device__ __forceinline__ uint32_t take_bottom() {
+        uint32_t v;
+        if (bot < gbase) {
+            v = LdG(slot(bot));
+        } else {
+            v = lds[bot & (DEPTH - 1)][threadIdx.x];
+            ++gbase;
+        }
+        ++bot;
         return v;
     }
 };
@@ -303,7 +341,6 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     const bool boxes = (hdr->flags & kAccelFlagAabbs) != 0;
     uint32_t* counters = scratch;
     const uint32_t lane = __lane_id();
-    const uint64_t lt = LaneMaskLt();
     const uint32_t part = (n + kQueues - 1) / kQueues;
 
     // Wave-uniform queue state.
@@ -313,11 +350,22 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     // Lane state.
     uint32_t rid = kInvalid;
     RayState r;
-    float tmax = 0.0f, best_t = 0.0f, limit = 0.0f, bV = 0.0f, bW = 0.0f, bdet = 1.0f;
+    // best_t starts at the ray's tmax and changes only with best_prim, so on
+    // a miss it still holds tmax.  bU, bV: barycentrics of the best hit so
+    // far (divided by the determinant when the hit is taken).
+    float best_t = 0.0f, bU = 0.0f, bV = 0.0f;
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
     LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries,
                         err, 0, 0};
+    // Closest hit, once the queues are drained: a lane without a ray may
+    // help a lane of its wave by traversing a subtree that lane deferred
+    // (see the drain step below).  One register: bits 0-6 the lane whose
+    // ray this lane helps with (kNoHelp: none), kHelpDone once that subtree
+    // is traversed, kWaiting on a lane whose own ray is traversed while
+    // helpers still work on it, bits 16-23 how many lanes help with this
+    // lane's ray.
+    uint32_t help = kNoHelp;
 
     for (;;) {
         // Refill idle lanes.
@@ -342,28 +390,98 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 }
             }
             const uint32_t take = min((uint32_t)__popcll(idle), q_end - q_next);
-            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            const uint32_t rank = RankIn(idle);
             if (rid == kInvalid && rank < take) {
                 rid = q_next + rank;
                 const RayIn ray = LoadRay(rays, rid);
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
-                tmax = ray.b.w;
-                best_t = tmax;
+                best_t = ray.b.w;
                 best_prim = kInvalid;
-                bV = 0.0f; bW = 0.0f; bdet = 1.0f;
-                limit = WidenLimit(tmax);
-                st.sp = 0;
-                st.gbase = 0;
+                bU = 0.0f; bV = 0.0f;
+                st.clear();
                 node = total ? 0u : kInvalid;
                 leaf = kInvalid;
             }
             q_next += take;
         }
-        if (__ballot(rid != kInvalid) == 0) {
+        if (!ANY && drained) {
+            // Drain step (the whole wave, no lane diverged here).  Closest
+            // hits do not depend on the order subtrees are visited in (ties
+            // go to the lowest primitive), so a long ray's deferred subtrees
+            // can be traversed by idle lanes of its wave, each with the
+            // ray's state and its own stack; their best hits are merged into
+            // the ray's lane one lane at a time, and the ray's hit is
+            // written once it and every helper are done.
+            // (a) Merge finished helpers into the lanes whose rays they helped.
+            for (uint64_t fin = __ballot((help & kHelpDone) != 0); fin; fin &= fin - 1) {
+                const int f = __builtin_ctzll(fin);
+                const uint32_t o = (uint32_t)__shfl((int)(help & kNoHelp), f, 64);
+                const float ft = __shfl(best_t, f, 64);
+                const uint32_t fp = (uint32_t)__shfl((int)best_prim, f, 64);
+                const float fu = __shfl(bU, f, 64), fv = __shfl(bV, f, 64);
+                if (lane == o) {
+                    if (fp != kInvalid && CloserHit(ft, 0u, fp, best_t, 0u, best_prim)) {
+                        best_t = ft; best_prim = fp;
+                        bU = fu; bV = fv;
+                    }
+                    help -= kHelper1;
+                }
+                if (lane == (uint32_t)f) help = kNoHelp;
+            }
+            // (b) Rays whose lane and helpers are all done.
+            if ((help & kWaiting) && help < kHelper1) {
+                const bool hit = best_prim != kInvalid;
+                WriteHit<OUT, false>(hits, rid, hit, hit ? bU : 0.0f, hit ? bV : 0.0f, 0u, best_prim,
+                                     best_t, best_t);
+                rid = kInvalid;
+                help = kNoHelp;
+            }
+            // (c) Idle lanes take the oldest deferred subtree of a lane still
+            //     traversing (the k-th idle lane from the k-th such lane).
+            const bool traversing = Traversing(rid, help);
+            const uint64_t idle_lanes = __ballot(rid == kInvalid && help == kNoHelp);
+            const uint64_t donors = __ballot(traversing && !st.empty());
+            const uint32_t pairs = min((uint32_t)__popcll(idle_lanes), (uint32_t)__popcll(donors));
+            if (pairs) {
+                uint32_t give = kInvalid, give_owner = 0;
+                if (((donors >> lane) & 1u) && RankIn(donors) < pairs) {
+                    give = st.take_bottom();
+                    give_owner = rid != kInvalid ? lane : (help & kNoHelp);
+                }
+                const uint32_t irank = RankIn(idle_lanes);
+                const bool takes = ((idle_lanes >> lane) & 1u) && irank < pairs;
+                const int src = takes ? (int)NthSetBit(donors, irank) : (int)lane;
+                const uint32_t node_in = (uint32_t)__shfl((int)give, src, 64);
+                const uint32_t owner_in = (uint32_t)__shfl((int)give_owner, src, 64);
+                // A lane that takes nothing reads its own values (src == lane).
+                r.ox = __shfl(r.ox, src, 64); r.oy = __shfl(r.oy, src, 64); r.oz = __shfl(r.oz, src, 64);
+                r.dx = __shfl(r.dx, src, 64); r.dy = __shfl(r.dy, src, 64); r.dz = __shfl(r.dz, src, 64);
+                r.tmin = __shfl(r.tmin, src, 64);
+                r.ix = __shfl(r.ix, src, 64); r.iy = __shfl(r.iy, src, 64); r.iz = __shfl(r.iz, src, 64);
+                r.sx = __shfl(r.sx, src, 64); r.sy = __shfl(r.sy, src, 64); r.sz = __shfl(r.sz, src, 64);
+                r.k = (uint32_t)__shfl((int)r.k, src, 64);
+                best_t = __shfl(best_t, src, 64);
+                if (takes) {
+                    help = owner_in;
+                    best_prim = kInvalid;
+                    bU = 0.0f; bV = 0.0f;
+                    st.clear();
+                    node = node_in < total ? node_in : kInvalid;
+                    leaf = kInvalid;
+                }
+                // Each ray's lane counts its new helpers.
+                for (uint64_t nh = __ballot(takes); nh; nh &= nh - 1) {
+                    const uint32_t o = (uint32_t)__shfl((int)help, __builtin_ctzll(nh), 64);
+                    if (lane == o) help += kHelper1;
+                }
+            }
+        }
+        if (__ballot(rid != kInvalid || help != kNoHelp) == 0) {
             if (drained) break;
             continue;
         }
-        if (rid == kInvalid) continue;
+        if (!Traversing(rid, help)) continue;
+        const uint64_t entered = __ballot(1);
 
         // Traverse until the ray ends or fewer than min_busy lanes are still
         // traversing, then go back and hand new rays to the idle lanes.
@@ -375,6 +493,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 const uint32_t c0 = __float_as_uint(q0.w);
                 const uint32_t c1 = __float_as_uint(q1.w);
                 float ta, tb;
+                const float limit = WidenLimit(best_t);  // recomputed: one register fewer across the loop
                 const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
                 const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
                 if (h0 || h1) {
@@ -395,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
             }
             // Phase B: the postponed leaf, then any leaves that come next in order.
             if (leaf == kInvalid && node != kInvalid && node >= n_int) {
-                leaf = node;
+                leaf = node < total ? node : kInvalid;  // corrupt index: dropped
                 node = st.empty() ? kInvalid : st.pop();
             }
             bool finished = false;
@@ -408,14 +527,13 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                     h.t >= r.tmin && h.t <= best_t) {
                     if (ANY) {
                         best_t = h.t; best_prim = prim;
-                        bV = h.V; bW = h.W; bdet = h.det;
+                        bU = h.V / h.det; bV = h.W / h.det;
                         finished = true;
                         break;
                     }
                     if (CloserHit(h.t, 0u, prim, best_t, 0u, best_prim)) {
                         best_t = h.t; best_prim = prim;
-                        bV = h.V; bW = h.W; bdet = h.det;
-                        limit = WidenLimit(best_t);
+                        bU = h.V / h.det; bV = h.W / h.det;
                     }
                 }
                 leaf = kInvalid;
@@ -425,15 +543,26 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 }
             }
             if (finished || (node == kInvalid && leaf == kInvalid)) {
-                const bool hit = best_prim != kInvalid;
-                WriteHit<OUT, false>(hits, rid, hit, hit ? bV / bdet : 0.0f, hit ? bW / bdet : 0.0f, 0u, best_prim,
-                                     best_t, tmax);
-                rid = kInvalid;
                 node = kInvalid;
                 leaf = kInvalid;
+                if (!ANY && rid == kInvalid) {
+                    help |= kHelpDone;  // a helper: merged into the ray's lane by the drain step
+                    break;
+                }
+                if (!ANY && help >= kHelper1) {
+                    help |= kWaiting;  // written by the drain step once the helpers are merged
+                    break;
+                }
+                const bool hit = best_prim != kInvalid;
+                WriteHit<OUT, false>(hits, rid, hit, hit ? bU : 0.0f, hit ? bV : 0.0f, 0u, best_prim,
+                                     best_t, best_t);
+                rid = kInvalid;
                 break;
             }
             if (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy) break;  // refill (nothing left once drained)
+            // Drained: back to the drain step once a lane of the wave has
+            // finished (it may help) while another still holds deferred work.
+            if (!ANY && drained && __ballot(1) != entered && __any(!st.empty())) break;
         }
     }
 }
@@ -533,8 +662,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                 best_prim = kInvalid;
                 best_inst = kInvalid;
                 bu = 0.0f; bv = 0.0f;
-                st.sp = 0;
-                st.gbase = 0;
+                st.clear();
                 inst = kInvalid;
                 cur = top_nodes;
                 cur_int = top_int;
@@ -767,8 +895,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 best_t = ray.b.w;
                 best_prim = kInvalid;
                 bu = 0.0f; bv = 0.0f;
-                st.sp = 0;
-                st.gbase = 0;
+                st.clear();
                 node = total ? 0u : kInvalid;
                 leaf = kInvalid;
             }
@@ -806,7 +933,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 if (!__any(leaf == kInvalid) || (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy)) break;
             }
             if (leaf == kInvalid && node != kInvalid && node >= n_int) {
-                leaf = node;
+                leaf = node < total ? node : kInvalid;  // corrupt index: dropped
                 node = st.empty() ? kInvalid : st.pop();
             }
             bool finished = false;
@@ -995,34 +1122,39 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
     const uint8_t* a = static_cast<const uint8_t*>(accel);
     const float4* r = static_cast<const float4*>(rays);
     uint32_t* sc = static_cast<uint32_t*>(scratch);
-    // kind 0 (not known on the host): launch both levels; each kernel reads
-    // the header and the one for the other level returns at once.
-    const bool single = kind != kAccelTlas, two = kind != kAccelBlas;
+    // Both levels are launched every time: each kernel reads the level from
+    // the buffer's header and the one for the other level returns at once.
+    // `kind` is the level the context recorded when it built this address;
+    // it is only a hint, because the caller may have freed that buffer and
+    // put another one (a copy, a build elsewhere) at the same address, so it
+    // decides which persistent kernel gets the full grid and which gets one
+    // block (all that an exiting kernel needs; a persistent kernel with one
+    // block still traces every ray, slowly, if the hint was stale).
     if (TraceMode() != kTraceModePlain) {
         // At most ceil(ray_count / kBlock) blocks: TraceScratchBytes holds
         // spill slots for that many lanes.  The queues are set up once: a
         // kernel of the wrong level returns before touching them.
         QueueInitKernel<<<1, 64, 0, s>>>(sc, ray_count, indirect);
         const uint32_t cap = (ray_count + kBlock - 1) / kBlock;
-        if (single) {
-            const uint32_t blocks = std::min(PersistentBlocks(false), cap);
-            if (query == kQueryAny) LaunchPersistent<true, false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
-            else LaunchPersistent<false, false>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
-        }
-        if (two) {
-            const uint32_t blocks = std::min(PersistentBlocks(true), cap);
-            if (query == kQueryAny) LaunchPersistent<true, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
-            else LaunchPersistent<false, true>(output, blocks, s, a, r, ray_count, indirect, hits, sc);
+        const uint32_t b1 = kind == kAccelTlas ? 1u : std::min(PersistentBlocks(false), cap);
+        const uint32_t b2 = kind == kAccelBlas ? 1u : std::min(PersistentBlocks(true), cap);
+        if (query == kQueryAny) {
+            LaunchPersistent<true, false>(output, b1, s, a, r, ray_count, indirect, hits, sc);
+            LaunchPersistent<true, true>(output, b2, s, a, r, ray_count, indirect, hits, sc);
+        } else {
+            LaunchPersistent<false, false>(output, b1, s, a, r, ray_count, indirect, hits, sc);
+            LaunchPersistent<false, true>(output, b2, s, a, r, ray_count, indirect, hits, sc);
         }
         return;
     }
+    // One lane per ray: both grids cover every ray.
     const dim3 g((ray_count + kBlock - 1) / kBlock);
     if (query == kQueryAny) {
-        if (single) LaunchOut<true, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
-        if (two) LaunchOut<true, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
+        LaunchOut<true, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
+        LaunchOut<true, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
     } else {
-        if (single) LaunchOut<false, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
-        if (two) LaunchOut<false, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
+        LaunchOut<false, false, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
+        LaunchOut<false, true, false>(output, g, s, a, r, ray_count, indirect, hits, sc, nullptr);
     }
 }
 
@@ -1033,14 +1165,14 @@ void TraceWithStats(const void* accel, AccelKind kind, QueryKind query, const vo
     const uint8_t* a = static_cast<const uint8_t*>(accel);
     const float4* r = static_cast<const float4*>(rays);
     uint32_t* sc = static_cast<uint32_t*>(scratch);
-    const bool single = kind != kAccelTlas, two = kind != kAccelBlas;
+    (void)kind;  // a hint only (see Trace): both levels run, the other one returns at once
     uint32_t* err = DeviceErrorWords();
     if (query == kQueryAny) {
-        if (single) TraceKernel<true, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
-        if (two) TraceKernel<true, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
+        TraceKernel<true, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
+        TraceKernel<true, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
     } else {
-        if (single) TraceKernel<false, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
-        if (two) TraceKernel<false, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
+        TraceKernel<false, kOutFullHitT, false, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
+        TraceKernel<false, kOutFullHitT, true, true><<<g, kBlock, 0, s>>>(a, r, ray_count, nullptr, hits, sc, stats, err);
     }
 }
 

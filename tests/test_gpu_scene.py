@@ -110,3 +110,51 @@ def test_persistent_two_level_equals_plain(ctx, dev, depth):
         assert ok, msg
     finally:
         autotune.apply(lib, saved)
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_trace_follows_the_header_not_the_recorded_level(ctx, dev, plain):
+    """The context records the level of every buffer it builds, by address,
+    and uses it to size the launches; the caller may free such a buffer and
+    put another at the same address (a copy, a build elsewhere, a host
+    upload), so traversal must follow the buffer's own header.  A BLAS
+    copied over a recorded TLAS, and a TLAS copied over a recorded BLAS,
+    trace exactly as the originals, in the persistent and the plain kernels
+    and with visit counters."""
+    say(f"level from the header over a recycled address, {'plain' if plain else 'persistent'} kernels")
+    import torch
+
+    tlas, blases, pick, xf, host = _scene(ctx, dev, 3, 300, 50, 400, 91)
+    rng = scenes.rng_for(2, 5091)
+    v, i = scenes.triangle_soup(100, rng, sigma=0.3)
+    blas = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev))
+    assert blas.buffer.numel() < tlas.buffer.numel(), "the BLAS must fit in the TLAS's buffer"
+    n = 20000
+    rays = scenes.make_rays(rng.uniform(-2, 2, size=(n, 3)).astype(np.float32), scenes.unit_sphere(n, rng))
+    drays = to_dev(rays, dev)
+    # A BLAS at an address recorded as a TLAS.
+    recorded = ctx.build_scene([blases[k] for k in pick], xf)
+    over_tlas = recorded.buffer
+    over_tlas[: blas.buffer.numel()].copy_(blas.buffer)
+    as_blas = api.Accel(over_tlas, "blas", 100)
+    # A TLAS at an address recorded as a BLAS (built into a buffer the TLAS fits in).
+    big = ctx.empty(tlas.buffer.numel() + 4096)
+    ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), result=big)
+    big[: tlas.buffer.numel()].copy_(tlas.buffer)
+    as_tlas = api.Accel(big, "tlas", len(pick), tlas.children)
+    torch.cuda.synchronize()
+    with autotune.plain_traversal(ctx.lib, plain):
+        for copy, orig in ((as_blas, blas), (as_tlas, tlas)):
+            for query in ("closest", "any"):
+                for out in ("full", "instance", "full_t"):
+                    got = ctx.intersect(copy, drays, query, out)
+                    ref = ctx.intersect(orig, drays, query, out)
+                    torch.cuda.synchronize()
+                    assert torch.equal(got, ref), f"{copy.kind} copy {query}/{out}"
+            gh, gs = ctx.intersect_with_stats(copy, drays, "closest")
+            rh, rs = ctx.intersect_with_stats(orig, drays, "closest")
+            torch.cuda.synchronize()
+            assert torch.equal(gh, rh) and torch.equal(gs, rs), f"{copy.kind} copy with stats"
+        for orig in (blas, tlas):
+            ref = api.decode_hits(ctx.intersect(orig, drays, "closest", "full"), "full")
+            assert (ref["prim_id"] != RR_INVALID).sum() > 200, orig.kind
