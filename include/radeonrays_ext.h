@@ -194,9 +194,10 @@ RR_API RRError rrxGetIntersectBatchMemoryRequirements(RRContext context,
 
 /* Run request_count queries as one submission.  Every geometry (BLAS)
  * request shares one persistent traversal launch whose waves take 64-ray
- * chunks of any request, so small requests fill the GPU together; scene
- * (TLAS) requests follow, one launch each.  Each request's hits are those an
- * rrCmdIntersect call with the same arguments writes.  No counterpart in the
+ * chunks of any request, so small requests fill the GPU together, and every
+ * scene (TLAS) request shares a second, two-level one; each launch takes
+ * the requests whose buffer header names its level.  Each request's hits
+ * are those an rrCmdIntersect call with the same arguments writes.  No counterpart in the
  * reference API (its rrCmdIntersect runs one query per dispatch,
  * vlk/intersector.cpp:269-339).  The request array is copied during the
  * call. */

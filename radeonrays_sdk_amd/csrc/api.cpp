@@ -1149,8 +1149,9 @@ RRError rrxGetIntersectBatchMemoryRequirements(RRContext context, uint32_t reque
     RR_REQUIRE(total_ray_count < (uint64_t(1) << 37), "rrxGetIntersectBatchMemoryRequirements");
     return Guarded("rrxGetIntersectBatchMemoryRequirements", [&]() -> RRError {
         DeviceGuard g(AsCtx(context)->device);
-        // Geometry requests share the batch launch; a scene request runs the
-        // per-ray-stack kernel on the same scratch afterwards.
+        // Every request shares the two batch launches (one per level).  The
+        // size never shrinks below what a single query of max_request_rays
+        // needs, which earlier versions ran scene requests with.
         *scratch_size = std::max(rr::TraceBatchScratchBytes(request_count, total_ray_count),
                                  rr::TraceScratchBytes(max_request_rays));
         return RR_SUCCESS;
@@ -1176,31 +1177,21 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
         DeviceGuard g(c->device);
-        // Split: geometry requests go into the batch table, scene requests
-        // run one launch each after it.
-        struct Single {
-            void* accel;
-            rr::AccelKind kind;
-            rr::QueryKind q;
-            rr::OutputKind o;
-            const void* rays;
-            uint32_t n;
-            void* hits;
-        };
+        // One table for every request: the geometry and the scene batch
+        // kernels each take the requests whose header names their level; the
+        // levels this context recorded only size the two launches.
         std::vector<rr::BatchRequest> table;
-        std::vector<Single> singles;
+        bool geometry_hint = false, scene_hint = false;
         uint64_t chunks = 0;
         for (uint32_t i = 0; i < request_count; ++i) {
             const RRxIntersectRequest& q = requests[i];
             if (q.ray_count == 0) continue;
             void* accel = Addr(q.scene_buffer);
             const rr::AccelKind kind = KindOf(c, accel);
+            geometry_hint |= kind != rr::kAccelTlas;
+            scene_hint |= kind != rr::kAccelBlas;
             const auto qk = q.query == RR_INTERSECT_QUERY_ANY ? rr::kQueryAny : rr::kQueryClosest;
             const auto ok = static_cast<rr::OutputKind>(q.query_output);
-            if (kind != rr::kAccelBlas) {
-                singles.push_back({accel, kind, qk, ok, Addr(q.rays), q.ray_count, Addr(q.hits)});
-                continue;
-            }
             rr::BatchRequest b{};
             b.accel = reinterpret_cast<uint64_t>(accel);
             b.rays = reinterpret_cast<uint64_t>(Addr(q.rays));
@@ -1224,10 +1215,8 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
                 auto* dev_table =
                     reinterpret_cast<rr::BatchRequest*>(static_cast<uint8_t*>(sc) + rr::kBatchTableOffset);
                 Check(hipMemcpyAsync(dev_table, host, table_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-                rr::TraceBatch(dev_table, n_req, n_chunks, sc, s);
+                rr::TraceBatch(dev_table, n_req, n_chunks, geometry_hint, scene_hint, sc, s);
             }
-            for (const Single& x : singles)
-                rr::Trace(x.accel, x.kind, x.q, x.o, x.rays, x.n, nullptr, x.hits, sc, s);
         };
         Record(cs, cmd);
         KeepStaging(c, cs, host);

@@ -57,7 +57,7 @@ enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };
 void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output, const void* rays, uint32_t ray_count,
            const uint32_t* indirect, void* hits, void* scratch, hipStream_t stream);
 
-// One request of a batched (coalesced) query: a single-level BVH, its rays
+// One request of a batched (coalesced) query: a BLAS or a scene, its rays
 // and its hit buffer.  flags: bit 0 any-hit, bits 1-2 OutputKind.
 struct BatchRequest {
     uint64_t accel;
@@ -76,9 +76,13 @@ constexpr uint32_t kBatchChunk = 64;
 size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays);
 constexpr size_t kBatchTableOffset = 1024;
 
-// Trace every request of the table (already in device memory) in one
-// persistent launch; requests may name different BVHs, queries and outputs.
-void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, void* scratch, hipStream_t stream);
+// Trace every request of the table (already in device memory): one
+// persistent launch per level, each taking the requests whose header names
+// its level; requests may name different BVHs, levels, queries and outputs.
+// geometry_hint / scene_hint: some request was recorded as a BLAS (or not
+// recorded) / as a scene (or not recorded); they size the launches only.
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, bool geometry_hint,
+                bool scene_hint, void* scratch, hipStream_t stream);
 
 // Same traversal with per-ray visit counters written to `stats`
 // (uint32 x4 per ray: internal nodes visited, leaves visited, box tests,

@@ -823,15 +823,58 @@ void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, cons
 }
 
 // ---------------------------------------------------------------------------
-// Batched traversal: one persistent launch over many requests.  Waves take
-// 64-ray chunks from one counter; a chunk belongs to one request (found by
-// binary search over first_chunk), and each lane carries its own request's
-// BVH, query and output, so lanes of a wave may serve different requests.
+// Batched traversal: one persistent launch per level over many requests.
+// Waves take 64-ray chunks from a counter; a chunk belongs to one request
+// (found by binary search over first_chunk), and each lane carries its own
+// request's BVH, query and output, so lanes of a wave may serve different
+// requests.  Both levels' kernels walk the same request table and each
+// takes the requests whose buffer header is its level (the geometry kernel
+// also takes buffers without a valid header, to report them), so the level
+// the host recorded for an address is never trusted for correctness.
+
+// Counter words of the two batch kernels in the batch scratch (separate
+// 256-byte lines, inside the table offset).
+constexpr uint32_t kBatchCounterWord = 0, kBatch2LCounterWord = 64;
+
+// Next chunk of a request this kernel takes, as [q_next, q_end) of request
+// q_req; false once every chunk is handed out.  A chunk of a request of the
+// other level moves the counter to the next request's first chunk (one
+// atomicMax for the whole request, instead of one fetch per chunk).  The
+// counter only grows, so the loop ends.  Wave-uniform: every lane calls it.
+template <bool TWO_LEVEL>
+__device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ table, uint32_t requests,
+                                               uint32_t total_chunks, uint32_t* counter, uint32_t lane,
+                                               uint32_t& q_req, uint32_t& q_next, uint32_t& q_end) {
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(counter, 1u);
+        c = __shfl(c, 0, 64);
+        if (c >= total_chunks) return false;
+        uint32_t lo = 0, hi = requests;  // last request with first_chunk <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (table[mid].first_chunk <= c) lo = mid;
+            else hi = mid;
+        }
+        const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(table[lo].accel);
+        const bool scene = LdG(&hdr->magic) == kAccelMagic && LdG(&hdr->kind) == kAccelTlas;
+        if (scene != TWO_LEVEL) {
+            const uint32_t next = lo + 1 < requests ? table[lo + 1].first_chunk : total_chunks;
+            if (lane == 0) atomicMax(counter, next);
+            continue;
+        }
+        q_req = lo;
+        q_next = (c - table[lo].first_chunk) * kBatchChunk;
+        q_end = min(q_next + kBatchChunk, table[lo].ray_count);
+        return true;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* __restrict__ table, uint32_t requests,
                                                            uint32_t total_chunks, uint32_t* __restrict__ scratch,
                                                            uint32_t min_busy, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
-    uint32_t* counter = scratch;
+    uint32_t* counter = scratch + kBatchCounterWord;
     const uint32_t lane = __lane_id();
     const uint64_t lt = LaneMaskLt();
     const size_t spill_base = (kBatchTableOffset + (((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255)) / 4;
@@ -853,24 +896,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
     for (;;) {
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
-            if (q_next == q_end) {
-                uint32_t c = 0;
-                if (lane == 0) c = atomicAdd(counter, 1u);
-                c = __shfl(c, 0, 64);
-                if (c >= total_chunks) {
-                    drained = true;
-                } else {
-                    uint32_t lo = 0, hi = requests;  // last request with first_chunk <= c
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) / 2;
-                        if (table[mid].first_chunk <= c) lo = mid;
-                        else hi = mid;
-                    }
-                    q_req = lo;
-                    q_next = (c - table[lo].first_chunk) * kBatchChunk;
-                    q_end = min(q_next + kBatchChunk, table[lo].ray_count);
-                }
-            }
+            if (q_next == q_end) drained = !NextBatchChunk<false>(table, requests, total_chunks, counter, lane, q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = (uint32_t)__popcll(idle & lt);
             if (rid == kInvalid && rank < take) {
@@ -882,7 +908,8 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
                 total = LdG(&hdr->node_count);
                 if (LdG(&hdr->magic) != kAccelMagic || LdG(&hdr->kind) != kAccelBlas) {
-                    // Not a geometry this library built: report it, answer misses.
+                    // Not a buffer this library built (scenes go to the
+                    // two-level kernel): report it, answer misses.
                     if (err) StG(err + kErrBadAccel, 1u);
                     total = 0;
                 }
@@ -979,8 +1006,216 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
     }
 }
 
-__global__ void BatchInitKernel(uint32_t* counter) {
-    if (threadIdx.x == 0) *counter = 0;
+// The two-level batch kernel: the scene requests of the table, with the
+// persistent two-level kernel's loop (instances entered at once, the world
+// ray in LDS, postponed leaves, the sentinel never popped while a leaf is
+// postponed), so each request's hits equal a TracePersistent2LKernel launch
+// of its own.  Per lane: the request's TLAS (its header is re-read on
+// instance entry and exit, which are rare next to node visits), query and
+// output.
+__global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest* __restrict__ table, uint32_t requests,
+                                                             uint32_t total_chunks, uint32_t* __restrict__ scratch,
+                                                             uint32_t min_busy, uint32_t* err) {
+    __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
+    __shared__ float s_world[9][kBlock];  // world ray: origin, direction, reciprocal direction
+    uint32_t* counter = scratch + kBatch2LCounterWord;
+    const uint32_t lane = __lane_id();
+    const size_t spill_base = (kBatchTableOffset + (((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255)) / 4;
+
+    // Wave-uniform queue: rays [q_next, q_end) of request q_req.
+    uint32_t q_req = 0, q_next = 0, q_end = 0;
+    bool drained = false;
+
+    // Lane state.
+    uint32_t rid = kInvalid, lreq = 0, lflags = 0;
+    const AccelHeader* top = nullptr;  // this lane's scene
+    uint32_t top_int = 0, top_total = 0;
+    const BvhNode* cur = nullptr;
+    uint32_t cur_int = 0, cur_total = 0, inst = kInvalid;
+    bool boxes = false;
+    RayState r;
+    float best_t = 0.0f, bu = 0.0f, bv = 0.0f;  // best_t is t_max until a hit; culling limit WidenLimit(best_t)
+    uint32_t best_prim = kInvalid, best_inst = kInvalid;
+    uint32_t node = kInvalid, leaf = kInvalid;
+    LaneStack<> st{s_stack, scratch + spill_base + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
+
+    for (;;) {
+        const uint64_t idle = __ballot(rid == kInvalid);
+        if (idle && !drained) {
+            if (q_next == q_end) drained = !NextBatchChunk<true>(table, requests, total_chunks, counter, lane, q_req, q_next, q_end);
+            const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
+            const uint32_t rank = RankIn(idle);
+            if (rid == kInvalid && rank < take) {
+                rid = q_next + rank;
+                lreq = q_req;
+                const BatchRequest& rq = table[q_req];
+                lflags = rq.flags;
+                top = reinterpret_cast<const AccelHeader*>(rq.accel);
+                top_total = LdG(&top->node_count);
+                const uint32_t prims = LdG(&top->prim_count);
+                top_int = (prims && top_total) ? prims - 1 : 0;
+                const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
+                s_world[0][threadIdx.x] = ray.a.x; s_world[1][threadIdx.x] = ray.a.y; s_world[2][threadIdx.x] = ray.a.z;
+                s_world[3][threadIdx.x] = ray.b.x; s_world[4][threadIdx.x] = ray.b.y; s_world[5][threadIdx.x] = ray.b.z;
+                SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
+                s_world[6][threadIdx.x] = r.ix; s_world[7][threadIdx.x] = r.iy; s_world[8][threadIdx.x] = r.iz;
+                best_t = ray.b.w;
+                best_prim = kInvalid;
+                best_inst = kInvalid;
+                bu = 0.0f; bv = 0.0f;
+                st.clear();
+                inst = kInvalid;
+                cur = reinterpret_cast<const BvhNode*>(reinterpret_cast<const uint8_t*>(top) + LdG(&top->nodes_offset));
+                cur_int = top_int;
+                cur_total = top_total;
+                boxes = false;
+                node = top_total ? 0u : kInvalid;
+                leaf = kInvalid;
+            }
+            q_next += take;
+        }
+        if (__ballot(rid != kInvalid) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (rid == kInvalid) continue;
+
+        const bool any = (lflags & 1u) != 0;
+        for (;;) {
+            // Phase A: inner nodes, instance entries and exits.
+            for (;;) {
+                if (node < cur_int) {
+                    float4 q0, q1, q2, q3;
+                    LoadNode(cur, node, q0, q1, q2, q3);
+                    const uint32_t c0 = __float_as_uint(q0.w);
+                    const uint32_t c1 = __float_as_uint(q1.w);
+                    float ta, tb;
+                    const float limit = WidenLimit(best_t);
+                    const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
+                    const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
+                    if (h0 || h1) {
+                        const bool c1first = h1 && (!h0 || ta > tb);
+                        if (h0 && h1) st.push(c1first ? c0 : c1);
+                        node = c1first ? c1 : c0;
+                    } else {
+                        node = st.empty() ? kInvalid : st.pop();
+                    }
+                } else if (inst == kInvalid && node < cur_total) {
+                    // Instance leaf: enter it (isect_2l.comp:222-237).
+                    const uint8_t* base = reinterpret_cast<const uint8_t*>(top);
+                    const uint32_t id = __float_as_uint(LdG4(&cur[node].a0_max_or_v1).w);
+                    const uint8_t* blas = nullptr;
+                    float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0;
+                    uint32_t bn = 0, bflags = 0;
+                    uint64_t boff = 0;
+                    if (id < LdG(&top->prim_count)) {
+                        const Transform* xforms = reinterpret_cast<const Transform*>(base + LdG(&top->transforms_offset));
+                        const uint64_t* blas_table = reinterpret_cast<const uint64_t*>(base + LdG(&top->blas_table_offset));
+                        const float4* xp = reinterpret_cast<const float4*>(xforms + 2 * (size_t)id);
+                        x0 = LdG4(xp); x1 = LdG4(xp + 1); x2 = LdG4(xp + 2);
+                        blas = reinterpret_cast<const uint8_t*>(LdG(blas_table + id));
+                        if (blas) {
+                            const AccelHeader* bh = reinterpret_cast<const AccelHeader*>(blas);
+                            bn = LdG(&bh->node_count);
+                            boff = LdG(&bh->nodes_offset);
+                            bflags = LdG(&bh->flags);
+                        }
+                    }
+                    if (bn != 0) {
+                        const float wox = s_world[0][threadIdx.x], woy = s_world[1][threadIdx.x], woz = s_world[2][threadIdx.x];
+                        const float wdx = s_world[3][threadIdx.x], wdy = s_world[4][threadIdx.x], wdz = s_world[5][threadIdx.x];
+                        const float ox = ((x0.x * wox + x0.y * woy) + x0.z * woz) + x0.w;
+                        const float oy = ((x1.x * wox + x1.y * woy) + x1.z * woz) + x1.w;
+                        const float oz = ((x2.x * wox + x2.y * woy) + x2.z * woz) + x2.w;
+                        const float dx = (x0.x * wdx + x0.y * wdy) + x0.z * wdz;
+                        const float dy = (x1.x * wdx + x1.y * wdy) + x1.z * wdz;
+                        const float dz = (x2.x * wdx + x2.y * wdy) + x2.z * wdz;
+                        SetupRay(r, ox, oy, oz, r.tmin, dx, dy, dz);
+                        inst = id;
+                        cur = reinterpret_cast<const BvhNode*>(blas + boff);
+                        cur_total = bn;
+                        cur_int = (bn - 1) / 2;
+                        boxes = (bflags & kAccelFlagAabbs) != 0;
+                        st.push(kTopLevelSentinel);
+                        node = 0;
+                    } else {
+                        node = st.empty() ? kInvalid : st.pop();
+                    }
+                } else if (node == kTopLevelSentinel && leaf == kInvalid) {
+                    // Leave the instance, restore the world ray (isect_2l.comp:278-286).
+                    inst = kInvalid;
+                    cur = reinterpret_cast<const BvhNode*>(reinterpret_cast<const uint8_t*>(top) + LdG(&top->nodes_offset));
+                    cur_int = top_int;
+                    cur_total = top_total;
+                    boxes = false;
+                    r.ox = s_world[0][threadIdx.x]; r.oy = s_world[1][threadIdx.x]; r.oz = s_world[2][threadIdx.x];
+                    r.ix = s_world[6][threadIdx.x]; r.iy = s_world[7][threadIdx.x]; r.iz = s_world[8][threadIdx.x];
+                    node = st.empty() ? kInvalid : st.pop();
+                } else if (inst != kInvalid && node < cur_total && leaf == kInvalid) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                } else {
+                    break;  // a second leaf or the sentinel behind a postponed leaf, or done
+                }
+                // An index past the level's nodes (corrupt buffer) ends the ray.
+                if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
+                if (!__any(leaf == kInvalid) || (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy)) break;
+            }
+            // Phase B: the postponed leaf, then the leaves that follow it in order.
+            if (leaf == kInvalid && inst != kInvalid && node >= cur_int && node < cur_total) {
+                leaf = node;
+                node = st.empty() ? kInvalid : st.pop();
+                if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
+            }
+            bool finished = false;
+            while (leaf != kInvalid) {
+                float4 q0, q1, q2, q3;
+                LoadNode(cur, leaf, q0, q1, q2, q3);
+                const uint32_t prim = __float_as_uint(q1.w);
+                TriHit h;
+                if (LeafTest(boxes, r, best_t, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, h) &&
+                    h.t >= r.tmin && h.t <= best_t) {
+                    if (any) {
+                        best_t = h.t; best_prim = prim; best_inst = inst;
+                        bu = h.V / h.det; bv = h.W / h.det;
+                        finished = true;
+                        break;
+                    }
+                    if (CloserHit(h.t, inst, prim, best_t, best_inst, best_prim)) {
+                        best_t = h.t; best_prim = prim; best_inst = inst;
+                        bu = h.V / h.det; bv = h.W / h.det;
+                    }
+                }
+                leaf = kInvalid;
+                if (node >= cur_int && node < cur_total) {
+                    leaf = node;
+                    node = st.empty() ? kInvalid : st.pop();
+                    if (node != kInvalid && node != kTopLevelSentinel && node >= cur_total) node = kInvalid;
+                }
+            }
+            if (finished || (node == kInvalid && leaf == kInvalid)) {
+                const bool hit = best_prim != kInvalid;
+                void* out = reinterpret_cast<void*>(table[lreq].hits);
+                switch ((lflags >> 1) & 3u) {  // a miss leaves best_t at t_max
+                    case kOutFullHit: WriteHit<kOutFullHit, true>(out, rid, hit, bu, bv, best_inst, best_prim, best_t, best_t); break;
+                    case kOutInstanceId: WriteHit<kOutInstanceId, true>(out, rid, hit, bu, bv, best_inst, best_prim, best_t, best_t); break;
+                    default: WriteHit<kOutFullHitT, true>(out, rid, hit, bu, bv, best_inst, best_prim, best_t, best_t); break;
+                }
+                rid = kInvalid;
+                node = kInvalid;
+                leaf = kInvalid;
+                break;
+            }
+            if (!drained && (uint32_t)__popcll(__ballot(1)) < min_busy) break;  // refill (nothing left once drained)
+        }
+    }
+}
+
+__global__ void BatchInitKernel(uint32_t* counters) {
+    if (threadIdx.x == 0) {
+        counters[kBatchCounterWord] = 0;
+        counters[kBatch2LCounterWord] = 0;
+    }
 }
 
 }  // namespace
@@ -1074,6 +1309,7 @@ uint32_t PersistentBlocks(bool two) {
 }
 
 uint32_t BatchBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatchKernel), 1); }
+uint32_t Batch2LBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatch2LKernel), 7); }
 
 // Rays per queue atomic for n rays over `blocks` blocks: the tuned value,
 // cut (to a multiple of 32, at least 32) when fewer rays than waves x fetch
@@ -1179,18 +1415,26 @@ void TraceWithStats(const void* accel, AccelKind kind, QueryKind query, const vo
 size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays) {
     // The launch has at most one block per 4 chunks, and the chunks number
     // fewer than total_rays / kBatchChunk + requests.
-    const uint64_t lanes = std::min<uint64_t>((uint64_t)BatchBlocks() * kBlock,
+    const uint64_t lanes = std::min<uint64_t>((uint64_t)std::max(BatchBlocks(), Batch2LBlocks()) * kBlock,
                                               AlignUp(total_rays + (uint64_t)requests * kBatchChunk, kBlock));
     return kBatchTableOffset + AlignUp((size_t)requests * sizeof(BatchRequest), 256) +
            AlignUp((size_t)lanes * kSpillEntries * 4, 256);
 }
 
-void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, void* scratch, hipStream_t s) {
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, bool geometry_hint,
+                bool scene_hint, void* scratch, hipStream_t s) {
     if (requests == 0 || total_chunks == 0) return;
     uint32_t* sc = static_cast<uint32_t*>(scratch);
-    const uint32_t blocks = std::min<uint64_t>(BatchBlocks(), ((uint64_t)total_chunks * kBatchChunk + kBlock - 1) / kBlock);
+    // Both kernels run; the hints (the levels the context recorded for the
+    // requests' addresses) only size their grids: a level with no recorded
+    // request gets one block, which skips the table (or, if a record was
+    // stale, still traces that level's requests).
+    const uint64_t want = ((uint64_t)total_chunks * kBatchChunk + kBlock - 1) / kBlock;
+    const uint32_t b1 = geometry_hint ? (uint32_t)std::min<uint64_t>(BatchBlocks(), want) : 1u;
+    const uint32_t b2 = scene_hint ? (uint32_t)std::min<uint64_t>(Batch2LBlocks(), want) : 1u;
     BatchInitKernel<<<1, 64, 0, s>>>(sc);
-    TraceBatchKernel<<<blocks, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes(), DeviceErrorWords());
+    TraceBatchKernel<<<b1, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes(), DeviceErrorWords());
+    TraceBatch2LKernel<<<b2, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes(), DeviceErrorWords());
 }
 
 }  // namespace rr

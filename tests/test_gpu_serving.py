@@ -217,3 +217,77 @@ def test_host_requests_need_an_arena(ctx, dev, small_world):
     with serving.Server(ctx, mode="batched", staging_bytes=4096) as srv:  # arena smaller than the request
         with pytest.raises(Exception):
             srv.submit_host([req])
+
+
+def test_scene_requests_share_the_batch_launch(ctx, dev, small_world):
+    """Scene (TLAS) requests go through the batched two-level kernel, mixed
+    with geometry requests in one table: every request equals its own
+    rrCmdIntersect call byte for byte, for both queries and all outputs, on
+    a scene of 300 overlapping instances that also holds instances of an
+    empty and of a one-triangle BLAS; batches of scenes only and of
+    geometries only (the other level's kernel then runs one block that skips
+    the table); and buffers copied over addresses the context recorded as
+    the other level (the kernels read the level from each header)."""
+    import torch
+
+    blas, bufs, tlas_small, pools, dpools, kinds = small_world
+    rng = scenes.rng_for(40, 500)
+    empty = ctx.build_geometry(to_dev(np.zeros((3, 3), np.float32), dev),
+                               to_dev(np.zeros((0, 3), np.int32), dev).reshape(0, 3))
+    v1, i1 = scenes.triangle_soup(1, rng, sigma=0.5)
+    one = ctx.build_geometry(to_dev(v1, dev), to_dev(i1.reshape(-1, 3), dev))
+    members = [blas[k] for k in rng.integers(0, len(blas), size=300)] + [empty, one] * 10
+    big = ctx.build_scene(members, scenes.random_transforms(rng, len(members), trans=0.6))
+    scenes_ = [big, tlas_small]
+    # Copies over recycled addresses: a geometry over a recorded scene, a scene over a recorded geometry.
+    recorded_scene = ctx.build_scene([blas[0]] * 200, scenes.random_transforms(rng, 200, trans=0.5))
+    assert blas[6].buffer.numel() <= recorded_scene.buffer.numel()
+    recorded_scene.buffer[: blas[6].buffer.numel()].copy_(blas[6].buffer)
+    geo_copy = api.Accel(recorded_scene.buffer, "blas", blas[6].prim_count)
+    holder = ctx.empty(tlas_small.buffer.numel() + 4096)
+    vv, ii = scenes.triangle_soup(10, rng)
+    ctx.build_geometry(to_dev(vv, dev), to_dev(ii, dev), result=holder)
+    holder[: tlas_small.buffer.numel()].copy_(tlas_small.buffer)
+    scene_copy = api.Accel(holder, "tlas", tlas_small.prim_count, tlas_small.children)
+    torch.cuda.synchronize()
+    copies = {id(geo_copy): blas[6], id(scene_copy): tlas_small}
+
+    def batch(targets, seed):
+        r = np.random.default_rng(seed)
+        reqs = []
+        for k, target in enumerate(targets):
+            n = int((0, 1, 64, 65, 1000, 4097, 20000)[k % 7])
+            off = int(r.integers(0, (1 << 16) - n + 1))
+            pool = int(r.integers(0, 4))
+            reqs.append((target, dpools[pool][off:off + n], ("closest", "any")[k % 2], ("full", "full_t", "instance")[k % 3]))
+        outs = ctx.intersect_batch(reqs)
+        torch.cuda.synchronize()
+        for k, (target, rays, query, output) in enumerate(reqs):
+            n = int(rays.shape[0])
+            if n == 0:
+                continue
+            ref = _alone(ctx, copies.get(id(target), target), rays, query, output)
+            got = outs[k].cpu().numpy()
+            assert np.array_equal(got[: n * HIT_BYTES[output]], ref[: n * HIT_BYTES[output]]), \
+                f"request {k} ({'scene' if target.kind == 'tlas' else 'geometry'}, {query}, {output}, {n} rays)"
+        return reqs
+
+    say("batched scene + geometry requests, scene-only and geometry-only batches, recycled addresses")
+    mixed = [scenes_[k % 2] if k % 3 else blas[k % len(blas)] for k in range(30)]
+    batch(mixed, 1)
+    batch([scenes_[k % 2] for k in range(14)], 2)
+    batch([blas[k % len(blas)] for k in range(14)], 3)
+    batch([geo_copy, scene_copy, blas[1], big, geo_copy, scene_copy], 4)
+    # Every record says geometry, one buffer is a scene (and the reverse):
+    # the other level's kernel runs one block and still answers it.
+    batch([blas[1], blas[4], scene_copy, scene_copy, scene_copy], 5)
+    batch([big, tlas_small, geo_copy, geo_copy, geo_copy], 6)
+    # The big scene's hits against the oracle on a sample.
+    rays = pools[0][:3000]
+    hits = ctx.intersect_batch([(big, to_dev(rays, dev), "closest", "full")])[0]
+    torch.cuda.synchronize()
+    host = {b.address: b.buffer.cpu().numpy() for b in members}
+    ref = oracle.trace(big.buffer.cpu().numpy(), rays, blas_map=host)
+    ok, msg = hits_equal(api.decode_hits(hits, "full"), ref, ("prim_id", "inst_id", "u", "v"))
+    assert ok, msg
+    assert (ref["inst_id"] != 0xFFFFFFFF).sum() > 300
