@@ -195,9 +195,12 @@ RR_API RRError rrxGetIntersectBatchMemoryRequirements(RRContext context,
 /* Run request_count queries as one submission.  Every geometry (BLAS)
  * request shares one persistent traversal launch whose waves take 64-ray
  * chunks of any request, so small requests fill the GPU together, and every
- * scene (TLAS) request shares a second, two-level one; each launch takes
- * the requests whose buffer header names its level.  Each request's hits
- * are those an rrCmdIntersect call with the same arguments writes.  No counterpart in the
+ * scene (TLAS) request shares a second, two-level one.  The level of a
+ * buffer this context built comes from the context's record of it, of any
+ * other buffer from its header; a buffer whose header disagrees with the
+ * record (another buffer put at a freed result's address) is reported as
+ * RRX_DEVICE_ERROR_INVALID_BUFFER and answered with misses.  Each request's
+ * hits are those an rrCmdIntersect call with the same arguments writes.  No counterpart in the
  * reference API (its rrCmdIntersect runs one query per dispatch,
  * vlk/intersector.cpp:269-339).  The request array is copied during the
  * call. */

@@ -994,10 +994,19 @@ RRError rrReleaseDevicePtr(RRContext context, RRDevicePtr ptr) {
         DevicePtr* p = AsPtr(ptr);
         LivePtrs().Remove(p);
         if (p->owned) {
-            DeviceGuard g(AsCtx(context)->device);
-            (void)hipStreamSynchronize(AsCtx(context)->stream);
+            Context* c = AsCtx(context);
+            DeviceGuard g(c->device);
+            (void)hipStreamSynchronize(c->stream);
             if (p->mapped) std::free(p->mapped);
             (void)hipFree(p->base);
+            // Forget the levels recorded for results built into this memory:
+            // the next allocation may reuse the addresses for anything.
+            std::lock_guard<std::mutex> lk(c->mu);
+            const uintptr_t lo = reinterpret_cast<uintptr_t>(p->base), hi = lo + p->size;
+            for (auto it = c->kinds.begin(); it != c->kinds.end();) {
+                if (it->first >= lo && it->first < hi) it = c->kinds.erase(it);
+                else ++it;
+            }
         }
         delete p;
         return RR_SUCCESS;
@@ -1177,19 +1186,17 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
         Context* c = AsCtx(context);
         CommandStream* cs = AsStream(command_stream);
         DeviceGuard g(c->device);
-        // One table for every request: the geometry and the scene batch
-        // kernels each take the requests whose header names their level; the
-        // levels this context recorded only size the two launches.
-        std::vector<rr::BatchRequest> table;
-        bool geometry_hint = false, scene_hint = false;
-        uint64_t chunks = 0;
+        // One table, ordered as: requests whose address this context
+        // recorded as a geometry, requests it has no record of (the kernels
+        // read their level from the header), requests recorded as a scene.
+        // The geometry kernel takes the first two groups, the two-level
+        // kernel the last two.
+        std::vector<rr::BatchRequest> group[3];
         for (uint32_t i = 0; i < request_count; ++i) {
             const RRxIntersectRequest& q = requests[i];
             if (q.ray_count == 0) continue;
             void* accel = Addr(q.scene_buffer);
             const rr::AccelKind kind = KindOf(c, accel);
-            geometry_hint |= kind != rr::kAccelTlas;
-            scene_hint |= kind != rr::kAccelBlas;
             const auto qk = q.query == RR_INTERSECT_QUERY_ANY ? rr::kQueryAny : rr::kQueryClosest;
             const auto ok = static_cast<rr::OutputKind>(q.query_output);
             rr::BatchRequest b{};
@@ -1198,13 +1205,25 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
             b.hits = reinterpret_cast<uint64_t>(Addr(q.hits));
             b.ray_count = q.ray_count;
             b.flags = (qk == rr::kQueryAny ? 1u : 0u) | (uint32_t(ok) << 1);
-            b.first_chunk = uint32_t(chunks);
-            chunks += (q.ray_count + rr::kBatchChunk - 1) / rr::kBatchChunk;
-            RR_REQUIRE(chunks < (uint64_t(1) << 32), "rrxCmdIntersectBatch");
-            table.push_back(b);
+            const int g = kind == rr::kAccelBlas ? 0 : kind == rr::kAccelTlas ? 2 : 1;
+            if (g == 1) b.flags |= rr::kBatchFlagCheckLevel;
+            group[g].push_back(b);
+        }
+        std::vector<rr::BatchRequest> table;
+        uint64_t chunks = 0, group_chunks[3] = {0, 0, 0};
+        for (int g = 0; g < 3; ++g) {
+            for (rr::BatchRequest& b : group[g]) {
+                b.first_chunk = uint32_t(chunks);
+                const uint64_t n = (b.ray_count + rr::kBatchChunk - 1) / rr::kBatchChunk;
+                chunks += n;
+                group_chunks[g] += n;
+                RR_REQUIRE(chunks < (uint64_t(1) << 32), "rrxCmdIntersectBatch");
+                table.push_back(b);
+            }
         }
         const uint32_t n_req = uint32_t(table.size());
-        const uint32_t n_chunks = uint32_t(chunks);
+        const uint32_t c_geo = uint32_t(group_chunks[0]), c_unrec = uint32_t(group_chunks[1]),
+                       c_scene = uint32_t(group_chunks[2]);
         void* sc = Addr(scratch);
         void* host = nullptr;
         const size_t table_bytes = sizeof(rr::BatchRequest) * n_req;
@@ -1215,7 +1234,7 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
                 auto* dev_table =
                     reinterpret_cast<rr::BatchRequest*>(static_cast<uint8_t*>(sc) + rr::kBatchTableOffset);
                 Check(hipMemcpyAsync(dev_table, host, table_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-                rr::TraceBatch(dev_table, n_req, n_chunks, geometry_hint, scene_hint, sc, s);
+                rr::TraceBatch(dev_table, n_req, c_geo, c_unrec, c_scene, sc, s);
             }
         };
         Record(cs, cmd);

@@ -71,18 +71,25 @@ struct BatchRequest {
 static_assert(sizeof(BatchRequest) == 48, "BatchRequest must be 48 bytes");
 constexpr uint32_t kBatchChunk = 64;
 
-// Scratch for a batch of `requests` requests and `total_rays` rays; the
-// request table is uploaded to byte kBatchTableOffset of it.
+// Scratch for a batch of `requests` requests and `total_rays` rays: a
+// 1 KB head (the two kernels' chunk counters), the request table (uploaded
+// to byte kBatchTableOffset, BatchTableBytes long), then spill slots.
 size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays);
 constexpr size_t kBatchTableOffset = 1024;
+inline size_t BatchTableBytes(uint32_t requests) {
+    return ((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255;
+}
+// BatchRequest.flags bit: the context has no record of this address's level,
+// so the kernels read it from the header.
+constexpr uint32_t kBatchFlagCheckLevel = 8;
 
-// Trace every request of the table (already in device memory): one
-// persistent launch per level, each taking the requests whose header names
-// its level; requests may name different BVHs, levels, queries and outputs.
-// geometry_hint / scene_hint: some request was recorded as a BLAS (or not
-// recorded) / as a scene (or not recorded); they size the launches only.
-void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, bool geometry_hint,
-                bool scene_hint, void* scratch, hipStream_t stream);
+// Trace every request of the table (already in device memory), ordered as
+// recorded geometries (geo_chunks chunks), unrecorded requests (flag
+// kBatchFlagCheckLevel), recorded scenes: one persistent launch per level
+// with requests present; requests may name different BVHs, queries and
+// outputs.
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunks, uint32_t unrecorded_chunks,
+                uint32_t scene_chunks, void* scratch, hipStream_t stream);
 
 // Same traversal with per-ray visit counters written to `stats`
 // (uint32 x4 per ray: internal nodes visited, leaves visited, box tests,

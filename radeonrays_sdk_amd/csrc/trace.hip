@@ -824,44 +824,54 @@ void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, cons
 
 // ---------------------------------------------------------------------------
 // Batched traversal: one persistent launch per level over many requests.
-// Waves take 64-ray chunks from a counter; a chunk belongs to one request
-// (found by binary search over first_chunk), and each lane carries its own
-// request's BVH, query and output, so lanes of a wave may serve different
-// requests.  Both levels' kernels walk the same request table and each
-// takes the requests whose buffer header is its level (the geometry kernel
-// also takes buffers without a valid header, to report them), so the level
-// the host recorded for an address is never trusted for correctness.
+// The host orders the request table as: requests whose address the context
+// recorded as a geometry, requests it has no record of, requests recorded as
+// a scene.  The geometry kernel walks the chunks of the first two groups and
+// the two-level kernel those of the last two; waves take 64-ray chunks from
+// the kernel's counter, a chunk belongs to one request (found by binary
+// search over first_chunk), and each lane carries its own request's BVH,
+// query and output, so lanes of a wave may serve different requests.  An
+// unrecorded request's level is read from its header when a wave takes one
+// of its chunks, and a request of the other level is skipped whole; a
+// recorded request whose header names the other level (the caller put
+// another buffer at that address) is reported as a bad buffer and answered
+// with misses.  No kernel beyond the two traversals and a one-wave init
+// runs per batch: with several streams in flight, a wider helper kernel
+// waits for another stream's persistent launch to leave the CUs.
 
-// Counter words of the two batch kernels in the batch scratch (separate
-// 256-byte lines, inside the table offset).
+// Counter words of the two batch kernels (separate 256-byte lines, inside
+// the table offset).
 constexpr uint32_t kBatchCounterWord = 0, kBatch2LCounterWord = 64;
 
-// Next chunk of a request this kernel takes, as [q_next, q_end) of request
-// q_req; false once every chunk is handed out.  A chunk of a request of the
-// other level moves the counter to the next request's first chunk (one
-// atomicMax for the whole request, instead of one fetch per chunk).  The
-// counter only grows, so the loop ends.  Wave-uniform: every lane calls it.
+// Next chunk of this kernel's range [first, first + count) of the table's
+// chunks, as [q_next, q_end) of request q_req; false once every chunk is
+// handed out.  An unrecorded request (kBatchFlagCheckLevel) of the other
+// level moves the counter past its chunks with one atomicMax.  The counter
+// only grows, so the loop ends.  Wave-uniform: every lane calls it.
 template <bool TWO_LEVEL>
 __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ table, uint32_t requests,
-                                               uint32_t total_chunks, uint32_t* counter, uint32_t lane,
+                                               uint32_t first, uint32_t count, uint32_t* counter, uint32_t lane,
                                                uint32_t& q_req, uint32_t& q_next, uint32_t& q_end) {
     for (;;) {
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(counter, 1u);
         c = __shfl(c, 0, 64);
-        if (c >= total_chunks) return false;
+        if (c >= count) return false;
+        c += first;
         uint32_t lo = 0, hi = requests;  // last request with first_chunk <= c
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;
             if (table[mid].first_chunk <= c) lo = mid;
             else hi = mid;
         }
-        const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(table[lo].accel);
-        const bool scene = LdG(&hdr->magic) == kAccelMagic && LdG(&hdr->kind) == kAccelTlas;
-        if (scene != TWO_LEVEL) {
-            const uint32_t next = lo + 1 < requests ? table[lo + 1].first_chunk : total_chunks;
-            if (lane == 0) atomicMax(counter, next);
-            continue;
+        if (table[lo].flags & kBatchFlagCheckLevel) {
+            const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(table[lo].accel);
+            const bool scene = LdG(&hdr->magic) == kAccelMagic && LdG(&hdr->kind) == kAccelTlas;
+            if (scene != TWO_LEVEL) {
+                const uint32_t next = lo + 1 < requests ? table[lo + 1].first_chunk : first + count;
+                if (lane == 0) atomicMax(counter, min(next, first + count) - first);
+                continue;
+            }
         }
         q_req = lo;
         q_next = (c - table[lo].first_chunk) * kBatchChunk;
@@ -871,13 +881,13 @@ __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ 
 }
 
 __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* __restrict__ table, uint32_t requests,
-                                                           uint32_t total_chunks, uint32_t* __restrict__ scratch,
+                                                           uint32_t first_chunk, uint32_t chunk_count,
+                                                           uint32_t* __restrict__ head, uint32_t* __restrict__ spill,
                                                            uint32_t min_busy, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
-    uint32_t* counter = scratch + kBatchCounterWord;
+    uint32_t* counter = head + kBatchCounterWord;
     const uint32_t lane = __lane_id();
     const uint64_t lt = LaneMaskLt();
-    const size_t spill_base = (kBatchTableOffset + (((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255)) / 4;
 
     // Wave-uniform queue: rays [q_next, q_end) of request q_req.
     uint32_t q_req = 0, q_next = 0, q_end = 0;
@@ -891,12 +901,13 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;  // best_t is t_max until a hit; culling limit WidenLimit(best_t)
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack<> st{s_stack, scratch + spill_base + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
+    LaneStack<> st{s_stack, spill + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
 
     for (;;) {
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
-            if (q_next == q_end) drained = !NextBatchChunk<false>(table, requests, total_chunks, counter, lane, q_req, q_next, q_end);
+            if (q_next == q_end)
+                drained = !NextBatchChunk<false>(table, requests, first_chunk, chunk_count, counter, lane, q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = (uint32_t)__popcll(idle & lt);
             if (rid == kInvalid && rank < take) {
@@ -908,8 +919,8 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
                 const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
                 total = LdG(&hdr->node_count);
                 if (LdG(&hdr->magic) != kAccelMagic || LdG(&hdr->kind) != kAccelBlas) {
-                    // Not a buffer this library built (scenes go to the
-                    // two-level kernel): report it, answer misses.
+                    // Not a geometry (no valid header, or a scene at an
+                    // address recorded as a geometry): report it, answer misses.
                     if (err) StG(err + kErrBadAccel, 1u);
                     total = 0;
                 }
@@ -1014,13 +1025,13 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
 // instance entry and exit, which are rare next to node visits), query and
 // output.
 __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest* __restrict__ table, uint32_t requests,
-                                                             uint32_t total_chunks, uint32_t* __restrict__ scratch,
+                                                             uint32_t first_chunk, uint32_t chunk_count,
+                                                             uint32_t* __restrict__ head, uint32_t* __restrict__ spill,
                                                              uint32_t min_busy, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
     __shared__ float s_world[9][kBlock];  // world ray: origin, direction, reciprocal direction
-    uint32_t* counter = scratch + kBatch2LCounterWord;
+    uint32_t* counter = head + kBatch2LCounterWord;
     const uint32_t lane = __lane_id();
-    const size_t spill_base = (kBatchTableOffset + (((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255)) / 4;
 
     // Wave-uniform queue: rays [q_next, q_end) of request q_req.
     uint32_t q_req = 0, q_next = 0, q_end = 0;
@@ -1037,12 +1048,13 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;  // best_t is t_max until a hit; culling limit WidenLimit(best_t)
     uint32_t best_prim = kInvalid, best_inst = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
-    LaneStack<> st{s_stack, scratch + spill_base + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
+    LaneStack<> st{s_stack, spill + (size_t)blockIdx.x * kBlock * kSpillEntries, err, 0, 0};
 
     for (;;) {
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
-            if (q_next == q_end) drained = !NextBatchChunk<true>(table, requests, total_chunks, counter, lane, q_req, q_next, q_end);
+            if (q_next == q_end)
+                drained = !NextBatchChunk<true>(table, requests, first_chunk, chunk_count, counter, lane, q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = RankIn(idle);
             if (rid == kInvalid && rank < take) {
@@ -1052,6 +1064,12 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
                 lflags = rq.flags;
                 top = reinterpret_cast<const AccelHeader*>(rq.accel);
                 top_total = LdG(&top->node_count);
+                if (LdG(&top->magic) != kAccelMagic || LdG(&top->kind) != kAccelTlas) {
+                    // A geometry (or no valid header) at an address recorded
+                    // as a scene: report it, answer misses.
+                    if (err) StG(err + kErrBadAccel, 1u);
+                    top_total = 0;
+                }
                 const uint32_t prims = LdG(&top->prim_count);
                 top_int = (prims && top_total) ? prims - 1 : 0;
                 const RayIn ray = LoadRay(reinterpret_cast<const float4*>(rq.rays), rid);
@@ -1211,10 +1229,11 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
     }
 }
 
-__global__ void BatchInitKernel(uint32_t* counters) {
+
+__global__ void BatchInitKernel(uint32_t* head) {
     if (threadIdx.x == 0) {
-        counters[kBatchCounterWord] = 0;
-        counters[kBatch2LCounterWord] = 0;
+        head[kBatchCounterWord] = 0;
+        head[kBatch2LCounterWord] = 0;
     }
 }
 
@@ -1417,24 +1436,29 @@ size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays) {
     // fewer than total_rays / kBatchChunk + requests.
     const uint64_t lanes = std::min<uint64_t>((uint64_t)std::max(BatchBlocks(), Batch2LBlocks()) * kBlock,
                                               AlignUp(total_rays + (uint64_t)requests * kBatchChunk, kBlock));
-    return kBatchTableOffset + AlignUp((size_t)requests * sizeof(BatchRequest), 256) +
-           AlignUp((size_t)lanes * kSpillEntries * 4, 256);
+    return kBatchTableOffset + BatchTableBytes(requests) + AlignUp((size_t)lanes * kSpillEntries * 4, 256);
 }
 
-void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t total_chunks, bool geometry_hint,
-                bool scene_hint, void* scratch, hipStream_t s) {
-    if (requests == 0 || total_chunks == 0) return;
-    uint32_t* sc = static_cast<uint32_t*>(scratch);
-    // Both kernels run; the hints (the levels the context recorded for the
-    // requests' addresses) only size their grids: a level with no recorded
-    // request gets one block, which skips the table (or, if a record was
-    // stale, still traces that level's requests).
-    const uint64_t want = ((uint64_t)total_chunks * kBatchChunk + kBlock - 1) / kBlock;
-    const uint32_t b1 = geometry_hint ? (uint32_t)std::min<uint64_t>(BatchBlocks(), want) : 1u;
-    const uint32_t b2 = scene_hint ? (uint32_t)std::min<uint64_t>(Batch2LBlocks(), want) : 1u;
-    BatchInitKernel<<<1, 64, 0, s>>>(sc);
-    TraceBatchKernel<<<b1, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes(), DeviceErrorWords());
-    TraceBatch2LKernel<<<b2, kBlock, 0, s>>>(table, requests, total_chunks, sc, MinBusyLanes(), DeviceErrorWords());
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunks, uint32_t unrecorded_chunks,
+                uint32_t scene_chunks, void* scratch, hipStream_t s) {
+    if (requests == 0 || geo_chunks + unrecorded_chunks + scene_chunks == 0) return;
+    uint8_t* base = static_cast<uint8_t*>(scratch);
+    uint32_t* head = reinterpret_cast<uint32_t*>(base);
+    auto* spill = reinterpret_cast<uint32_t*>(base + kBatchTableOffset + BatchTableBytes(requests));
+    const uint32_t mb = MinBusyLanes();
+    BatchInitKernel<<<1, 64, 0, s>>>(head);
+    // The geometry kernel takes chunks [0, geo + unrecorded), the two-level
+    // kernel [geo, geo + unrecorded + scene); a kernel with no chunk is not
+    // launched.
+    const uint32_t n1 = geo_chunks + unrecorded_chunks, n2 = unrecorded_chunks + scene_chunks;
+    if (n1) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(BatchBlocks(), ((uint64_t)n1 * kBatchChunk + kBlock - 1) / kBlock);
+        TraceBatchKernel<<<b, kBlock, 0, s>>>(table, requests, 0u, n1, head, spill, mb, DeviceErrorWords());
+    }
+    if (n2) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(Batch2LBlocks(), ((uint64_t)n2 * kBatchChunk + kBlock - 1) / kBlock);
+        TraceBatch2LKernel<<<b, kBlock, 0, s>>>(table, requests, geo_chunks, n2, head, spill, mb, DeviceErrorWords());
+    }
 }
 
 }  // namespace rr

@@ -219,16 +219,52 @@ def test_host_requests_need_an_arena(ctx, dev, small_world):
             srv.submit_host([req])
 
 
+def _owned_copy(ctx, accel):
+    """A byte copy of `accel` in memory from rrAllocateDeviceBuffer, written
+    through rrMapDevicePtr / rrUnmapDevicePtr: an address this context
+    recorded no level for.  Returns (handle, accel-like view)."""
+    import ctypes
+    from ctypes import byref, c_void_p
+    from types import SimpleNamespace
+
+    from radeonrays_sdk_amd import rr
+
+    data = accel.buffer.cpu().numpy()
+    h, m = c_void_p(), c_void_p()
+    rr.check(ctx.lib.rrAllocateDeviceBuffer(ctx.ctx, data.nbytes, byref(h)), "rrAllocateDeviceBuffer")
+    rr.check(ctx.lib.rrMapDevicePtr(ctx.ctx, h, byref(m)), "rrMapDevicePtr")
+    ctypes.memmove(m.value, data.ctypes.data, data.nbytes)
+    rr.check(ctx.lib.rrUnmapDevicePtr(ctx.ctx, h, byref(m)), "rrUnmapDevicePtr")
+    view = SimpleNamespace(address=ctx.lib.rrGetHipAddress(h), kind=accel.kind, prim_count=accel.prim_count,
+                           children=accel.children)
+    return h, view
+
+
+def _device_errors(ctx) -> int:
+    from ctypes import byref, c_uint32
+
+    from radeonrays_sdk_amd import rr
+
+    e = c_uint32()
+    rr.check(ctx.lib.rrxGetDeviceErrors(ctx.ctx, byref(e)), "rrxGetDeviceErrors")
+    return e.value
+
+
 def test_scene_requests_share_the_batch_launch(ctx, dev, small_world):
     """Scene (TLAS) requests go through the batched two-level kernel, mixed
     with geometry requests in one table: every request equals its own
     rrCmdIntersect call byte for byte, for both queries and all outputs, on
     a scene of 300 overlapping instances that also holds instances of an
-    empty and of a one-triangle BLAS; batches of scenes only and of
-    geometries only (the other level's kernel then runs one block that skips
-    the table); and buffers copied over addresses the context recorded as
-    the other level (the kernels read the level from each header)."""
+    empty and of a one-triangle BLAS, and in batches of scenes only and of
+    geometries only.  Buffers at addresses the context recorded no level for
+    (copies in rrAllocateDeviceBuffer memory) are sent to the right kernel
+    by their headers; one at an address recorded as the other level is
+    reported (RRX_DEVICE_ERROR_INVALID_BUFFER) and answered with misses, and the
+    rest of its batch is unaffected; releasing an owned buffer forgets the
+    levels recorded in it."""
     import torch
+
+    from radeonrays_sdk_amd import rr
 
     blas, bufs, tlas_small, pools, dpools, kinds = small_world
     rng = scenes.rng_for(40, 500)
@@ -238,21 +274,13 @@ def test_scene_requests_share_the_batch_launch(ctx, dev, small_world):
     one = ctx.build_geometry(to_dev(v1, dev), to_dev(i1.reshape(-1, 3), dev))
     members = [blas[k] for k in rng.integers(0, len(blas), size=300)] + [empty, one] * 10
     big = ctx.build_scene(members, scenes.random_transforms(rng, len(members), trans=0.6))
-    scenes_ = [big, tlas_small]
-    # Copies over recycled addresses: a geometry over a recorded scene, a scene over a recorded geometry.
-    recorded_scene = ctx.build_scene([blas[0]] * 200, scenes.random_transforms(rng, 200, trans=0.5))
-    assert blas[6].buffer.numel() <= recorded_scene.buffer.numel()
-    recorded_scene.buffer[: blas[6].buffer.numel()].copy_(blas[6].buffer)
-    geo_copy = api.Accel(recorded_scene.buffer, "blas", blas[6].prim_count)
-    holder = ctx.empty(tlas_small.buffer.numel() + 4096)
-    vv, ii = scenes.triangle_soup(10, rng)
-    ctx.build_geometry(to_dev(vv, dev), to_dev(ii, dev), result=holder)
-    holder[: tlas_small.buffer.numel()].copy_(tlas_small.buffer)
-    scene_copy = api.Accel(holder, "tlas", tlas_small.prim_count, tlas_small.children)
     torch.cuda.synchronize()
-    copies = {id(geo_copy): blas[6], id(scene_copy): tlas_small}
+    scenes_ = [big, tlas_small]
+    h_geo, geo_copy = _owned_copy(ctx, blas[6])
+    h_scene, scene_copy = _owned_copy(ctx, tlas_small)
+    originals = {id(geo_copy): blas[6], id(scene_copy): tlas_small}
 
-    def batch(targets, seed):
+    def batch(targets, seed, stale=()):
         r = np.random.default_rng(seed)
         reqs = []
         for k, target in enumerate(targets):
@@ -260,28 +288,66 @@ def test_scene_requests_share_the_batch_launch(ctx, dev, small_world):
             off = int(r.integers(0, (1 << 16) - n + 1))
             pool = int(r.integers(0, 4))
             reqs.append((target, dpools[pool][off:off + n], ("closest", "any")[k % 2], ("full", "full_t", "instance")[k % 3]))
+        assert _device_errors(ctx) == 0
         outs = ctx.intersect_batch(reqs)
         torch.cuda.synchronize()
+        assert _device_errors(ctx) == (rr.RRX_DEVICE_ERROR_INVALID_BUFFER if stale else 0)
         for k, (target, rays, query, output) in enumerate(reqs):
             n = int(rays.shape[0])
             if n == 0:
                 continue
-            ref = _alone(ctx, copies.get(id(target), target), rays, query, output)
-            got = outs[k].cpu().numpy()
-            assert np.array_equal(got[: n * HIT_BYTES[output]], ref[: n * HIT_BYTES[output]]), \
-                f"request {k} ({'scene' if target.kind == 'tlas' else 'geometry'}, {query}, {output}, {n} rays)"
-        return reqs
+            got = api.decode_hits(outs[k][: n * HIT_BYTES[output]], output)
+            if k in stale:
+                assert (got["prim_id" if output != "instance" else "id"] == 0xFFFFFFFF).all(), f"request {k}"
+                continue
+            ref = _alone(ctx, originals.get(id(target), target), rays, query, output)
+            assert np.array_equal(outs[k].cpu().numpy()[: n * HIT_BYTES[output]], ref[: n * HIT_BYTES[output]]), \
+                f"request {k} ({target.kind}, {query}, {output}, {n} rays)"
 
-    say("batched scene + geometry requests, scene-only and geometry-only batches, recycled addresses")
-    mixed = [scenes_[k % 2] if k % 3 else blas[k % len(blas)] for k in range(30)]
-    batch(mixed, 1)
+    say("batched scene + geometry requests, scene-only and geometry-only batches, unrecorded and stale addresses")
+    batch([scenes_[k % 2] if k % 3 else blas[k % len(blas)] for k in range(30)], 1)
     batch([scenes_[k % 2] for k in range(14)], 2)
     batch([blas[k % len(blas)] for k in range(14)], 3)
-    batch([geo_copy, scene_copy, blas[1], big, geo_copy, scene_copy], 4)
-    # Every record says geometry, one buffer is a scene (and the reverse):
-    # the other level's kernel runs one block and still answers it.
-    batch([blas[1], blas[4], scene_copy, scene_copy, scene_copy], 5)
-    batch([big, tlas_small, geo_copy, geo_copy, geo_copy], 6)
+    batch([blas[1], big, geo_copy, scene_copy, geo_copy, scene_copy, blas[4]], 4)
+    batch([blas[1], blas[4], scene_copy, scene_copy, scene_copy], 5)  # no recorded scene: the two-level kernel
+    batch([big, tlas_small, geo_copy, geo_copy, geo_copy], 6)         # still runs, for the unrecorded ones
+    # A geometry at an address recorded as a scene: reported, answered with misses.
+    recorded = ctx.build_scene([blas[0]] * 200, scenes.random_transforms(rng, 200, trans=0.5))
+    assert blas[6].buffer.numel() <= recorded.buffer.numel()
+    recorded.buffer[: blas[6].buffer.numel()].copy_(blas[6].buffer)
+    stale_geo = api.Accel(recorded.buffer, "blas", blas[6].prim_count)
+    torch.cuda.synchronize()
+    batch([blas[1], blas[4], stale_geo, stale_geo, big], 7, stale=(2, 3))
+    # Releasing owned memory forgets what was built into it: a geometry built
+    # into rrAllocateDeviceBuffer memory (recorded as a geometry), the memory
+    # released, a scene copied into the next allocation (often at the same
+    # address): traced as the scene.
+    from ctypes import byref, c_void_p
+
+    class _Owned:  # what build_geometry needs of a result buffer
+        def __init__(self, addr, nbytes):
+            self.addr, self.n = addr, nbytes
+        def numel(self):
+            return self.n
+        def element_size(self):
+            return 1
+        def data_ptr(self):
+            return self.addr
+
+    vv, ii = scenes.triangle_soup(10, rng)
+    nbytes = max(int(tlas_small.buffer.numel()), int(ctx.geometry_requirements(10, 30, False).result_buffer_size))
+    h = c_void_p()
+    rr.check(ctx.lib.rrAllocateDeviceBuffer(ctx.ctx, nbytes, byref(h)), "rrAllocateDeviceBuffer")
+    out = ctx.lib.rrGetHipAddress(h)
+    ctx.build_geometry(to_dev(vv, dev), to_dev(ii, dev), result=_Owned(out, nbytes))
+    torch.cuda.synchronize()
+    rr.check(ctx.lib.rrReleaseDevicePtr(ctx.ctx, h), "rrReleaseDevicePtr")
+    h_again, again = _owned_copy(ctx, tlas_small)
+    originals[id(again)] = tlas_small
+    say(f"owned address reused after release: {again.address == out}")
+    batch([blas[1], again, again, again], 8)
+    for hh in (h_geo, h_scene, h_again):
+        rr.check(ctx.lib.rrReleaseDevicePtr(ctx.ctx, hh), "rrReleaseDevicePtr")
     # The big scene's hits against the oracle on a sample.
     rays = pools[0][:3000]
     hits = ctx.intersect_batch([(big, to_dev(rays, dev), "closest", "full")])[0]
