@@ -1393,12 +1393,20 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
         const uint32_t cap = (ray_count + kBlock - 1) / kBlock;
         const uint32_t b1 = kind == kAccelTlas ? 1u : std::min(PersistentBlocks(false), cap);
         const uint32_t b2 = kind == kAccelBlas ? 1u : std::min(PersistentBlocks(true), cap);
-        if (query == kQueryAny) {
-            LaunchPersistent<true, false>(output, b1, s, a, r, ray_count, indirect, hits, sc);
-            LaunchPersistent<true, true>(output, b2, s, a, r, ray_count, indirect, hits, sc);
-        } else {
-            LaunchPersistent<false, false>(output, b1, s, a, r, ray_count, indirect, hits, sc);
-            LaunchPersistent<false, true>(output, b2, s, a, r, ray_count, indirect, hits, sc);
+        // The one-block launch goes first: with other streams' persistent
+        // launches holding the CUs it waits as long as the full launch
+        // behind it would, instead of delaying the query's end.
+        const bool single_first = kind == kAccelTlas;
+        for (int k = 0; k < 2; ++k) {
+            const bool two = (k == 0) != single_first;
+            const uint32_t b = two ? b2 : b1;
+            if (query == kQueryAny) {
+                if (two) LaunchPersistent<true, true>(output, b, s, a, r, ray_count, indirect, hits, sc);
+                else LaunchPersistent<true, false>(output, b, s, a, r, ray_count, indirect, hits, sc);
+            } else {
+                if (two) LaunchPersistent<false, true>(output, b, s, a, r, ray_count, indirect, hits, sc);
+                else LaunchPersistent<false, false>(output, b, s, a, r, ray_count, indirect, hits, sc);
+            }
         }
         return;
     }
