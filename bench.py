@@ -524,6 +524,8 @@ def profile_section(which: str, reps: int = 3) -> None:
       refit        the 1,045,458-triangle heightfield's refit (UPDATE)
       batch-refit  the 8,192-mesh batched refit (rrxCmdBuildGeometryBatch UPDATE)
       trace        the heightfield's closest-hit query of 4,194,304 rays
+      instanced    the instanced scene's closest-hit queries: 8,388,608
+                   secondary, then 4,194,304 primary rays, per launch pair
     Prints one JSON line naming the workload and its algorithmic bytes per
     launch."""
     import torch
@@ -562,6 +564,27 @@ def profile_section(which: str, reps: int = 3) -> None:
             for _ in range(reps):
                 ctx.intersect(accel, rays, "closest", "full", hits=hits, scratch=tscratch)
             out.update(workload=f"heightfield closest hit, {rays.shape[0]} rays, {n} triangles")
+    elif which == "instanced":
+        w = scenes.instanced()
+        blases = [ctx.build_geometry(torch.from_numpy(v).to(dev), torch.from_numpy(i).to(dev), restructure=True)
+                  for v, i in w.meshes]
+        tlas = ctx.build_scene([blases[m] for m in w.mesh_of_instance], w.transforms)
+        primary = scenes.camera_rays(w.width, w.height, w.eye)
+        dprim = torch.from_numpy(primary).to(dev)
+        h = api.decode_hits(ctx.intersect(tlas, dprim, "closest", "full_t"), "full_t")
+        dsec = torch.from_numpy(scenes.secondary_rays(w, primary, h["prim_id"], h["inst_id"], h["t"])).to(dev)
+        hits = ctx.empty(dsec.shape[0] * HIT_BYTES)
+        tscratch = ctx.trace_scratch(dsec.shape[0])
+        for _ in range(reps):
+            ctx.intersect(tlas, dsec, "closest", "full", hits=hits, scratch=tscratch)
+            ctx.intersect(tlas, dprim, "closest", "full", hits=hits, scratch=tscratch)
+        visits = {}
+        for name, r in (("secondary", dsec), ("primary", dprim)):  # every 16th ray, counting kernel
+            _, st = ctx.intersect_with_stats(tlas, r[::16].contiguous(), "closest")
+            s_ = st.cpu().numpy().astype(np.int64)
+            visits[name] = {"inner": round(float(s_[:, 0].mean()), 2), "leaves": round(float(s_[:, 1].mean()), 2)}
+        out.update(workload=f"instanced scene closest hit, {dsec.shape[0]} secondary + {dprim.shape[0]} primary rays",
+                   visits_per_ray=visits)
     elif which == "batch-refit":
         w = scenes.batched()
         V = torch.from_numpy(w.vertices).to(dev)
@@ -601,7 +624,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (profiler byte-counter calibration)")
-    ap.add_argument("--profile", choices=["sort", "refit", "batch-refit", "trace"], default=None,
+    ap.add_argument("--profile", choices=["sort", "refit", "batch-refit", "trace", "instanced"], default=None,
                     help="run only this kernel's workload a few times (for a rocprofv3 counter pass) and exit")
     args = ap.parse_args()
     if args.profile:
