@@ -235,10 +235,12 @@ def serving_bench(ctx, dev, n_requests, log):
 
     log("serving: generating 32 scenes")
     sc = scenes.serving_scenes()
+    arena = ctx.result_arena([ctx.geometry_requirements(s.indices.shape[0], s.vertices.shape[0], True).result_buffer_size
+                              for s in sc])  # the resident scenes in one allocation
     accels = []
-    for s in sc:
+    for s, r in zip(sc, arena):
         accels.append(ctx.build_geometry(torch.from_numpy(s.vertices).to(dev), torch.from_numpy(s.indices).to(dev),
-                                         restructure=True))
+                                         restructure=True, result=r))
     pool_rays = 1 << 20
     pools = [torch.from_numpy(p).to(dev) for p in scenes.serving_ray_pools(pool_rays)]
     reqs = scenes.serving_requests(n_requests, sc, pool_rays)
@@ -349,8 +351,11 @@ def instanced_bench(ctx, dev, log, reps=5):
 
     log("instanced: generating 64 meshes x 4096 instances")
     w = scenes.instanced()
-    blases = [ctx.build_geometry(torch.from_numpy(v).to(dev), torch.from_numpy(i).to(dev), restructure=True)
-              for v, i in w.meshes]
+    # The BLASes in one allocation (api.Context.result_arena).
+    arena = ctx.result_arena([ctx.geometry_requirements(i.shape[0], v.shape[0], True).result_buffer_size
+                              for v, i in w.meshes])
+    blases = [ctx.build_geometry(torch.from_numpy(v).to(dev), torch.from_numpy(i).to(dev), restructure=True,
+                                 result=r) for (v, i), r in zip(w.meshes, arena)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tlas = ctx.build_scene([blases[m] for m in w.mesh_of_instance], w.transforms)
@@ -431,7 +436,7 @@ def batched_bench(ctx, dev, log, frames=16):
     meshes = [(V[o:o + 3 * n], I[:n]) for o, n in spans]
     rays = torch.from_numpy(w.rays).to(dev)
     R = rays.shape[0]
-    results = [ctx.empty(ctx.geometry_requirements(n, 3 * n).result_buffer_size) for _, n in spans]
+    results = ctx.result_arena([ctx.geometry_requirements(n, 3 * n).result_buffer_size for _, n in spans])
     scratch = ctx.empty(ctx.batch_build_requirements(meshes))
     torch.cuda.synchronize()
     log("batched: build")
@@ -484,7 +489,7 @@ def batched_bench(ctx, dev, log, frames=16):
     # GPU times from graph replays (the eager calls' host-side setup of
     # 8192-entry tables is not GPU work); the graphs replay the same calls.
     build_gpu_ms = timed(graph_of(build).replay, 3, warm=1)
-    rs_results = [ctx.empty(ctx.geometry_requirements(n, 3 * n, True).result_buffer_size) for _, n in spans]
+    rs_results = ctx.result_arena([ctx.geometry_requirements(n, 3 * n, True).result_buffer_size for _, n in spans])
     rs_scratch = ctx.empty(ctx.batch_build_requirements(meshes, True))
 
     def build_restructured():
@@ -566,8 +571,10 @@ def profile_section(which: str, reps: int = 3) -> None:
             out.update(workload=f"heightfield closest hit, {rays.shape[0]} rays, {n} triangles")
     elif which == "instanced":
         w = scenes.instanced()
-        blases = [ctx.build_geometry(torch.from_numpy(v).to(dev), torch.from_numpy(i).to(dev), restructure=True)
-                  for v, i in w.meshes]
+        arena = ctx.result_arena([ctx.geometry_requirements(i.shape[0], v.shape[0], True).result_buffer_size
+                                  for v, i in w.meshes])
+        blases = [ctx.build_geometry(torch.from_numpy(v).to(dev), torch.from_numpy(i).to(dev), restructure=True,
+                                     result=r) for (v, i), r in zip(w.meshes, arena)]
         tlas = ctx.build_scene([blases[m] for m in w.mesh_of_instance], w.transforms)
         primary = scenes.camera_rays(w.width, w.height, w.eye)
         dprim = torch.from_numpy(primary).to(dev)
@@ -592,7 +599,8 @@ def profile_section(which: str, reps: int = 3) -> None:
         spans = [(int(o), int(n)) for o, n in zip(w.vertex_offsets, w.tri_counts)]
         meshes = [(V[o:o + 3 * n], I[:n]) for o, n in spans]
         scratch = ctx.empty(ctx.batch_build_requirements(meshes))
-        accels = ctx.build_geometry_batch(meshes, scratch=scratch)
+        results = ctx.result_arena([ctx.geometry_requirements(n, 3 * n).result_buffer_size for _, n in spans])
+        accels = ctx.build_geometry_batch(meshes, scratch=scratch, results=results)
         for _ in range(reps):
             ctx.update_geometry_batch(accels, meshes, scratch=scratch)
         tris = int(w.tri_counts.sum())
