@@ -253,3 +253,29 @@ def test_visit_counters_match_oracle_stats(ctx, dev, query):
     st = stats.cpu().numpy().view(np.uint32)
     for k, f in enumerate(("internal_nodes", "leaves", "box_tests", "tri_tests")):
         assert np.array_equal(st[:, k], rst[f]), f
+
+
+def test_result_arena_builds_equal_separate_builds(ctx, dev):
+    """Result buffers carved from one allocation (api.Context.result_arena):
+    256-byte aligned, disjoint, and each build in them equals the same build
+    into its own allocation node for node, with the same hits."""
+    import torch
+
+    meshes = [scenes.triangle_soup(n, scenes.rng_for(60, k)) for k, n in enumerate((1, 100, 5000, 777, 2))]
+    sizes = [ctx.geometry_requirements(i.shape[0], v.shape[0], True).result_buffer_size for v, i in meshes]
+    arena = ctx.result_arena(sizes)
+    spans = sorted((int(a.data_ptr()), int(a.data_ptr()) + int(a.numel())) for a in arena)
+    assert all(lo % 256 == 0 for lo, _ in spans)
+    assert all(spans[k][1] <= spans[k + 1][0] for k in range(len(spans) - 1))
+    assert [int(a.numel()) for a in arena] == [int(n) for n in sizes]
+    rays = to_dev(scenes.make_rays(scenes.rng_for(60, 99).uniform(-0.5, 1.5, (4000, 3)).astype(np.float32),
+                                   scenes.unit_sphere(4000, scenes.rng_for(60, 98))), dev)
+    for (v, i), r in zip(meshes, arena):
+        a = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True, result=r)
+        b = ctx.build_geometry(to_dev(v, dev), to_dev(i, dev), restructure=True)
+        torch.cuda.synchronize()
+        ok, msg = nodes_equal(a.buffer.cpu().numpy(), b.buffer.cpu().numpy())
+        assert ok, msg
+        ha, hb = ctx.intersect(a, rays, "closest", "full_t"), ctx.intersect(b, rays, "closest", "full_t")
+        torch.cuda.synchronize()
+        assert torch.equal(ha, hb)
