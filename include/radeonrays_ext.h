@@ -112,8 +112,11 @@ RR_API RRError rrxGetTuning(RRxTuning key, uint32_t* value);
  * ends, vlk/device.cpp:249-262). */
 #define RRX_DEVICE_ERROR_STACK_OVERFLOW   (1u << 0) /* a ray needed more than 80 stack entries; it ended early */
 #define RRX_DEVICE_ERROR_SORT_LOOKBACK    (1u << 1) /* a sort tile gave up waiting; that sort's order is wrong */
-#define RRX_DEVICE_ERROR_INVALID_BUFFER   (1u << 2) /* a query named a buffer without a valid header; its hits
-                                                       were not written */
+#define RRX_DEVICE_ERROR_INVALID_BUFFER   (1u << 2) /* a query named a buffer without a valid header
+                                                       (rrCmdIntersect then writes no hits), or a batched
+                                                       request named one whose header disagrees with the
+                                                       level the context recorded for that address
+                                                       (rrxCmdIntersectBatch answers it with misses) */
 RR_API RRError rrxGetDeviceErrors(RRContext context, uint32_t* errors);
 
 /* Scratch bytes for rrxCmdBuildGeometryBatch over `count` build inputs
