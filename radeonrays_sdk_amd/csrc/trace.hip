@@ -324,15 +324,27 @@ __global__ void QueueInitKernel(uint32_t* counters, uint32_t ray_count, const ui
     if (k < kQueues) counters[k * kQueueStride] = min(k * part, n);
 }
 
+// The same set-up done by a one-block persistent launch that runs before
+// the full one (instead of a QueueInitKernel launch): the fence and barrier
+// make it visible to the block's own atomics, stream order to the next launch.
+__device__ __forceinline__ void InitQueues(uint32_t* counters, uint32_t n) {
+    const uint32_t part = (n + kQueues - 1) / kQueues;
+    if (threadIdx.x < kQueues) counters[threadIdx.x * kQueueStride] = min(threadIdx.x * part, n);
+    __threadfence();
+    __syncthreads();
+}
+
 template <bool ANY, int OUT, int DEPTH>
 __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* __restrict__ accel,
                                                                 const float4* __restrict__ rays, uint32_t ray_count,
                                                                 const uint32_t* __restrict__ indirect,
                                                                 void* __restrict__ hits, uint32_t* __restrict__ scratch,
-                                                                uint32_t min_busy, uint32_t fetch, uint32_t* err) {
+                                                                uint32_t min_busy, uint32_t fetch, uint32_t init_queues,
+                                                                uint32_t* err) {
     __shared__ uint32_t s_stack[DEPTH][kBlock];
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
+    if (init_queues) InitQueues(scratch, n);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
     if (!HeaderIs(hdr, kAccelBlas, err)) return;
     const uint32_t total = hdr->node_count;
@@ -585,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                                                                   const uint32_t* __restrict__ indirect,
                                                                   void* __restrict__ hits,
                                                                   uint32_t* __restrict__ scratch, uint32_t min_busy,
-                                                                  uint32_t fetch, uint32_t* err) {
+                                                                  uint32_t fetch, uint32_t init_queues, uint32_t* err) {
     __shared__ uint32_t s_stack[DEPTH][kBlock];
     // Each lane's world ray: origin xyz, direction xyz and reciprocal
     // direction, restored without divisions when the lane leaves an
@@ -595,6 +607,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
     __shared__ float s_world[9][kBlock];
     uint32_t n = ray_count;
     if (indirect) n = min(n, *indirect);
+    if (init_queues) InitQueues(scratch, n);
     const AccelHeader* hdr = reinterpret_cast<const AccelHeader*>(accel);
     if (!HeaderIs(hdr, kAccelTlas, err)) return;
     const uint32_t top_total = hdr->node_count;
@@ -1342,30 +1355,32 @@ uint32_t FetchFor(uint32_t n, uint32_t blocks) {
 
 template <bool ANY, bool TWO, int OUT, int DEPTH>
 void LaunchPersistentKernel(uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
-                            const uint32_t* ind, void* hits, uint32_t* sc, uint32_t mb, uint32_t f) {
+                            const uint32_t* ind, void* hits, uint32_t* sc, uint32_t mb, uint32_t f, uint32_t init) {
     uint32_t* err = DeviceErrorWords();
-    if (TWO) TracePersistent2LKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f, err);
-    else TracePersistentKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f, err);
+    if (TWO) TracePersistent2LKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f, init, err);
+    else TracePersistentKernel<ANY, OUT, DEPTH><<<blocks, kBlock, 0, s>>>(a, r, n, ind, hits, sc, mb, f, init, err);
 }
 
 template <bool ANY, bool TWO, int DEPTH>
 void LaunchPersistentDepth(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r,
-                           uint32_t n, const uint32_t* ind, void* hits, uint32_t* sc) {
+                           uint32_t n, const uint32_t* ind, void* hits, uint32_t* sc, uint32_t init) {
     const uint32_t mb = MinBusyLanes(), f = FetchFor(n, blocks);
     switch (out) {
-        case kOutFullHit: LaunchPersistentKernel<ANY, TWO, kOutFullHit, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f); break;
-        case kOutInstanceId: LaunchPersistentKernel<ANY, TWO, kOutInstanceId, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f); break;
-        default: LaunchPersistentKernel<ANY, TWO, kOutFullHitT, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f); break;
+        case kOutFullHit: LaunchPersistentKernel<ANY, TWO, kOutFullHit, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f, init); break;
+        case kOutInstanceId: LaunchPersistentKernel<ANY, TWO, kOutInstanceId, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f, init); break;
+        default: LaunchPersistentKernel<ANY, TWO, kOutFullHitT, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f, init); break;
     }
 }
 
+// init: this launch (one block) sets the queues up first (see InitQueues).
 template <bool ANY, bool TWO>
 void LaunchPersistent(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r, uint32_t n,
-                      const uint32_t* ind, void* hits, uint32_t* sc) {
+                      const uint32_t* ind, void* hits, uint32_t* sc, bool init) {
+    const uint32_t i = init ? 1u : 0u;
     switch (LdsStackDepth()) {
-        case 8: LaunchPersistentDepth<ANY, TWO, 8>(out, blocks, s, a, r, n, ind, hits, sc); break;
-        case 32: LaunchPersistentDepth<ANY, TWO, 32>(out, blocks, s, a, r, n, ind, hits, sc); break;
-        default: LaunchPersistentDepth<ANY, TWO, 16>(out, blocks, s, a, r, n, ind, hits, sc); break;
+        case 8: LaunchPersistentDepth<ANY, TWO, 8>(out, blocks, s, a, r, n, ind, hits, sc, i); break;
+        case 32: LaunchPersistentDepth<ANY, TWO, 32>(out, blocks, s, a, r, n, ind, hits, sc, i); break;
+        default: LaunchPersistentDepth<ANY, TWO, 16>(out, blocks, s, a, r, n, ind, hits, sc, i); break;
     }
 }
 
@@ -1389,7 +1404,10 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
         // At most ceil(ray_count / kBlock) blocks: TraceScratchBytes holds
         // spill slots for that many lanes.  The queues are set up once: a
         // kernel of the wrong level returns before touching them.
-        QueueInitKernel<<<1, 64, 0, s>>>(sc, ray_count, indirect);
+        // With a recorded level the one-block launch (first) sets the queues
+        // up; with none, both grids are full and a QueueInitKernel does.
+        const bool recorded = kind != kAccelUnknown;
+        if (!recorded) QueueInitKernel<<<1, 64, 0, s>>>(sc, ray_count, indirect);
         const uint32_t cap = (ray_count + kBlock - 1) / kBlock;
         const uint32_t b1 = kind == kAccelTlas ? 1u : std::min(PersistentBlocks(false), cap);
         const uint32_t b2 = kind == kAccelBlas ? 1u : std::min(PersistentBlocks(true), cap);
@@ -1400,12 +1418,13 @@ void Trace(const void* accel, AccelKind kind, QueryKind query, OutputKind output
         for (int k = 0; k < 2; ++k) {
             const bool two = (k == 0) != single_first;
             const uint32_t b = two ? b2 : b1;
+            const bool init = recorded && k == 0;
             if (query == kQueryAny) {
-                if (two) LaunchPersistent<true, true>(output, b, s, a, r, ray_count, indirect, hits, sc);
-                else LaunchPersistent<true, false>(output, b, s, a, r, ray_count, indirect, hits, sc);
+                if (two) LaunchPersistent<true, true>(output, b, s, a, r, ray_count, indirect, hits, sc, init);
+                else LaunchPersistent<true, false>(output, b, s, a, r, ray_count, indirect, hits, sc, init);
             } else {
-                if (two) LaunchPersistent<false, true>(output, b, s, a, r, ray_count, indirect, hits, sc);
-                else LaunchPersistent<false, false>(output, b, s, a, r, ray_count, indirect, hits, sc);
+                if (two) LaunchPersistent<false, true>(output, b, s, a, r, ray_count, indirect, hits, sc, init);
+                else LaunchPersistent<false, false>(output, b, s, a, r, ray_count, indirect, hits, sc, init);
             }
         }
         return;
