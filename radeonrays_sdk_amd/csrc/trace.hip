@@ -366,6 +366,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
     // a miss it still holds tmax.  bU, bV: barycentrics of the best hit so
     // far (divided by the determinant when the hit is taken).
     float best_t = 0.0f, bU = 0.0f, bV = 0.0f;
+    float any_limit = 0.0f;  // any hit: the culling limit, fixed per ray (the first hit ends the ray)
     uint32_t best_prim = kInvalid;
     uint32_t node = kInvalid, leaf = kInvalid;
     LaneStack<DEPTH> st{s_stack, scratch + kQueueHeadWords + (size_t)blockIdx.x * kBlock * kSpillEntries,
@@ -408,6 +409,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 const RayIn ray = LoadRay(rays, rid);
                 SetupRay(r, ray.a.x, ray.a.y, ray.a.z, ray.a.w, ray.b.x, ray.b.y, ray.b.z);
                 best_t = ray.b.w;
+                if (ANY) any_limit = WidenLimit(best_t);
                 best_prim = kInvalid;
                 bU = 0.0f; bV = 0.0f;
                 st.clear();
@@ -505,7 +507,8 @@ __global__ __launch_bounds__(kBlock) void TracePersistentKernel(const uint8_t* _
                 const uint32_t c0 = __float_as_uint(q0.w);
                 const uint32_t c1 = __float_as_uint(q1.w);
                 float ta, tb;
-                const float limit = WidenLimit(best_t);  // recomputed: one register fewer across the loop
+                // Closest hit: recomputed (one register fewer across the loop).
+                const float limit = ANY ? any_limit : WidenLimit(best_t);
                 const bool h0 = SlabTest(r, limit, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z, ta);
                 const bool h1 = SlabTest(r, limit, q2.x, q2.y, q2.z, q3.x, q3.y, q3.z, tb);
                 if (h0 || h1) {
