@@ -621,7 +621,6 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
     const uint64_t* blas_table = reinterpret_cast<const uint64_t*>(accel + hdr->blas_table_offset);
     uint32_t* counters = scratch;
     const uint32_t lane = __lane_id();
-    const uint64_t lt = LaneMaskLt();
     const uint32_t part = (n + kQueues - 1) / kQueues;
 
     // Wave-uniform queue state.
@@ -666,7 +665,7 @@ __global__ __launch_bounds__(kBlock) void TracePersistent2LKernel(const uint8_t*
                 }
             }
             const uint32_t take = min((uint32_t)__popcll(idle), q_end - q_next);
-            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            const uint32_t rank = RankIn(idle);
             if (rid == kInvalid && rank < take) {
                 rid = q_next + rank;
                 const RayIn ray = LoadRay(rays, rid);
