@@ -701,6 +701,9 @@ def main():
     alg_bytes = n_rays * (RAY_BYTES + HIT_BYTES) + NODE_BYTES * visits
     achieved = alg_bytes / (closest_ms * 1e-3) / 1e9
 
+    from radeonrays_sdk_amd import autotune as _tuning
+
+    depth = _tuning.get(ctx.lib, _tuning.rr.RRX_TUNING_TRACE_LDS_STACK)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -721,7 +724,7 @@ def main():
         "any_mrays_s": round(n_rays / any_ms / 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(),
-                     "kernel": "TracePersistentKernel<closest, FULL_HIT, LDS stack 8>", "kernel_ms": round(closest_ms, 4),
+                     "kernel": f"TracePersistentKernel<closest, FULL_HIT, LDS stack {depth}>", "kernel_ms": round(closest_ms, 4),
                      "algorithmic_bytes": int(alg_bytes),
                      "visits_per_ray": round(visits / n_rays, 3)},
     }

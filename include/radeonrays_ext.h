@@ -75,7 +75,7 @@ typedef enum RRxTuning
 {
     RRX_TUNING_TRACE_MIN_BUSY = 0,        /* refill a wave once fewer lanes traverse (1..64; default 24) */
     RRX_TUNING_TRACE_RAYS_PER_FETCH = 1,  /* ray indices per queue atomic (32, 64, 128; default 128) */
-    RRX_TUNING_TRACE_LDS_STACK = 2,       /* LDS stack entries per lane (8, 16, 32; default 8) */
+    RRX_TUNING_TRACE_LDS_STACK = 2,       /* LDS stack entries per lane (8, 16, 32; default 16) */
     RRX_TUNING_TRACE_PLAIN = 3,           /* 1: one lane per ray for its whole traversal, no
                                              persistent kernels (default 0, or 1 when the
                                              environment sets RR_TRACE_MODE=plain) */
@@ -92,13 +92,22 @@ typedef enum RRxTuning
                                              one unpublished (default 2^22; 0 gives up at the
                                              first).  A tile that gives up sorts wrongly and the
                                              failure is reported (rrxGetDeviceErrors) */
-    RRX_TUNING_SORT_HYBRID = 7            /* 1: rrxCmdSortPairs on 2^22..2^24 keys takes one
+    RRX_TUNING_SORT_HYBRID = 7,           /* 1: rrxCmdSortPairs on 2^22..2^24 keys takes one
                                              10-bit most-significant-digit pass and then sorts each
                                              bucket in one workgroup's LDS (two passes over the data
                                              instead of four); a bucket of more than 20,480 keys is
                                              sorted by its workgroup through memory, so keys
                                              concentrated in a few buckets sort slower.  0 (default):
                                              the LSD passes.  Same order either way */
+    RRX_TUNING_TRACE_RAYS_PER_LANE = 8    /* k sizing a single query's persistent grid (0..64;
+                                             default 2): at most four 256-lane workgroups per CU
+                                             unless 512 x k rays per workgroup need more; below
+                                             that a workgroup per 256 x k rays, at least one per CU
+                                             (single level) or one ray per lane (two level); never
+                                             more than one ray per lane or the resident grid.
+                                             Queries that do not fill the resident grid finish
+                                             sooner on fewer waves per SIMD.  0: the resident grid
+                                             whenever the rays fill it */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

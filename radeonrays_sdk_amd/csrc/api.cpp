@@ -1400,6 +1400,8 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             RR_REQUIRE(value <= 1, "rrxSetTuning");
             rr::SetSortHybrid(value != 0);
             return RR_SUCCESS;
+        case RRX_TUNING_TRACE_RAYS_PER_LANE:
+            return rr::SetRaysPerLane(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1415,6 +1417,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_TREELET_LEAVES: *value = (uint32_t)rr::TreeletLeaves(); return RR_SUCCESS;
         case RRX_TUNING_SORT_SPIN_LIMIT: *value = rr::SortSpinLimit(); return RR_SUCCESS;
         case RRX_TUNING_SORT_HYBRID: *value = rr::SortHybrid() ? 1u : 0u; return RR_SUCCESS;
+        case RRX_TUNING_TRACE_RAYS_PER_LANE: *value = rr::RaysPerLane(); return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

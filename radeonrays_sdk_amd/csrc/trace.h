@@ -40,11 +40,21 @@ void SetMinBusyLanes(uint32_t lanes);
 // 32).  Setters reject other values.  Defaults: the sweep's best on the
 // 1M-triangle heightfield with 4M incoherent rays.
 constexpr uint32_t kDefaultFetch = 128;
-constexpr uint32_t kDefaultLdsStack = 8;
+constexpr uint32_t kDefaultLdsStack = 16;
 uint32_t RaysPerFetch();
 bool SetRaysPerFetch(uint32_t rays);
 uint32_t LdsStackDepth();
 bool SetLdsStackDepth(uint32_t entries);
+
+// Rays per lane k that size a single query's persistent grid (0..64; setter
+// rejects others): at most four workgroups per CU unless 2 x kBlock x k rays
+// per workgroup need more; below that a workgroup per kBlock x k rays, at
+// least one per CU (single level) or one ray per lane (two level); never
+// more than one ray per lane or the resident grid.  0: the resident grid
+// whenever the rays fill it.
+constexpr uint32_t kDefaultRaysPerLane = 2;
+uint32_t RaysPerLane();
+bool SetRaysPerLane(uint32_t rays);
 
 enum QueryKind : int { kQueryClosest = 0, kQueryAny = 1 };
 enum OutputKind : int { kOutFullHit = 0, kOutInstanceId = 1, kOutFullHitT = 2 };

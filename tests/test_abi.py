@@ -121,15 +121,15 @@ def test_tuning_validation():
     from ctypes import c_uint32
     v = c_uint32()
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, byref(v)) == rr.RR_SUCCESS and v.value == 128
-    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, byref(v)) == rr.RR_SUCCESS and v.value == 8
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, byref(v)) == rr.RR_SUCCESS and v.value == 16
     for key, bad in ((rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, 48), (rr.RRX_TUNING_TRACE_LDS_STACK, 12),
                      (rr.RRX_TUNING_TRACE_MIN_BUSY, 0), (rr.RRX_TUNING_TRACE_MIN_BUSY, 65),
-                     (rr.RRX_TUNING_SORT_HYBRID, 2), (8, 1)):
+                     (rr.RRX_TUNING_SORT_HYBRID, 2), (rr.RRX_TUNING_TRACE_RAYS_PER_LANE, 65), (9, 1)):
         assert lib.rrxSetTuning(key, bad) == rr.RR_ERROR_INVALID_PARAMETER
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, None) == rr.RR_ERROR_INVALID_PARAMETER
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 32) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, byref(v)) == rr.RR_SUCCESS and v.value == 32
-    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 8) == rr.RR_SUCCESS
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 16) == rr.RR_SUCCESS
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_PLAIN, 2) == rr.RR_ERROR_INVALID_PARAMETER
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_PLAIN, byref(v)) == rr.RR_SUCCESS
     before = v.value
@@ -140,6 +140,10 @@ def test_tuning_validation():
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_HYBRID, 1) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_HYBRID, byref(v)) == rr.RR_SUCCESS and v.value == 1
     assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_HYBRID, 0) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_LANE, byref(v)) == rr.RR_SUCCESS and v.value == 2
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_LANE, 0) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_LANE, byref(v)) == rr.RR_SUCCESS and v.value == 0
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_LANE, 2) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 8
     for bad in (0, 6, 9, 16):
         assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, bad) == rr.RR_ERROR_INVALID_PARAMETER

@@ -3,7 +3,7 @@ run-time knobs must return the same hits (the tuner raises otherwise), and
 the fastest combination is left set."""
 import pytest
 
-from radeonrays_sdk_amd import autotune, scenes
+from radeonrays_sdk_amd import api, autotune, scenes
 from tests.gpu_util import say, to_dev
 
 pytestmark = pytest.mark.gpu
@@ -92,3 +92,44 @@ def test_treelet_sweep_keeps_closest_hits(ctx, dev):
         assert autotune.get(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES) == out["best"]["treelet_leaves"]
     finally:
         autotune.set_(lib, autotune.rr.RRX_TUNING_TREELET_LEAVES, start)
+
+
+def test_query_grid_size_keeps_results(ctx, dev):
+    """RRX_TUNING_TRACE_RAYS_PER_LANE changes only how many workgroups a
+    query's persistent launch gets (from one per CU up to the resident
+    grid): closest and any hits must equal the plain one-lane-per-ray
+    kernel's at every setting, for both levels and for ray counts below,
+    inside and above the range the setting shapes."""
+    import numpy as np
+    import torch
+
+    say("query grid sizes: 256x256 heightfield and a 12-instance scene, rays per lane 0/1/2/3/64")
+    lib = ctx.lib
+    w = scenes.heightfield(res=256, n_rays=600000)
+    blas = ctx.build_geometry(to_dev(w.vertices, dev), to_dev(w.indices, dev), restructure=True)
+    soups = [scenes.triangle_soup(n, scenes.rng_for(11, n), sigma=0.2) for n in (3000, 20000, 50000)]
+    blases = [ctx.build_geometry(to_dev(v, dev), to_dev(i.reshape(-1, 3), dev), restructure=True) for v, i in soups]
+    xf = scenes.random_transforms(scenes.rng_for(11, 99), 12, trans=1.0)
+    tlas = ctx.build_scene([blases[k % 3] for k in range(12)], xf)
+    rng = scenes.rng_for(11, 7)
+    o = rng.uniform(-3, 3, size=(600000, 3)).astype(np.float32)
+    scene_rays = to_dev(scenes.make_rays(o, scenes.unit_sphere(600000, rng)), dev)
+    rays = to_dev(w.rays, dev)
+    torch.cuda.synchronize()
+    start = autotune.get(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE)
+    try:
+        for accel, r, what in ((blas, rays, "single level"), (tlas, scene_rays, "two level")):
+            for n in (1000, 70000, 300000, 600000):
+                sub = r[:n].contiguous()
+                for query in ("closest", "any"):
+                    with autotune.plain_traversal(lib):
+                        ref = ctx.intersect(accel, sub, query, "full_t")
+                    for k in (0, 1, 2, 3, 64):
+                        autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE, k)
+                        got = ctx.intersect(accel, sub, query, "full_t")
+                        torch.cuda.synchronize()
+                        assert torch.equal(got, ref), f"{what}, {n} rays, {query}, rays per lane {k}"
+                    hit = api.decode_hits(ref, "full_t")["prim_id"] != 0xFFFFFFFF
+                    assert int(hit.sum()) > n // 100, f"{what}, {n} rays, {query}: almost no hits"
+    finally:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE, start)
