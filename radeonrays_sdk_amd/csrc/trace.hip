@@ -1520,14 +1520,14 @@ void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunk
     BatchInitKernel<<<1, 64, 0, s>>>(head);
     // The geometry kernel takes chunks [0, geo + unrecorded), the two-level
     // kernel [geo, geo + unrecorded + scene); a kernel with no chunk is not
-    // launched.
+    // launched.  Grids as a single query's of the chunks' lanes.
     const uint32_t n1 = geo_chunks + unrecorded_chunks, n2 = unrecorded_chunks + scene_chunks;
     if (n1) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(BatchBlocks(), ((uint64_t)n1 * kBatchChunk + kBlock - 1) / kBlock);
+        const uint32_t b = std::min(BatchBlocks(), QueryBlocks((uint64_t)n1 * kBatchChunk, false));
         TraceBatchKernel<<<b, kBlock, 0, s>>>(table, requests, 0u, n1, head, spill, mb, DeviceErrorWords());
     }
     if (n2) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(Batch2LBlocks(), ((uint64_t)n2 * kBatchChunk + kBlock - 1) / kBlock);
+        const uint32_t b = std::min(Batch2LBlocks(), QueryBlocks((uint64_t)n2 * kBatchChunk, true));
         TraceBatch2LKernel<<<b, kBlock, 0, s>>>(table, requests, geo_chunks, n2, head, spill, mb, DeviceErrorWords());
     }
 }

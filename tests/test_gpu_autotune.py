@@ -117,6 +117,7 @@ def test_query_grid_size_keeps_results(ctx, dev):
     rays = to_dev(w.rays, dev)
     torch.cuda.synchronize()
     start = autotune.get(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE)
+    refs = {}
     try:
         for accel, r, what in ((blas, rays, "single level"), (tlas, scene_rays, "two level")):
             for n in (1000, 70000, 300000, 600000):
@@ -124,6 +125,7 @@ def test_query_grid_size_keeps_results(ctx, dev):
                 for query in ("closest", "any"):
                     with autotune.plain_traversal(lib):
                         ref = ctx.intersect(accel, sub, query, "full_t")
+                    refs[(what, n, query)] = ref
                     for k in (0, 1, 2, 3, 64):
                         autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE, k)
                         got = ctx.intersect(accel, sub, query, "full_t")
@@ -131,5 +133,14 @@ def test_query_grid_size_keeps_results(ctx, dev):
                         assert torch.equal(got, ref), f"{what}, {n} rays, {query}, rays per lane {k}"
                     hit = api.decode_hits(ref, "full_t")["prim_id"] != 0xFFFFFFFF
                     assert int(hit.sum()) > n // 100, f"{what}, {n} rays, {query}: almost no hits"
+        # The batch kernels size their grids by the same rule.
+        mix = [(blas, rays, "single level", 70000, "closest"), (tlas, scene_rays, "two level", 300000, "any"),
+               (blas, rays, "single level", 600000, "any"), (tlas, scene_rays, "two level", 1000, "closest")]
+        for k in (0, 1, 2, 64):
+            autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE, k)
+            outs = ctx.intersect_batch([(a, r[:n].contiguous(), q, "full_t") for a, r, _, n, q in mix])
+            torch.cuda.synchronize()
+            for (_, _, what, n, q), got in zip(mix, outs):
+                assert torch.equal(got, refs[(what, n, q)]), f"batch: {what}, {n} rays, {q}, rays per lane {k}"
     finally:
         autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE, start)
