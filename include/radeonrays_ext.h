@@ -99,7 +99,7 @@ typedef enum RRxTuning
                                              sorted by its workgroup through memory, so keys
                                              concentrated in a few buckets sort slower.  0 (default):
                                              the LSD passes.  Same order either way */
-    RRX_TUNING_TRACE_RAYS_PER_LANE = 8    /* k sizing a single query's persistent grid (0..64;
+    RRX_TUNING_TRACE_RAYS_PER_LANE = 8,   /* k sizing a single query's persistent grid (0..64;
                                              default 2): at most four 256-lane workgroups per CU
                                              unless 512 x k rays per workgroup need more; below
                                              that a workgroup per 256 x k rays, at least one per CU
@@ -108,6 +108,12 @@ typedef enum RRxTuning
                                              Queries that do not fill the resident grid finish
                                              sooner on fewer waves per SIMD.  0: the resident grid
                                              whenever the rays fill it */
+    RRX_TUNING_TRACE_2L_MIN_BUSY = 9,     /* RRX_TUNING_TRACE_MIN_BUSY for the persistent two-level
+                                             kernel of rrCmdIntersect on a scene (1..64; default 32);
+                                             RRX_TUNING_TRACE_MIN_BUSY sets the single-level and
+                                             the batch kernels */
+    RRX_TUNING_TRACE_2L_RAYS_PER_FETCH = 10 /* RRX_TUNING_TRACE_RAYS_PER_FETCH for the same kernel
+                                             (32, 64, 128; default 64) */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

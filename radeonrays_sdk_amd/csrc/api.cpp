@@ -1402,6 +1402,10 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             return RR_SUCCESS;
         case RRX_TUNING_TRACE_RAYS_PER_LANE:
             return rr::SetRaysPerLane(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_TRACE_2L_MIN_BUSY:
+            return rr::SetMinBusyLanes2L(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_TRACE_2L_RAYS_PER_FETCH:
+            return rr::SetRaysPerFetch2L(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1418,6 +1422,8 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_SORT_SPIN_LIMIT: *value = rr::SortSpinLimit(); return RR_SUCCESS;
         case RRX_TUNING_SORT_HYBRID: *value = rr::SortHybrid() ? 1u : 0u; return RR_SUCCESS;
         case RRX_TUNING_TRACE_RAYS_PER_LANE: *value = rr::RaysPerLane(); return RR_SUCCESS;
+        case RRX_TUNING_TRACE_2L_MIN_BUSY: *value = rr::MinBusyLanes2L(); return RR_SUCCESS;
+        case RRX_TUNING_TRACE_2L_RAYS_PER_FETCH: *value = rr::RaysPerFetch2L(); return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

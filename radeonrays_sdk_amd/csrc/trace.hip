@@ -1284,6 +1284,21 @@ bool SetRaysPerFetch(uint32_t v) {
     g_fetch.store(v, std::memory_order_relaxed);
     return true;
 }
+
+static std::atomic<uint32_t> g_min_busy_2l{kDefaultMinBusy2L};
+static std::atomic<uint32_t> g_fetch_2l{kDefaultFetch2L};
+uint32_t MinBusyLanes2L() { return g_min_busy_2l.load(std::memory_order_relaxed); }
+bool SetMinBusyLanes2L(uint32_t v) {
+    if (v < 1 || v > 64) return false;
+    g_min_busy_2l.store(v, std::memory_order_relaxed);
+    return true;
+}
+uint32_t RaysPerFetch2L() { return g_fetch_2l.load(std::memory_order_relaxed); }
+bool SetRaysPerFetch2L(uint32_t v) {
+    if (v != 32 && v != 64 && v != 128) return false;
+    g_fetch_2l.store(v, std::memory_order_relaxed);
+    return true;
+}
 uint32_t LdsStackDepth() { return g_depth.load(std::memory_order_relaxed); }
 bool SetLdsStackDepth(uint32_t v) {
     if (v != 8 && v != 16 && v != 32) return false;
@@ -1388,14 +1403,14 @@ uint32_t QueryBlocks(uint64_t n, bool two) {
 uint32_t BatchBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatchKernel), 1); }
 uint32_t Batch2LBlocks() { return ResidentBlocks(reinterpret_cast<const void*>(&TraceBatch2LKernel), 7); }
 
-// Rays per queue atomic for n rays over `blocks` blocks: the tuned value,
-// cut (to a multiple of 32, at least 32) when fewer rays than waves x fetch
-// would leave waves without any.
-uint32_t FetchFor(uint32_t n, uint32_t blocks) {
+// Rays per queue atomic for n rays over `blocks` blocks: the level's tuned
+// value, cut (to a multiple of 32, at least 32) when fewer rays than waves x
+// fetch would leave waves without any.
+uint32_t FetchFor(uint32_t n, uint32_t blocks, bool two) {
     const uint64_t waves = (uint64_t)blocks * (kBlock / 64);
     const uint64_t per = (n + waves - 1) / waves;
     const uint32_t f = (uint32_t)std::max<uint64_t>(32, (per + 31) / 32 * 32);
-    return std::min(RaysPerFetch(), f);
+    return std::min(two ? RaysPerFetch2L() : RaysPerFetch(), f);
 }
 
 template <bool ANY, bool TWO, int OUT, int DEPTH>
@@ -1409,7 +1424,7 @@ void LaunchPersistentKernel(uint32_t blocks, hipStream_t s, const uint8_t* a, co
 template <bool ANY, bool TWO, int DEPTH>
 void LaunchPersistentDepth(OutputKind out, uint32_t blocks, hipStream_t s, const uint8_t* a, const float4* r,
                            uint32_t n, const uint32_t* ind, void* hits, uint32_t* sc, uint32_t init) {
-    const uint32_t mb = MinBusyLanes(), f = FetchFor(n, blocks);
+    const uint32_t mb = TWO ? MinBusyLanes2L() : MinBusyLanes(), f = FetchFor(n, blocks, TWO);
     switch (out) {
         case kOutFullHit: LaunchPersistentKernel<ANY, TWO, kOutFullHit, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f, init); break;
         case kOutInstanceId: LaunchPersistentKernel<ANY, TWO, kOutInstanceId, DEPTH>(blocks, s, a, r, n, ind, hits, sc, mb, f, init); break;

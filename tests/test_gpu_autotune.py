@@ -144,3 +144,39 @@ def test_query_grid_size_keeps_results(ctx, dev):
                 assert torch.equal(got, refs[(what, n, q)]), f"batch: {what}, {n} rays, {q}, rays per lane {k}"
     finally:
         autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_RAYS_PER_LANE, start)
+
+
+def test_two_level_knobs_keep_results(ctx, dev):
+    """RRX_TUNING_TRACE_2L_MIN_BUSY / _2L_RAYS_PER_FETCH tune only the
+    persistent two-level kernel's refills: its hits must equal the plain
+    kernel's at every setting, and the single-level knobs must not move them."""
+    import numpy as np
+    import torch
+
+    say("two-level refill knobs on a 24-instance scene, 200k rays")
+    lib = ctx.lib
+    soups = [scenes.triangle_soup(n, scenes.rng_for(12, n), sigma=0.2) for n in (5000, 40000)]
+    blases = [ctx.build_geometry(to_dev(v, dev), to_dev(i.reshape(-1, 3), dev), restructure=True) for v, i in soups]
+    tlas = ctx.build_scene([blases[k % 2] for k in range(24)], scenes.random_transforms(scenes.rng_for(12, 5), 24,
+                                                                                         trans=1.5))
+    rng = scenes.rng_for(12, 6)
+    o = rng.uniform(-3, 3, size=(200000, 3)).astype(np.float32)
+    rays = to_dev(scenes.make_rays(o, scenes.unit_sphere(200000, rng)), dev)
+    torch.cuda.synchronize()
+    keys = (autotune.rr.RRX_TUNING_TRACE_2L_MIN_BUSY, autotune.rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH)
+    start = [autotune.get(lib, k) for k in keys]
+    try:
+        for query in ("closest", "any"):
+            with autotune.plain_traversal(lib):
+                ref = ctx.intersect(tlas, rays, query, "full_t")
+            for mb, f in ((1, 32), (16, 128), (32, 64), (64, 64), (48, 32)):
+                autotune.set_(lib, keys[0], mb)
+                autotune.set_(lib, keys[1], f)
+                got = ctx.intersect(tlas, rays, query, "full_t")
+                torch.cuda.synchronize()
+                assert torch.equal(got, ref), f"{query}, min busy {mb}, fetch {f}"
+            hit = api.decode_hits(ref, "full_t")["prim_id"] != 0xFFFFFFFF
+            assert int(hit.sum()) > 2000, f"{query}: almost no hits"
+    finally:
+        for k, v in zip(keys, start):
+            autotune.set_(lib, k, v)
