@@ -113,7 +113,7 @@ typedef enum RRxTuning
                                              RRX_TUNING_TRACE_MIN_BUSY sets the single-level and
                                              the batch kernels */
     RRX_TUNING_TRACE_2L_RAYS_PER_FETCH = 10 /* RRX_TUNING_TRACE_RAYS_PER_FETCH for the same kernel
-                                             (32, 64, 128; default 64) */
+                                             (32, 64, 128; default 32) */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

@@ -46,7 +46,7 @@ bool SetRaysPerFetch(uint32_t rays);
 // The same two knobs for the persistent two-level kernel, whose rays are
 // longer: the sweep's best on the 64-BLAS, 4,096-instance scene.
 constexpr uint32_t kDefaultMinBusy2L = 32;
-constexpr uint32_t kDefaultFetch2L = 64;
+constexpr uint32_t kDefaultFetch2L = 32;
 uint32_t MinBusyLanes2L();
 bool SetMinBusyLanes2L(uint32_t lanes);
 uint32_t RaysPerFetch2L();

@@ -147,12 +147,12 @@ def test_tuning_validation():
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_LANE, byref(v)) == rr.RR_SUCCESS and v.value == 0
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_LANE, 2) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_MIN_BUSY, byref(v)) == rr.RR_SUCCESS and v.value == 32
-    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, byref(v)) == rr.RR_SUCCESS and v.value == 64
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, byref(v)) == rr.RR_SUCCESS and v.value == 32
     # The single-level knobs leave the two-level ones alone.
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_MIN_BUSY, 40) == rr.RR_SUCCESS
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, 32) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_MIN_BUSY, byref(v)) == rr.RR_SUCCESS and v.value == 32
-    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, byref(v)) == rr.RR_SUCCESS and v.value == 64
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, byref(v)) == rr.RR_SUCCESS and v.value == 32
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_MIN_BUSY, 24) == rr.RR_SUCCESS
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_RAYS_PER_FETCH, 128) == rr.RR_SUCCESS
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_2L_MIN_BUSY, 16) == rr.RR_SUCCESS
