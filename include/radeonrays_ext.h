@@ -108,10 +108,10 @@ typedef enum RRxTuning
                                              Queries that do not fill the resident grid finish
                                              sooner on fewer waves per SIMD.  0: the resident grid
                                              whenever the rays fill it */
-    RRX_TUNING_TRACE_2L_MIN_BUSY = 9,     /* RRX_TUNING_TRACE_MIN_BUSY for the persistent two-level
-                                             kernel of rrCmdIntersect on a scene (1..64; default 32);
-                                             RRX_TUNING_TRACE_MIN_BUSY sets the single-level and
-                                             the batch kernels */
+    RRX_TUNING_TRACE_2L_MIN_BUSY = 9,     /* RRX_TUNING_TRACE_MIN_BUSY for the two-level kernels
+                                             (rrCmdIntersect on a scene, scene requests of
+                                             rrxCmdIntersectBatch; 1..64; default 32);
+                                             RRX_TUNING_TRACE_MIN_BUSY sets the single-level ones */
     RRX_TUNING_TRACE_2L_RAYS_PER_FETCH = 10 /* RRX_TUNING_TRACE_RAYS_PER_FETCH for the same kernel
                                              (32, 64, 128; default 32) */
 } RRxTuning;

@@ -43,8 +43,9 @@ constexpr uint32_t kDefaultFetch = 128;
 constexpr uint32_t kDefaultLdsStack = 16;
 uint32_t RaysPerFetch();
 bool SetRaysPerFetch(uint32_t rays);
-// The same two knobs for the persistent two-level kernel, whose rays are
-// longer: the sweep's best on the 64-BLAS, 4,096-instance scene.
+// The same two knobs for the two-level kernels (the persistent one and the
+// batch kernel's scene requests, which take the refill threshold), whose
+// rays are longer: the sweep's best on the 64-BLAS, 4,096-instance scene.
 constexpr uint32_t kDefaultMinBusy2L = 32;
 constexpr uint32_t kDefaultFetch2L = 32;
 uint32_t MinBusyLanes2L();

@@ -1543,7 +1543,8 @@ void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunk
     }
     if (n2) {
         const uint32_t b = std::min(Batch2LBlocks(), QueryBlocks((uint64_t)n2 * kBatchChunk, true));
-        TraceBatch2LKernel<<<b, kBlock, 0, s>>>(table, requests, geo_chunks, n2, head, spill, mb, DeviceErrorWords());
+        TraceBatch2LKernel<<<b, kBlock, 0, s>>>(table, requests, geo_chunks, n2, head, spill, MinBusyLanes2L(),
+                                                DeviceErrorWords());
     }
 }
 

@@ -173,8 +173,12 @@ def test_two_level_knobs_keep_results(ctx, dev):
                 autotune.set_(lib, keys[0], mb)
                 autotune.set_(lib, keys[1], f)
                 got = ctx.intersect(tlas, rays, query, "full_t")
+                # The batch kernel's scene requests take the same threshold.
+                parts = ctx.intersect_batch([(tlas, rays[:70000], query, "full_t"),
+                                             (tlas, rays[70000:], query, "full_t")])
                 torch.cuda.synchronize()
                 assert torch.equal(got, ref), f"{query}, min busy {mb}, fetch {f}"
+                assert torch.equal(torch.cat(parts), ref), f"batch: {query}, min busy {mb}"
             hit = api.decode_hits(ref, "full_t")["prim_id"] != 0xFFFFFFFF
             assert int(hit.sum()) > 2000, f"{query}: almost no hits"
     finally:
