@@ -111,9 +111,13 @@ typedef enum RRxTuning
     RRX_TUNING_TRACE_2L_MIN_BUSY = 9,     /* RRX_TUNING_TRACE_MIN_BUSY for the two-level kernels
                                              (rrCmdIntersect on a scene, scene requests of
                                              rrxCmdIntersectBatch; 1..64; default 32);
-                                             RRX_TUNING_TRACE_MIN_BUSY sets the single-level ones */
-    RRX_TUNING_TRACE_2L_RAYS_PER_FETCH = 10 /* RRX_TUNING_TRACE_RAYS_PER_FETCH for the same kernel
-                                             (32, 64, 128; default 32) */
+                                             RRX_TUNING_TRACE_MIN_BUSY sets the persistent
+                                             single-level kernel's */
+    RRX_TUNING_TRACE_2L_RAYS_PER_FETCH = 10, /* RRX_TUNING_TRACE_RAYS_PER_FETCH for the persistent
+                                             two-level kernel (32, 64, 128; default 32) */
+    RRX_TUNING_TRACE_BATCH_MIN_BUSY = 11  /* RRX_TUNING_TRACE_MIN_BUSY for the geometry requests
+                                             of rrxCmdIntersectBatch (1..64; default 24; batches of
+                                             many small BLASes run faster at 12) */
 } RRxTuning;
 
 RR_API RRError rrxSetTuning(RRxTuning key, uint32_t value);

@@ -1406,6 +1406,8 @@ RRError rrxSetTuning(RRxTuning key, uint32_t value) {
             return rr::SetMinBusyLanes2L(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
         case RRX_TUNING_TRACE_2L_RAYS_PER_FETCH:
             return rr::SetRaysPerFetch2L(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
+        case RRX_TUNING_TRACE_BATCH_MIN_BUSY:
+            return rr::SetMinBusyLanesBatch(value) ? RR_SUCCESS : RR_ERROR_INVALID_PARAMETER;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }
@@ -1424,6 +1426,7 @@ RRError rrxGetTuning(RRxTuning key, uint32_t* value) {
         case RRX_TUNING_TRACE_RAYS_PER_LANE: *value = rr::RaysPerLane(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_2L_MIN_BUSY: *value = rr::MinBusyLanes2L(); return RR_SUCCESS;
         case RRX_TUNING_TRACE_2L_RAYS_PER_FETCH: *value = rr::RaysPerFetch2L(); return RR_SUCCESS;
+        case RRX_TUNING_TRACE_BATCH_MIN_BUSY: *value = rr::MinBusyLanesBatch(); return RR_SUCCESS;
     }
     return RR_ERROR_INVALID_PARAMETER;
 }

@@ -52,6 +52,12 @@ uint32_t MinBusyLanes2L();
 bool SetMinBusyLanes2L(uint32_t lanes);
 uint32_t RaysPerFetch2L();
 bool SetRaysPerFetch2L(uint32_t rays);
+// Refill threshold of the batch kernel's geometry requests (64-ray chunks).
+// 24 suits the serving mix (BLASes of 10k-2M triangles); batches of many
+// small BLASes with short rays run faster at 12.
+constexpr uint32_t kDefaultMinBusyBatch = 24;
+uint32_t MinBusyLanesBatch();
+bool SetMinBusyLanesBatch(uint32_t lanes);
 uint32_t LdsStackDepth();
 bool SetLdsStackDepth(uint32_t entries);
 

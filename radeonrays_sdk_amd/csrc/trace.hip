@@ -1293,6 +1293,13 @@ bool SetMinBusyLanes2L(uint32_t v) {
     g_min_busy_2l.store(v, std::memory_order_relaxed);
     return true;
 }
+static std::atomic<uint32_t> g_min_busy_batch{kDefaultMinBusyBatch};
+uint32_t MinBusyLanesBatch() { return g_min_busy_batch.load(std::memory_order_relaxed); }
+bool SetMinBusyLanesBatch(uint32_t v) {
+    if (v < 1 || v > 64) return false;
+    g_min_busy_batch.store(v, std::memory_order_relaxed);
+    return true;
+}
 uint32_t RaysPerFetch2L() { return g_fetch_2l.load(std::memory_order_relaxed); }
 bool SetRaysPerFetch2L(uint32_t v) {
     if (v != 32 && v != 64 && v != 128) return false;
@@ -1531,7 +1538,7 @@ void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunk
     uint8_t* base = static_cast<uint8_t*>(scratch);
     uint32_t* head = reinterpret_cast<uint32_t*>(base);
     auto* spill = reinterpret_cast<uint32_t*>(base + kBatchTableOffset + BatchTableBytes(requests));
-    const uint32_t mb = MinBusyLanes();
+    const uint32_t mb = MinBusyLanesBatch();
     BatchInitKernel<<<1, 64, 0, s>>>(head);
     // The geometry kernel takes chunks [0, geo + unrecorded), the two-level
     // kernel [geo, geo + unrecorded + scene); a kernel with no chunk is not

@@ -64,6 +64,7 @@ RRX_TUNING_SORT_HYBRID = 7
 RRX_TUNING_TRACE_RAYS_PER_LANE = 8
 RRX_TUNING_TRACE_2L_MIN_BUSY = 9
 RRX_TUNING_TRACE_2L_RAYS_PER_FETCH = 10
+RRX_TUNING_TRACE_BATCH_MIN_BUSY = 11
 
 RRX_DEVICE_ERROR_STACK_OVERFLOW = 1 << 0
 RRX_DEVICE_ERROR_SORT_LOOKBACK = 1 << 1

@@ -126,7 +126,8 @@ def test_tuning_validation():
                      (rr.RRX_TUNING_TRACE_MIN_BUSY, 0), (rr.RRX_TUNING_TRACE_MIN_BUSY, 65),
                      (rr.RRX_TUNING_SORT_HYBRID, 2), (rr.RRX_TUNING_TRACE_RAYS_PER_LANE, 65),
                      (rr.RRX_TUNING_TRACE_2L_MIN_BUSY, 0), (rr.RRX_TUNING_TRACE_2L_MIN_BUSY, 65),
-                     (rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, 48), (11, 1)):
+                     (rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, 48),
+                     (rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, 0), (rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, 65), (12, 1)):
         assert lib.rrxSetTuning(key, bad) == rr.RR_ERROR_INVALID_PARAMETER
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, None) == rr.RR_ERROR_INVALID_PARAMETER
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_LDS_STACK, 32) == rr.RR_SUCCESS
@@ -158,6 +159,10 @@ def test_tuning_validation():
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_2L_MIN_BUSY, 16) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_2L_MIN_BUSY, byref(v)) == rr.RR_SUCCESS and v.value == 16
     assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_2L_MIN_BUSY, 32) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, byref(v)) == rr.RR_SUCCESS and v.value == 24
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, 40) == rr.RR_SUCCESS
+    assert lib.rrxGetTuning(rr.RRX_TUNING_TRACE_MIN_BUSY, byref(v)) == rr.RR_SUCCESS and v.value == 24
+    assert lib.rrxSetTuning(rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, 24) == rr.RR_SUCCESS
     assert lib.rrxGetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, byref(v)) == rr.RR_SUCCESS and v.value == 8
     for bad in (0, 6, 9, 16):
         assert lib.rrxSetTuning(rr.RRX_TUNING_SORT_DIGIT_BITS, bad) == rr.RR_ERROR_INVALID_PARAMETER

@@ -184,3 +184,34 @@ def test_two_level_knobs_keep_results(ctx, dev):
     finally:
         for k, v in zip(keys, start):
             autotune.set_(lib, k, v)
+
+
+def test_batch_refill_knob_keeps_results(ctx, dev):
+    """RRX_TUNING_TRACE_BATCH_MIN_BUSY moves only when the batch kernel's
+    waves refill: the geometry requests' hits must equal the plain kernel's
+    at every setting."""
+    import torch
+
+    say("batch refill threshold 1/12/24/64 on a 256x256 heightfield, 6 requests")
+    lib = ctx.lib
+    w = scenes.heightfield(res=256, n_rays=300000)
+    blas = ctx.build_geometry(to_dev(w.vertices, dev), to_dev(w.indices, dev), restructure=True)
+    rays = to_dev(w.rays, dev)
+    cuts = [0, 1, 700, 50000, 50001, 200000, 300000]
+    torch.cuda.synchronize()
+    refs = []
+    with autotune.plain_traversal(lib):
+        for k in range(len(cuts) - 1):
+            q = "closest" if k % 2 else "any"
+            refs.append(ctx.intersect(blas, rays[cuts[k]:cuts[k + 1]].contiguous(), q, "full_t"))
+    start = autotune.get(lib, autotune.rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY)
+    try:
+        for mb in (1, 12, 24, 64):
+            autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, mb)
+            outs = ctx.intersect_batch([(blas, rays[cuts[k]:cuts[k + 1]].contiguous(), "closest" if k % 2 else "any",
+                                         "full_t") for k in range(len(cuts) - 1)])
+            torch.cuda.synchronize()
+            for k, (got, ref) in enumerate(zip(outs, refs)):
+                assert torch.equal(got, ref), f"request {k}, min busy {mb}"
+    finally:
+        autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, start)
