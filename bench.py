@@ -340,7 +340,7 @@ def serving_bench(ctx, dev, n_requests, log):
     }
 
 
-def instanced_bench(ctx, dev, log, reps=5):
+def instanced_bench(ctx, dev, log, reps=5, tune=True):
     """Two-level scene: 64 soups (log-uniform 10k-500k triangles) under 4,096
     instances; 2048x2048 primary rays from (0, 0, -150), then 8,388,608
     cosine-weighted secondary rays from the primary hits (no-op rays where
@@ -374,13 +374,25 @@ def instanced_bench(ctx, dev, log, reps=5):
     prim_hits = ctx.empty(primary.shape[0] * HIT_BYTES)
     ms_sec = timed(lambda: ctx.intersect(tlas, dsec, "closest", "full", hits=hits2, scratch=scratch), reps)
     ms_prim = timed(lambda: ctx.intersect(tlas, dprim, "closest", "full", hits=prim_hits, scratch=scratch), reps)
+    tuned = None
+    if tune:
+        from radeonrays_sdk_amd import autotune
+
+        log("instanced: two-level refill sweep on the secondary rays")
+        keys = (autotune.rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, autotune.rr.RRX_TUNING_TRACE_2L_MIN_BUSY)
+        start = [autotune.get(ctx.lib, k) for k in keys]
+        t = autotune.tune_two_level(ctx, tlas, dsec, reps=3)
+        for k, v in zip(keys, start):  # the other sections run with the defaults
+            autotune.set_(ctx.lib, k, v)
+        tuned = {"default": t["default"], "best": t["best"], "configs": len(t["table"]),
+                 "note": "hits identical under every setting"}
     return {"workload": f"64 soups ({sum(i.shape[0] for _, i in w.meshes)} tris) x 4096 instances, "
                         f"{n} secondary rays (2 per pixel of a 2048x2048 primary pass), closest, FULL_HIT",
             "secondary_mrays_s": round(n / ms_sec / 1e3, 2), "secondary_ms": round(ms_sec, 4),
             "noop_rays": int((sec[:, 7] < 0).sum()),
             "primary_mrays_s": round(primary.shape[0] / ms_prim / 1e3, 2), "primary_ms": round(ms_prim, 4),
             "primary_hit_fraction": round(float((h["inst_id"] != 0xFFFFFFFF).mean()), 4),
-            "tlas_build_ms_wall": round(tlas_ms, 3)}
+            "tlas_build_ms_wall": round(tlas_ms, 3), "two_level_tuning": tuned}
 
 
 def soup_bench(ctx, dev, log, reps=10):
@@ -417,7 +429,7 @@ def soup_bench(ctx, dev, log, reps=10):
     return out, accel, rays, w.rays
 
 
-def batched_bench(ctx, dev, log, frames=16):
+def batched_bench(ctx, dev, log, frames=16, tune=True):
     """Batched build / refit: 8,192 soups (log-uniform 64-16,384 triangles)
     built by one rrxCmdBuildGeometryBatch (against one rrCmdBuildGeometry per
     mesh), then refit + query frames replayed from a HIP graph: each frame
@@ -510,6 +522,17 @@ def batched_bench(ctx, dev, log, frames=16):
         torch.cuda.synchronize()
         frame_ms.append(a.elapsed_time(b))
     fm = float(np.median(frame_ms))
+    tuned = None
+    if tune:
+        from radeonrays_sdk_amd import autotune
+
+        log("batched: batch refill threshold sweep on the frame's query")
+        key = autotune.rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY
+        start = autotune.get(ctx.lib, key)
+        t = autotune.tune_batch(ctx, requests, reps=3, scratch=qscratch)
+        autotune.set_(ctx.lib, key, start)  # the other sections run with the defaults
+        tuned = {"default": t["default"], "best": t["best"], "table": t["table"],
+                 "note": "graph replays; hits identical under every setting"}
     tris = int(w.tri_counts.sum())
     return {"workload": f"{M} soups, {tris} tris (log-uniform 64-16384), {rpm} any-hit rays per mesh per frame, "
                         f"{frames} refit+query frames replayed from a HIP graph",
@@ -519,7 +542,8 @@ def batched_bench(ctx, dev, log, frames=16):
             "per_call_build_ms_wall": round(per_call_ms, 1), "batch_equals_per_call": bool(same),
             "refit_ms": round(refit_ms, 3), "refit_gbs": round(tris * 304 / (refit_ms * 1e-3) / 1e9, 1),
             "query_ms": round(query_ms, 3), "query_mrays_s": round(R / query_ms / 1e3, 1),
-            "graph_frame_ms": round(fm, 3), "graph_frame_mrays_s": round(R / fm / 1e3, 1)}, w
+            "graph_frame_ms": round(fm, 3), "graph_frame_mrays_s": round(R / fm / 1e3, 1),
+            "query_tuning": tuned}, w
 
 
 def profile_section(which: str, reps: int = 3) -> None:
@@ -785,10 +809,10 @@ def main():
     if not args.no_soup:
         out["soup"], *soup = soup_bench(ctx, dev, log)
     if not args.no_instanced:
-        out["instanced"] = instanced_bench(ctx, dev, log)
+        out["instanced"] = instanced_bench(ctx, dev, log, tune=not args.no_autotune)
     batched_w = None
     if not args.no_batched:
-        out["batched"], batched_w = batched_bench(ctx, dev, log)
+        out["batched"], batched_w = batched_bench(ctx, dev, log, tune=not args.no_autotune)
     if not args.no_serving:
         out["serving"] = serving_bench(ctx, dev, args.serving_requests, log)
 

@@ -17,6 +17,11 @@ The treelet size of the restructuring (5, 7 or 9 leaves) is swept by
 tune_treelet: it trades build time (the subset dynamic programme grows as
 3^leaves) against tree quality (traversal time); closest hits are the same
 for every size.
+
+The two-level kernel has its own refill threshold and rays per fetch
+(tune_two_level, on the caller's scene and rays), and the batch kernel's
+geometry requests their own refill threshold (tune_batch, on the caller's
+batch of requests): the best values depend on how long the rays are.
 """
 from __future__ import annotations
 
@@ -228,4 +233,132 @@ def tune_treelet(ctx, vertices, indices, rays, reps: int = 3, leaves=TREELET_LEA
     best = min(table, key=lambda r: r["score_ms"])
     default = next((r for r in table if r["treelet_leaves"] == start), None)
     set_(lib, rr.RRX_TUNING_TREELET_LEAVES, best["treelet_leaves"])
+    return {"best": best, "default": default, "table": table}
+
+
+def _timed_median(run, reps: int) -> float:
+    import torch
+
+    run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        run()
+        b.record()
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+
+TWO_LEVEL_FETCH = (32, 64, 128)
+TWO_LEVEL_MIN_BUSY = (16, 24, 32, 40)
+
+
+def tune_two_level(ctx, scene, rays, queries=("closest",), reps: int = 5, fetch=TWO_LEVEL_FETCH,
+                   min_busy=TWO_LEVEL_MIN_BUSY) -> dict:
+    """Sweep the two-level kernel's rays per fetch and refill threshold
+    (RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, RRX_TUNING_TRACE_2L_MIN_BUSY) on
+    rrCmdIntersect of (scene, rays); returns {"best", "default", "table"} and
+    leaves the best set.  Raises if any setting's hits differ from the
+    starting setting's."""
+    import torch
+
+    lib = ctx.lib
+    keys = {"rays_per_fetch": rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, "min_busy": rr.RRX_TUNING_TRACE_2L_MIN_BUSY}
+    start = {k: get(lib, v) for k, v in keys.items()}
+    n = int(rays.shape[0])
+    hits = ctx.empty(n * 16)
+    scratch = ctx.trace_scratch(n)
+
+    def run():
+        for q in queries:
+            ctx.intersect(scene, rays, q, "full", hits=hits, scratch=scratch)
+
+    def sample():
+        outs = []
+        for q in queries:
+            ctx.intersect(scene, rays, q, "full", hits=hits, scratch=scratch)
+            torch.cuda.synchronize()
+            outs.append(hits.cpu().numpy().copy())
+        return outs
+
+    reference = sample()
+    table = []
+    try:
+        for f, m in itertools.product(fetch, min_busy):
+            cfg = {"rays_per_fetch": f, "min_busy": m}
+            for k, v in cfg.items():
+                set_(lib, keys[k], v)
+            if not all(np.array_equal(a, b) for a, b in zip(sample(), reference)):
+                raise RuntimeError(f"hits differ under two-level tuning {cfg}")
+            ms = _timed_median(run, reps)
+            table.append({**cfg, "ms": round(ms, 4), "mrays_s": round(len(queries) * n / ms / 1e3, 1)})
+    except BaseException:
+        for k, v in start.items():
+            set_(lib, keys[k], v)
+        raise
+    best = min(table, key=lambda r: r["ms"])
+    default = next((r for r in table if all(r[k] == start[k] for k in start)), None)
+    for k in start:
+        set_(lib, keys[k], best[k])
+    return {"best": best, "default": default, "table": table}
+
+
+BATCH_MIN_BUSY = (8, 12, 16, 24, 32)
+
+
+def _graph_of(ctx, fn):
+    """One HIP graph of fn's work on a side stream (the bench's capture recipe)."""
+    import torch
+
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
+            with ctx.on_stream(s):
+                fn()
+    torch.cuda.synchronize()
+    return g
+
+
+def tune_batch(ctx, requests, reps: int = 5, min_busy=BATCH_MIN_BUSY, scratch=None) -> dict:
+    """Sweep the batch kernel's refill threshold for geometry requests
+    (RRX_TUNING_TRACE_BATCH_MIN_BUSY) on rrxCmdIntersectBatch of the caller's
+    requests ([(accel, rays, query, output[, hits])], as
+    api.Context.intersect_batch takes them); returns {"best", "default",
+    "table"} and leaves the best set.  Each setting is captured in a HIP
+    graph (the threshold is a launch argument) and timed by replays, so the
+    host's per-call table set-up of large batches does not enter the times.
+    Raises if any setting's hits differ from the starting setting's."""
+    import torch
+
+    lib = ctx.lib
+    key = rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY
+    start = get(lib, key)
+    if scratch is None:
+        counts = [int(r[1].shape[0]) for r in requests]
+        scratch = ctx.batch_scratch(len(requests), sum(counts), max(counts + [1]))
+    outs = ctx.intersect_batch([r[:4] for r in requests], scratch=scratch)
+    reqs = [tuple(r[:4]) + (o,) for r, o in zip(requests, outs)]
+    torch.cuda.synchronize()
+    reference = [o.cpu().numpy().copy() for o in outs]
+    table = []
+    try:
+        for m in min_busy:
+            set_(lib, key, m)
+            g = _graph_of(ctx, lambda: ctx.intersect_batch(reqs, scratch=scratch))
+            for o in outs:
+                o.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            if not all(np.array_equal(o.cpu().numpy(), r) for o, r in zip(outs, reference)):
+                raise RuntimeError(f"batch hits differ at refill threshold {m}")
+            table.append({"min_busy": m, "ms": round(_timed_median(g.replay, reps), 4)})
+    except BaseException:
+        set_(lib, key, start)
+        raise
+    best = min(table, key=lambda r: r["ms"])
+    default = next((r for r in table if r["min_busy"] == start), None)
+    set_(lib, key, best["min_busy"])
     return {"best": best, "default": default, "table": table}

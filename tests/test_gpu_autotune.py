@@ -215,3 +215,34 @@ def test_batch_refill_knob_keeps_results(ctx, dev):
                 assert torch.equal(got, ref), f"request {k}, min busy {mb}"
     finally:
         autotune.set_(lib, autotune.rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY, start)
+
+
+def test_two_level_and_batch_tuners(ctx, dev):
+    """tune_two_level and tune_batch sweep their knobs on the caller's own
+    work, raise if any setting changes a hit, and leave the fastest set."""
+    import numpy as np
+    import torch
+
+    say("tune_two_level on a 16-instance scene, tune_batch on 40 requests")
+    lib = ctx.lib
+    v, i = scenes.triangle_soup(20000, scenes.rng_for(13, 1), sigma=0.2)
+    blas = ctx.build_geometry(to_dev(v, dev), to_dev(i.reshape(-1, 3), dev), restructure=True)
+    tlas = ctx.build_scene([blas] * 16, scenes.random_transforms(scenes.rng_for(13, 2), 16, trans=1.5))
+    rng = scenes.rng_for(13, 3)
+    o = rng.uniform(-3, 3, size=(100000, 3)).astype(np.float32)
+    rays = to_dev(scenes.make_rays(o, scenes.unit_sphere(100000, rng)), dev)
+    torch.cuda.synchronize()
+    keys2 = (autotune.rr.RRX_TUNING_TRACE_2L_RAYS_PER_FETCH, autotune.rr.RRX_TUNING_TRACE_2L_MIN_BUSY)
+    kb = autotune.rr.RRX_TUNING_TRACE_BATCH_MIN_BUSY
+    start = [autotune.get(lib, k) for k in keys2] + [autotune.get(lib, kb)]
+    try:
+        t = autotune.tune_two_level(ctx, tlas, rays, queries=("closest", "any"), reps=2, min_busy=(16, 32))
+        assert len(t["table"]) == 3 * 2 and t["default"] is not None
+        assert [autotune.get(lib, k) for k in keys2] == [t["best"]["rays_per_fetch"], t["best"]["min_busy"]]
+        reqs = [(blas, rays[k * 2500:(k + 1) * 2500], "any" if k % 3 else "closest", "full") for k in range(40)]
+        b = autotune.tune_batch(ctx, reqs, reps=2, min_busy=(8, 24, 48))
+        assert [r["min_busy"] for r in b["table"]] == [8, 24, 48] and b["default"]["min_busy"] == 24
+        assert autotune.get(lib, kb) == b["best"]["min_busy"]
+    finally:
+        for k, val in zip(keys2 + (kb,), start):
+            autotune.set_(lib, k, val)
