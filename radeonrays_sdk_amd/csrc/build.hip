@@ -36,10 +36,15 @@ constexpr uint32_t kBoundsGrid = 1024;
 inline uint32_t Blocks(size_t n, int per = kBlock) { return (uint32_t)std::max<size_t>(1, (n + per - 1) / per); }
 
 // Scratch block at the head of every build scratch buffer.
+// Ordered-uint bounds (min xyz, pad, max xyz, pad) in kBoundsCopies copies:
+// workgroup b reduces into copy b % kBoundsCopies — one per XCD, as
+// workgroups go to the XCDs round-robin — so device atomics from different
+// XCDs never meet on one address; readers take the union of the copies.
+constexpr int kBoundsCopies = 8;
 struct BuildHead {
-    uint32_t bounds[8];   // ordered-uint min xyz, pad, max xyz, pad
-    uint32_t misc[56];
+    uint32_t bounds[kBoundsCopies][8];
 };
+static_assert(sizeof(BuildHead) == 256, "the build head fills its 256-byte scratch slot");
 
 struct BlasScratchLayout {
     size_t head, keys, vals, keys_tmp, vals_tmp, sort, total;
@@ -100,16 +105,18 @@ __global__ __launch_bounds__(kBlock) void InitBuildKernel(AccelHeader* hdr, uint
         h.build_serial = serial;
         *hdr = h;
     }
-    if (t < 8) head->bounds[t] = (t < 4) ? 0xFFFFFFFFu : 0u;
+    if (t < 8 * kBoundsCopies) (&head->bounds[0][0])[t] = ((t & 7u) < 4) ? 0xFFFFFFFFu : 0u;
     for (uint32_t i = t; i < kSortHistCopies * kSortHistWords; i += kBlock)
         sort_hist[(i / kSortHistWords) * kSortHistStride + i % kSortHistWords] = 0;
     if (t < kSortMaxPasses) sort_tickets[t] = 0;
 }
 
 // Block min/max of ordered-uint coordinates, then 6 device atomics per
-// block (lbvh_calc_mesh_aabb.comp:121-179 reduces in a 128-wide LDS tree
-// with atomics per loop iteration).
-__device__ __forceinline__ void ReduceBoundsToGlobal(uint32_t mn[3], uint32_t mx[3], uint32_t* bounds) {
+// block into the block's copy of the bounds (lbvh_calc_mesh_aabb.comp:121-179
+// reduces in a 128-wide LDS tree with atomics per loop iteration).
+__device__ __forceinline__ void ReduceBoundsToGlobal(uint32_t mn[3], uint32_t mx[3],
+                                                     uint32_t (*copies)[8]) {
+    uint32_t* bounds = copies[blockIdx.x % kBoundsCopies];
     __shared__ uint32_t s_red[kBlock / 64][6];
     for (int a = 0; a < 3; ++a) {
         mn[a] = WaveMin(mn[a]);
@@ -128,17 +135,27 @@ __device__ __forceinline__ void ReduceBoundsToGlobal(uint32_t mn[3], uint32_t mx
     }
 }
 
+// Each step loads kBoundsTris triangles a grid stride apart before using
+// any of them, so a thread waits for one index load and one vertex load
+// per step rather than per triangle.  A triangle index past the end is
+// replaced by the last triangle, which leaves the bounds unchanged.
+constexpr int kBoundsTris = 4;
 __global__ __launch_bounds__(kBlock) void MeshBoundsKernel(MeshView m, BuildHead* head) {
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < m.tri_count; t += gridDim.x * kBlock) {
-        float v[9];
-        LoadTriangle(m, t, v);
-        for (int k = 0; k < 3; ++k)
-            for (int a = 0; a < 3; ++a) {
-                const uint32_t o = FloatToOrdered(v[3 * k + a]);
-                mn[a] = min(mn[a], o);
-                mx[a] = max(mx[a], o);
-            }
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t last = m.tri_count - 1;  // launched only for tri_count >= 1
+    for (uint64_t t0 = blockIdx.x * kBlock + threadIdx.x; t0 < m.tri_count; t0 += kBoundsTris * stride) {
+        float v[kBoundsTris][9];
+#pragma unroll
+        for (int j = 0; j < kBoundsTris; ++j) LoadTriangle(m, (uint32_t)min(t0 + j * stride, last), v[j]);
+#pragma unroll
+        for (int j = 0; j < kBoundsTris; ++j)
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    const uint32_t o = FloatToOrdered(v[j][3 * k + a]);
+                    mn[a] = min(mn[a], o);
+                    mx[a] = max(mx[a], o);
+                }
     }
     ReduceBoundsToGlobal(mn, mx, head->bounds);
 }
@@ -149,8 +166,13 @@ struct SceneBounds {
 __device__ __forceinline__ SceneBounds ReadBounds(const BuildHead* head) {
     SceneBounds b;
     for (int a = 0; a < 3; ++a) {
-        const float lo = OrderedToFloat(head->bounds[a]);
-        const float hi = OrderedToFloat(head->bounds[4 + a]);
+        uint32_t lo_o = head->bounds[0][a], hi_o = head->bounds[0][4 + a];
+        for (int c = 1; c < kBoundsCopies; ++c) {
+            lo_o = min(lo_o, head->bounds[c][a]);
+            hi_o = max(hi_o, head->bounds[c][4 + a]);
+        }
+        const float lo = OrderedToFloat(lo_o);
+        const float hi = OrderedToFloat(hi_o);
         b.lo[a] = lo;
         b.ext[a] = hi - lo;
     }
@@ -1137,7 +1159,7 @@ void BuildBlas(const MeshDesc& mesh, bool restructure, void* scratch, void* resu
                                          sc.hist, sc.tickets);
     if (n == 0) return;
     const MeshView m = ViewOf(mesh);
-    MeshBoundsKernel<<<std::min(kBoundsGrid, Blocks(n)), kBlock, 0, s>>>(m, head);
+    MeshBoundsKernel<<<std::min(kBoundsGrid, (Blocks(n) + kBoundsTris - 1) / kBoundsTris), kBlock, 0, s>>>(m, head);
     uint32_t* keys = At<uint32_t>(scratch, L.keys);
     uint32_t* vals = At<uint32_t>(scratch, L.vals);
     const int passes = SortPasses(30);
