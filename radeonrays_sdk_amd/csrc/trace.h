@@ -97,10 +97,11 @@ static_assert(sizeof(BatchRequest) == 48, "BatchRequest must be 48 bytes");
 constexpr uint32_t kBatchChunk = 64;
 
 // Scratch for a batch of `requests` requests and `total_rays` rays: a
-// 1 KB head (the two kernels' chunk counters), the request table (uploaded
-// to byte kBatchTableOffset, BatchTableBytes long), then spill slots.
+// 2 KB head (the two kernels' chunk counters, eight each), the request
+// table (uploaded to byte kBatchTableOffset, BatchTableBytes long), then
+// spill slots.
 size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays);
-constexpr size_t kBatchTableOffset = 1024;
+constexpr size_t kBatchTableOffset = 2048;
 inline size_t BatchTableBytes(uint32_t requests) {
     return ((size_t)requests * sizeof(BatchRequest) + 255) & ~(size_t)255;
 }

@@ -854,9 +854,15 @@ void LaunchOut(OutputKind out, dim3 g, hipStream_t s, const uint8_t* accel, cons
 // runs per batch: with several streams in flight, a wider helper kernel
 // waits for another stream's persistent launch to leave the CUs.
 
-// Counter words of the two batch kernels (separate 256-byte lines, inside
-// the table offset).
-constexpr uint32_t kBatchCounterWord = 0, kBatch2LCounterWord = 64;
+// Each batch kernel's chunk range is cut into kQueues contiguous parts, one
+// per XCD as the persistent kernels' rays are: a wave starts on part
+// blockIdx % kQueues and moves to the next part once its own is drained, so
+// the chunk atomics of different XCDs land on different words (one 128-byte
+// line each; kBatch2LCounterWord is the two-level kernel's first, all
+// inside the table offset).  One counter for the whole range took every
+// wave's atomic on the same address.
+constexpr uint32_t kBatchCounterWord = 0, kBatch2LCounterWord = kQueues * kQueueStride;
+static_assert(2 * kQueues * kQueueStride * 4 <= kBatchTableOffset, "batch counters inside the head");
 
 // Next chunk of this kernel's range [first, first + count) of the table's
 // chunks, as [q_next, q_end) of request q_req; false once every chunk is
@@ -865,14 +871,23 @@ constexpr uint32_t kBatchCounterWord = 0, kBatch2LCounterWord = 64;
 // only grows, so the loop ends.  Wave-uniform: every lane calls it.
 template <bool TWO_LEVEL>
 __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ table, uint32_t requests,
-                                               uint32_t first, uint32_t count, uint32_t* counter, uint32_t lane,
-                                               uint32_t& q_req, uint32_t& q_next, uint32_t& q_end) {
+                                               uint32_t first, uint32_t count, uint32_t* counters, uint32_t lane,
+                                               uint32_t& part, uint32_t& tries, uint32_t& q_req, uint32_t& q_next,
+                                               uint32_t& q_end) {
     for (;;) {
+        // Part `part` holds chunks [first + p_lo, first + p_hi).
+        const uint32_t p_lo = (uint32_t)((uint64_t)count * part / kQueues);
+        const uint32_t p_hi = (uint32_t)((uint64_t)count * (part + 1) / kQueues);
+        uint32_t* counter = counters + part * kQueueStride;
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(counter, 1u);
         c = __shfl(c, 0, 64);
-        if (c >= count) return false;
-        c += first;
+        if (c >= p_hi - p_lo) {
+            if (++tries == kQueues) return false;
+            part = part + 1 == kQueues ? 0 : part + 1;
+            continue;
+        }
+        c += first + p_lo;
         uint32_t lo = 0, hi = requests;  // last request with first_chunk <= c
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;
@@ -884,7 +899,7 @@ __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ 
             const bool scene = LdG(&hdr->magic) == kAccelMagic && LdG(&hdr->kind) == kAccelTlas;
             if (scene != TWO_LEVEL) {
                 const uint32_t next = lo + 1 < requests ? table[lo + 1].first_chunk : first + count;
-                if (lane == 0) atomicMax(counter, min(next, first + count) - first);
+                if (lane == 0) atomicMax(counter, min(next, first + p_hi) - (first + p_lo));
                 continue;
             }
         }
@@ -906,6 +921,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
 
     // Wave-uniform queue: rays [q_next, q_end) of request q_req.
     uint32_t q_req = 0, q_next = 0, q_end = 0;
+    uint32_t part = blockIdx.x % kQueues, tries = 0;  // this wave's chunk part, parts found drained
     bool drained = false;
 
     // Lane state.
@@ -922,7 +938,8 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
             if (q_next == q_end)
-                drained = !NextBatchChunk<false>(table, requests, first_chunk, chunk_count, counter, lane, q_req, q_next, q_end);
+                drained = !NextBatchChunk<false>(table, requests, first_chunk, chunk_count, counter, lane, part, tries,
+                                                 q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = (uint32_t)__popcll(idle & lt);
             if (rid == kInvalid && rank < take) {
@@ -1050,6 +1067,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
 
     // Wave-uniform queue: rays [q_next, q_end) of request q_req.
     uint32_t q_req = 0, q_next = 0, q_end = 0;
+    uint32_t part = blockIdx.x % kQueues, tries = 0;  // this wave's chunk part, parts found drained
     bool drained = false;
 
     // Lane state.
@@ -1069,7 +1087,8 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
             if (q_next == q_end)
-                drained = !NextBatchChunk<true>(table, requests, first_chunk, chunk_count, counter, lane, q_req, q_next, q_end);
+                drained = !NextBatchChunk<true>(table, requests, first_chunk, chunk_count, counter, lane, part, tries,
+                                                q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = RankIn(idle);
             if (rid == kInvalid && rank < take) {
@@ -1246,9 +1265,9 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
 
 
 __global__ void BatchInitKernel(uint32_t* head) {
-    if (threadIdx.x == 0) {
-        head[kBatchCounterWord] = 0;
-        head[kBatch2LCounterWord] = 0;
+    if (threadIdx.x < kQueues) {
+        head[kBatchCounterWord + threadIdx.x * kQueueStride] = 0;
+        head[kBatch2LCounterWord + threadIdx.x * kQueueStride] = 0;
     }
 }
 
