@@ -865,10 +865,14 @@ constexpr uint32_t kBatchCounterWord = 0, kBatch2LCounterWord = kQueues * kQueue
 static_assert(2 * kQueues * kQueueStride * 4 <= kBatchTableOffset, "batch counters inside the head");
 
 // Next chunk of this kernel's range [first, first + count) of the table's
-// chunks, as [q_next, q_end) of request q_req; false once every chunk is
-// handed out.  An unrecorded request (kBatchFlagCheckLevel) of the other
-// level moves the counter past its chunks with one atomicMax.  The counter
-// only grows, so the loop ends.  Wave-uniform: every lane calls it.
+// chunks, as [q_next, q_end) of request q_req; false once every part is
+// drained.  `part` is the part the wave takes chunks from (counters[part *
+// kQueueStride] counts them) and `tries` how many parts it found drained;
+// a drained part is never revisited, since its counter only grows.  An
+// unrecorded request (kBatchFlagCheckLevel) of the other level moves the
+// part's counter past its chunks, up to the part's end, with one atomicMax.
+// The loop ends because every counter only grows.  Wave-uniform: every lane
+// calls it.
 template <bool TWO_LEVEL>
 __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ table, uint32_t requests,
                                                uint32_t first, uint32_t count, uint32_t* counters, uint32_t lane,
