@@ -1210,11 +1210,15 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
             group[g].push_back(b);
         }
         std::vector<rr::BatchRequest> table;
+        uint64_t total_rays = 0;
+        for (int g = 0; g < 3; ++g)
+            for (const rr::BatchRequest& b : group[g]) total_rays += b.ray_count;
+        const uint32_t chunk = rr::BatchChunkFor(total_rays, request_count);
         uint64_t chunks = 0, group_chunks[3] = {0, 0, 0};
         for (int g = 0; g < 3; ++g) {
             for (rr::BatchRequest& b : group[g]) {
                 b.first_chunk = uint32_t(chunks);
-                const uint64_t n = (b.ray_count + rr::kBatchChunk - 1) / rr::kBatchChunk;
+                const uint64_t n = (b.ray_count + chunk - 1) / chunk;
                 chunks += n;
                 group_chunks[g] += n;
                 RR_REQUIRE(chunks < (uint64_t(1) << 32), "rrxCmdIntersectBatch");
@@ -1234,7 +1238,7 @@ RRError rrxCmdIntersectBatch(RRContext context, const RRxIntersectRequest* reque
                 auto* dev_table =
                     reinterpret_cast<rr::BatchRequest*>(static_cast<uint8_t*>(sc) + rr::kBatchTableOffset);
                 Check(hipMemcpyAsync(dev_table, host, table_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-                rr::TraceBatch(dev_table, n_req, c_geo, c_unrec, c_scene, sc, s);
+                rr::TraceBatch(dev_table, n_req, chunk, c_geo, c_unrec, c_scene, sc, s);
             }
         };
         Record(cs, cmd);

@@ -90,11 +90,20 @@ struct BatchRequest {
     uint64_t hits;
     uint32_t ray_count;
     uint32_t flags;
-    uint32_t first_chunk;  // chunks of kBatchChunk rays before this request
+    uint32_t first_chunk;  // chunks of the batch's chunk size before this request
     uint32_t pad[3];
 };
 static_assert(sizeof(BatchRequest) == 48, "BatchRequest must be 48 bytes");
 constexpr uint32_t kBatchChunk = 64;
+// Rays per chunk of one batch: kBatchChunk, or kBatchChunkLarge when its
+// requests average at least kBatchLargeChunkRays rays (half the chunk
+// atomics and table searches for large requests; small requests keep more,
+// finer pieces of work).
+constexpr uint32_t kBatchChunkLarge = 128;
+constexpr uint64_t kBatchLargeChunkRays = 2048;
+inline uint32_t BatchChunkFor(uint64_t total_rays, uint32_t requests) {
+    return requests && total_rays >= (uint64_t)requests * kBatchLargeChunkRays ? kBatchChunkLarge : kBatchChunk;
+}
 
 // Scratch for a batch of `requests` requests and `total_rays` rays: a
 // 2 KB head (the two kernels' chunk counters, eight each), the request
@@ -114,8 +123,8 @@ constexpr uint32_t kBatchFlagCheckLevel = 8;
 // kBatchFlagCheckLevel), recorded scenes: one persistent launch per level
 // with requests present; requests may name different BVHs, queries and
 // outputs.
-void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunks, uint32_t unrecorded_chunks,
-                uint32_t scene_chunks, void* scratch, hipStream_t stream);
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t chunk, uint32_t geo_chunks,
+                uint32_t unrecorded_chunks, uint32_t scene_chunks, void* scratch, hipStream_t stream);
 
 // Same traversal with per-ray visit counters written to `stats`
 // (uint32 x4 per ray: internal nodes visited, leaves visited, box tests,

@@ -875,9 +875,9 @@ static_assert(2 * kQueues * kQueueStride * 4 <= kBatchTableOffset, "batch counte
 // calls it.
 template <bool TWO_LEVEL>
 __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ table, uint32_t requests,
-                                               uint32_t first, uint32_t count, uint32_t* counters, uint32_t lane,
-                                               uint32_t& part, uint32_t& tries, uint32_t& q_req, uint32_t& q_next,
-                                               uint32_t& q_end) {
+                                               uint32_t chunk, uint32_t first, uint32_t count, uint32_t* counters,
+                                               uint32_t lane, uint32_t& part, uint32_t& tries, uint32_t& q_req,
+                                               uint32_t& q_next, uint32_t& q_end) {
     for (;;) {
         // Part `part` holds chunks [first + p_lo, first + p_hi).
         const uint32_t p_lo = (uint32_t)((uint64_t)count * part / kQueues);
@@ -908,8 +908,8 @@ __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ 
             }
         }
         q_req = lo;
-        q_next = (c - table[lo].first_chunk) * kBatchChunk;
-        q_end = min(q_next + kBatchChunk, table[lo].ray_count);
+        q_next = (c - table[lo].first_chunk) * chunk;
+        q_end = min(q_next + chunk, table[lo].ray_count);
         return true;
     }
 }
@@ -917,7 +917,7 @@ __device__ __forceinline__ bool NextBatchChunk(const BatchRequest* __restrict__ 
 __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* __restrict__ table, uint32_t requests,
                                                            uint32_t first_chunk, uint32_t chunk_count,
                                                            uint32_t* __restrict__ head, uint32_t* __restrict__ spill,
-                                                           uint32_t min_busy, uint32_t* err) {
+                                                           uint32_t min_busy, uint32_t chunk, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
     uint32_t* counter = head + kBatchCounterWord;
     const uint32_t lane = __lane_id();
@@ -942,7 +942,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
             if (q_next == q_end)
-                drained = !NextBatchChunk<false>(table, requests, first_chunk, chunk_count, counter, lane, part, tries,
+                drained = !NextBatchChunk<false>(table, requests, chunk, first_chunk, chunk_count, counter, lane, part, tries,
                                                  q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = (uint32_t)__popcll(idle & lt);
@@ -1063,7 +1063,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatchKernel(const BatchRequest* _
 __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest* __restrict__ table, uint32_t requests,
                                                              uint32_t first_chunk, uint32_t chunk_count,
                                                              uint32_t* __restrict__ head, uint32_t* __restrict__ spill,
-                                                             uint32_t min_busy, uint32_t* err) {
+                                                             uint32_t min_busy, uint32_t chunk, uint32_t* err) {
     __shared__ uint32_t s_stack[kLdsStackEntries][kBlock];
     __shared__ float s_world[9][kBlock];  // world ray: origin, direction, reciprocal direction
     uint32_t* counter = head + kBatch2LCounterWord;
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) void TraceBatch2LKernel(const BatchRequest*
         const uint64_t idle = __ballot(rid == kInvalid);
         if (idle && !drained) {
             if (q_next == q_end)
-                drained = !NextBatchChunk<true>(table, requests, first_chunk, chunk_count, counter, lane, part, tries,
+                drained = !NextBatchChunk<true>(table, requests, chunk, first_chunk, chunk_count, counter, lane, part, tries,
                                                 q_req, q_next, q_end);
             const uint32_t take = drained ? 0u : min((uint32_t)__popcll(idle), q_end - q_next);
             const uint32_t rank = RankIn(idle);
@@ -1548,14 +1548,14 @@ void TraceWithStats(const void* accel, AccelKind kind, QueryKind query, const vo
 }
 
 size_t TraceBatchScratchBytes(uint32_t requests, uint64_t total_rays) {
-    // The launch has at most one block per 4 chunks, and the chunks number
-    // fewer than total_rays / kBatchChunk + requests.
+    // The launch has at most one lane per chunked ray, and the chunks of at
+    // most kBatchChunkLarge rays number fewer than total_rays / chunk + requests.
     const uint64_t lanes = std::min<uint64_t>((uint64_t)std::max(BatchBlocks(), Batch2LBlocks()) * kBlock,
-                                              AlignUp(total_rays + (uint64_t)requests * kBatchChunk, kBlock));
+                                              AlignUp(total_rays + (uint64_t)requests * kBatchChunkLarge, kBlock));
     return kBatchTableOffset + BatchTableBytes(requests) + AlignUp((size_t)lanes * kSpillEntries * 4, 256);
 }
 
-void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunks, uint32_t unrecorded_chunks,
+void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t chunk, uint32_t geo_chunks, uint32_t unrecorded_chunks,
                 uint32_t scene_chunks, void* scratch, hipStream_t s) {
     if (requests == 0 || geo_chunks + unrecorded_chunks + scene_chunks == 0) return;
     uint8_t* base = static_cast<uint8_t*>(scratch);
@@ -1568,12 +1568,12 @@ void TraceBatch(const BatchRequest* table, uint32_t requests, uint32_t geo_chunk
     // launched.  Grids as a single query's of the chunks' lanes.
     const uint32_t n1 = geo_chunks + unrecorded_chunks, n2 = unrecorded_chunks + scene_chunks;
     if (n1) {
-        const uint32_t b = std::min(BatchBlocks(), QueryBlocks((uint64_t)n1 * kBatchChunk, false));
-        TraceBatchKernel<<<b, kBlock, 0, s>>>(table, requests, 0u, n1, head, spill, mb, DeviceErrorWords());
+        const uint32_t b = std::min(BatchBlocks(), QueryBlocks((uint64_t)n1 * chunk, false));
+        TraceBatchKernel<<<b, kBlock, 0, s>>>(table, requests, 0u, n1, head, spill, mb, chunk, DeviceErrorWords());
     }
     if (n2) {
-        const uint32_t b = std::min(Batch2LBlocks(), QueryBlocks((uint64_t)n2 * kBatchChunk, true));
-        TraceBatch2LKernel<<<b, kBlock, 0, s>>>(table, requests, geo_chunks, n2, head, spill, MinBusyLanes2L(),
+        const uint32_t b = std::min(Batch2LBlocks(), QueryBlocks((uint64_t)n2 * chunk, true));
+        TraceBatch2LKernel<<<b, kBlock, 0, s>>>(table, requests, geo_chunks, n2, head, spill, MinBusyLanes2L(), chunk,
                                                 DeviceErrorWords());
     }
 }
