@@ -481,15 +481,31 @@ __device__ __forceinline__ void WindowTreeletRoot(const BvhNode* __restrict__ no
     }
 }
 
+// Roots are staged in LDS and appended with one device atomic per
+// workgroup (a per-wave append put thousands of atomics on one counter word,
+// and those serialise across XCDs); a block that finds more than 2 x kBlock
+// roots appends the rest one by one.  Root order does not matter: each
+// root's treelet is restructured on its own.
 __global__ __launch_bounds__(kBlock) void FindTreeletRootsKernel(const BvhNode* __restrict__ nodes, uint32_t n,
                                                                  uint32_t min_prims, uint32_t* counts, uint32_t* counter,
                                                                  uint32_t* roots) {
     __shared__ RootsLds<kBlock> lds;
+    __shared__ uint32_t s_roots[2 * kBlock];
+    __shared__ uint32_t s_nroots, s_base;
+    if (threadIdx.x == 0) s_nroots = 0;
     const uint32_t base = blockIdx.x * kBlock;
     const uint32_t i = base + threadIdx.x;
     WindowTreeletRoot<kBlock>(nodes, n, base, kBlock, 0, i < n, i + 1 < n ? i : kInvalid, n - 1 + i, min_prims,
-                              counts, counts, 0u, lds,
-                              [&](uint32_t, uint32_t root) { roots[atomicAdd(counter, 1u)] = root; });
+                              counts, counts, 0u, lds, [&](uint32_t, uint32_t root) {
+                                  const uint32_t k = atomicAdd(&s_nroots, 1u);
+                                  if (k < 2 * kBlock) s_roots[k] = root;
+                                  else roots[atomicAdd(counter, 1u)] = root;
+                              });
+    __syncthreads();
+    const uint32_t cnt = min(s_nroots, 2u * kBlock);
+    if (threadIdx.x == 0 && cnt) s_base = atomicAdd(counter, cnt);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) roots[s_base + k] = s_roots[k];
 }
 
 // One wave per treelet root.  Lane 0 does the pointer-chasing parts (forming
@@ -902,6 +918,9 @@ __global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const Batc
                                                                       uint32_t min_prims, uint32_t* roots_counter,
                                                                       uint2* roots) {
     __shared__ RootsLds<kBlock> lds;
+    __shared__ uint2 s_roots[2 * kBlock];
+    __shared__ uint32_t s_nroots, s_base;
+    if (threadIdx.x == 0) s_nroots = 0;
     constexpr uint32_t kChunksPerBlock = kBlock / kRefitChunk;
     const uint32_t cb = blockIdx.x * kChunksPerBlock;
     const uint32_t c = cb + threadIdx.x / kRefitChunk;
@@ -926,8 +945,16 @@ __global__ __launch_bounds__(kBlock) void BatchFindTreeletRootsKernel(const Batc
     WindowTreeletRoot<kBlock>(
         nodes, n, (first - bm.first_chunk) * kRefitChunk, active ? (last - first) * kRefitChunk : 0,
         (first - cb) * kRefitChunk, active, active && i + 1 < n ? i : kInvalid, n - 1 + i, min_prims, counts,
-        roots_counter, lo, lds,
-        [&](uint32_t mesh, uint32_t root) { roots[atomicAdd(roots_counter, 1u)] = make_uint2(mesh, root); });
+        roots_counter, lo, lds, [&](uint32_t mesh, uint32_t root) {
+            const uint32_t k = atomicAdd(&s_nroots, 1u);
+            if (k < 2 * kBlock) s_roots[k] = make_uint2(mesh, root);
+            else roots[atomicAdd(roots_counter, 1u)] = make_uint2(mesh, root);
+        });
+    __syncthreads();
+    const uint32_t cnt = min(s_nroots, 2u * kBlock);
+    if (threadIdx.x == 0 && cnt) s_base = atomicAdd(roots_counter, cnt);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) roots[s_base + k] = s_roots[k];
 }
 
 // Eight waves per SIMD (64 VGPRs): the batched rounds run hundreds of
